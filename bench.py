@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-substep GRMHD advance on one MI355X.
+
+Workload: the plain FP64 3D torus of 192x192x128 zones -- Kerr a = 0.9 in
+modified Kerr-Schild coordinates, PPM reconstruction, HLL fluxes, PPM-EMF
+constrained transport, optically thin cooling to H/r = 0.1, floors
+1e-7 r^-1.5 / 1e-9 r^-2.5 -- one GPU's share of the domain.  A step is one
+step_ch(): predictor and corrector advance() with boundaries, fixups and the
+CFL time step.  value = interior zones x steps / wall time of the timed steps
+(zone-cycles/s, the reference's unit, doc/CHANGES:1826), with the state
+resident in HBM.
+
+Also reported:
+  roofline      -- the dominant kernel's algorithmic HBM bytes per launch over
+                   its average launch time (HIP events on the library stream,
+                   during the timed steps), against 8 TB/s;
+  cpu_baseline  -- the reference C (oracle/_ref/c1, built from the reference
+                   sources) on the host cores: independent processes on x3
+                   slabs of the same grid, timed around step_ch() only.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "zone-cycles/s per RK2 step, 1 MI355X, mixed vs FP64; con2prim solves/s"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per unit of work of each kernel (FP64 state):
+#   flux   : per face   -- 8 primitives of the zone read once (64 B) + 9 face values written (72 B)
+#   emf    : per edge   -- 6 B-field face fluxes read (48 B) + 3 EMFs written (24 B)
+#   update : per zone   -- p read (64 B), 27 face values + 3 EMFs read (240 B), result written (64 B),
+#                          gamma written (8 B); the corrector also reads ph (64 B): 376 / 440 B
+ALGO_BYTES = {"flux": 136.0, "emf": 72.0, "update": 0.5 * (376.0 + 440.0)}
+
+
+def units_per_launch(n1, n2, n3):
+    ng = 3
+    n1t, n2t, n3t = n1 + 2 * ng, n2 + 2 * ng, n3 + 2 * ng
+    faces = ((n1 + 1) * n2t * n3t + n1t * (n2 + 1) * n3t + n1t * n2t * (n3 + 1)) / 3.0
+    return {"flux": faces, "emf": (n1 + 1) * (n2 + 1) * (n3 + 1), "update": n1 * n2 * n3}
+
+
+def cpu_worker(args):
+    """One reference process on an x3 slab: prints zone-cycles and seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from refwrap import RefRun, torus_params
+    from coolharm3d_amd.problems import torus_grid, torus
+    n1, n2, n3 = args.n1, args.n2, args.slab
+    g = torus_grid(n1, n2, n3, x3_extent=0.5 * math.pi * n3 / args.n3)
+    R = RefRun("c1", n1, n2, n3, params=torus_params(g))
+    R.set_prims(torus(g, seed=1234 + args.worker))
+    R.fixup_bounds()
+    R.set_time(0.0, 1.0e-5, g.cour)
+    R.step(1)  # first step also builds the reference's cooling tables
+    t0 = time.perf_counter()
+    R.step(args.slab_steps)
+    el = time.perf_counter() - t0
+    import resource
+    rss = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1024.0
+    print(json.dumps({"zc": n1 * n2 * n3 * args.slab_steps, "s": el, "rss_mb": rss}), flush=True)
+
+
+def cpu_baseline(args):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from refwrap import have_ref
+    if not have_ref("c1"):
+        return {"value": None, "unit": "zone-cycles/s", "cores": 0, "kind": "reference",
+                "sample": "oracle/_ref/c1 not built on this machine"}
+    ncores = min(16, os.cpu_count() or 1)
+    slab = 4
+    procs = []
+    for w in range(ncores):
+        cmd = [sys.executable, os.path.abspath(__file__), "--_cpu-worker", "--worker", str(w),
+               "--n1", str(args.n1), "--n2", str(args.n2), "--n3", str(args.n3),
+               "--slab", str(slab), "--slab-steps", str(args.cpu_steps)]
+        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                                      env=dict(os.environ, OMP_NUM_THREADS="1")))
+    rate, n_ok = 0.0, 0
+    for p in procs:
+        out, _ = p.communicate()
+        for line in out.splitlines():
+            if line.startswith("{"):
+                r = json.loads(line)
+                rate += r["zc"] / r["s"]
+                n_ok += 1
+    return {"value": rate if n_ok else None, "unit": "zone-cycles/s", "cores": n_ok, "kind": "reference",
+            "sample": (f"reference step_ch() (gcc -O2), {n_ok} independent processes, each a "
+                       f"{args.n1}x{args.n2}x{slab} x3-slab of the {args.n1}x{args.n2}x{args.n3} torus, "
+                       f"{args.cpu_steps} timed steps after 1 untimed; rates summed (no halo exchange: "
+                       f"an upper bound on an MPI run)")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n1", type=int, default=192)
+    ap.add_argument("--n2", type=int, default=192)
+    ap.add_argument("--n3", type=int, default=128)
+    ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--_cpu-worker", action="store_true", dest="cpu_worker")
+    ap.add_argument("--worker", type=int, default=0)
+    ap.add_argument("--slab", type=int, default=4)
+    ap.add_argument("--slab-steps", type=int, default=4)
+    args = ap.parse_args()
+
+    if args.cpu_worker:
+        cpu_worker(args)
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU halo exchange is a design note, not part of this build)", file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+
+    import numpy as np
+    from coolharm3d_amd import Harm3D, torus_grid, torus
+
+    n1, n2, n3 = args.n1, args.n2, args.n3
+    g = torus_grid(n1, n2, n3)
+    H = Harm3D(g, track_nfail=1)
+    H.set_prims(torus(g))
+    H.fixup_bounds()
+    H.set_time(0.0, 1.0e-5)
+    H.step_ch(args.warmup)
+    H.sync()
+
+    H.profile_enable(True)
+    H.sync()
+    t0 = time.perf_counter()
+    H.step_ch(args.steps)
+    H.sync()
+    el = time.perf_counter() - t0
+    ncell = n1 * n2 * n3
+    value = ncell * args.steps / el
+
+    kern = {}
+    for nm in ("flux", "emf", "update", "bounds", "fixup", "step"):
+        ms, cnt = H.profile_read(nm)
+        kern[nm] = (ms, cnt)
+    H.profile_enable(False)
+    p = H.prims()
+    assert np.isfinite(p).all(), "non-finite state after the timed steps"
+
+    units = units_per_launch(n1, n2, n3)
+    dom = max(("flux", "emf", "update"), key=lambda k: kern[k][0])
+    avg_s = kern[dom][0] / max(kern[dom][1], 1) / 1e3
+    achieved = ALGO_BYTES[dom] * units[dom] / avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "avg_launch_ms": avg_s * 1e3, "algo_bytes_per_launch": ALGO_BYTES[dom] * units[dom],
+                "kernel_ms_per_step": {k: kern[k][0] / args.steps for k in ("flux", "emf", "update", "bounds", "fixup")},
+                "state_roofline_frac": value * 320.0 / (HBM_PEAK_GBS * 1e9)}
+
+    cpu = None
+    if rank == 0 and args.cpu_baseline == "auto":
+        cpu = cpu_baseline(args)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "zone-cycles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Fishbone-Moncrief torus, seeded +-2% u perturbation)",
+        "config": {"workload": f"torus_{n1}x{n2}x{n3}_fp64", "grid": [n1, n2, n3], "spin": 0.9,
+                   "recon": "PPM+MC", "flux": "HLL", "ct": "PPM EMFs", "cooling": "h/r=0.1",
+                   "precision": "fp64 (mixed-precision variant not built)"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if rank == 0:
+        print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

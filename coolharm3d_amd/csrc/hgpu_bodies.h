@@ -1,0 +1,793 @@
+// hgpu_bodies.h -- per-thread bodies of the substep kernels.
+//
+// Each body handles one face, edge or zone and is called by a thin __global__
+// wrapper (hgpu_kernels.hip).  The same bodies are compiled for the host by the
+// check build (hgpu_hostcheck.cpp), which runs them in loops so the kernels'
+// arithmetic can be compared with the reference on a machine without a GPU.
+#pragma once
+#include "hgpu_phys.h"
+
+namespace hgpu {
+
+// --------------------------------------------------------------------------
+// small helpers
+
+HD double dmax(double a, double b) { return (a > b) ? a : b; }   // MAX(), src/macros.h:474
+
+HD unsigned long long dbl_bits(double x) {
+  union { double d; unsigned long long u; } c;
+  c.d = x;
+  return c.u;
+}
+HD double bits_dbl(unsigned long long u) {
+  union { double d; unsigned long long u; } c;
+  c.u = u;
+  return c.d;
+}
+
+// Index of the x1 rescaling factor per variable: rho -> r^1.5, u -> r, rest -> r^3
+// (rscale_prims_r(), src/recon.c:1173-1189).
+HD int rs_slot(int v) { return (v == RHO) ? 0 : ((v == UU) ? 2 : 1); }
+
+// Value of a primitive after the in-place x1 rescale/unscale round trip that
+// reconstruct_fast() applies to every zone (src/recon.c:996, :1008, :1173-1210).
+HD double rt_val(const Params& P, const Ptrs& X, int v, int i, double x) {
+  if (!P.rescale_r) return x;
+  int s = rs_slot(v);
+  return (x * X.rscale[s*P.n1t + i]) * X.irscale[s*P.n1t + i];
+}
+
+// Tally of fail() calls of one zone; written out by the kernel wrapper.
+struct FailRec {
+  int nf[N_NFAIL];
+  int pflag;        // last pflag response (0: none)
+  int boundary;     // check_boundary_pflag() hit a physical boundary
+  HD void clear() {
+    for (int l = 0; l < N_NFAIL; l++) nf[l] = 0;
+    pflag = 0;
+    boundary = 0;
+  }
+};
+
+// fail(type, state) bookkeeping, src/diag.c:159-300: state 1 -> INTERP_PRIM,
+// state 2 -> INTERP_V; check_boundary_pflag(), src/harm_mpi.c:1388-1430.
+HD void fail_set(const Params& P, FailRec& r, int nfail_id, int state, int i, int j, int k) {
+  r.nf[nfail_id]++;
+  if (state) {
+    r.pflag = (state == 1) ? PFLAG_INTERP_PRIM : PFLAG_INTERP_V;
+    int n1e = P.n1 + NG - 1, n2e = P.n2 + NG - 1, n3e = P.n3 + NG - 1;
+    if (i < NG + NG || i > n1e - NG || j < NG + NG || j > n2e - NG || k < NG + NG || k > n3e - NG)
+      r.boundary = 1;
+  }
+}
+
+HD const Geom* geom_at(const Params& P, const Ptrs& X, int i, int j, int k, int pos) {
+  return X.geom + geom_id(P, i, j, k, pos);
+}
+
+// --------------------------------------------------------------------------
+// numerical_flux() for one face, src/step_ch.c:636-790 (with the face states of
+// reconstruct_fast(), src/recon.c:894-1103).  src is the state being
+// reconstructed (p in the predictor, ph in the corrector).
+HD void flux_face(const Params& P, const Ptrs& X, const double* __restrict__ src, int d,
+                  int i, int j, int k, FailRec& fr) {
+  const long long NT = P.nt;
+  double pl[NP], pr[NP];
+  const int di = (d == 0), dj = (d == 1), dk = (d == 2);
+
+  if (P.degen[d]) {
+    // degenerate direction: p_L = p_R = p_h of the zone at the direction's
+    // first index; degenerate directions after d are clamped by the later
+    // copy passes, earlier ones are not (src/recon.c:1050-1075)
+    int ii = i, jj = j, kk = k;
+    if (d == 0) { ii = NG; if (P.degen[1]) jj = NG; if (P.degen[2]) kk = NG; }
+    if (d == 1) { jj = NG; if (P.degen[2]) kk = NG; }
+    if (d == 2) { kk = NG; }
+    long long c = cidx(P, ii, jj, kk);
+    for (int v = 0; v < NP; v++) pl[v] = pr[v] = rt_val(P, X, v, ii, src[v*NT + c]);
+  } else {
+    int ii = i, jj = j, kk = k;
+    if (P.degen[0]) ii = NG;
+    if (P.degen[1]) jj = NG;
+    if (P.degen[2]) kk = NG;
+    if (P.recon == RECON_PPM_MC) {
+      long long c0 = cidx(P, ii - 3*di, jj - 3*dj, kk - 3*dk);
+      long long st = cidx(P, di, dj, dk) - cidx(P, 0, 0, 0);
+      for (int v = 0; v < NP; v++) {
+        double y[6];
+        for (int m = 0; m < 6; m++) {
+          double x = src[v*NT + c0 + m*st];
+          if (d == 0) {
+            if (P.rescale_r) x *= X.rscale[rs_slot(v)*P.n1t + (ii - 3 + m)];
+          } else {
+            x = rt_val(P, X, v, ii, x);
+          }
+          y[m] = x;
+        }
+        ppm_interface(y, &pl[v], &pr[v]);
+        if (d == 0 && P.rescale_r) {
+          double f = X.irscale[(3 + rs_slot(v))*P.n1t + i];
+          pl[v] *= f;
+          pr[v] *= f;
+        }
+      }
+    } else {
+      // MC slopes, src/recon.c:960-986
+      long long c0 = cidx(P, ii - 2*di, jj - 2*dj, kk - 2*dk);
+      long long st = cidx(P, di, dj, dk) - cidx(P, 0, 0, 0);
+      for (int v = 0; v < NP; v++) {
+        double y[4];
+        for (int m = 0; m < 4; m++) {
+          double x = src[v*NT + c0 + m*st];
+          if (d == 0) {
+            if (P.rescale_r) x *= X.rscale[rs_slot(v)*P.n1t + (ii - 2 + m)];
+          } else {
+            x = rt_val(P, X, v, ii, x);
+          }
+          y[m] = x;
+        }
+        double dql = 0.5*mc_slope(y[0], y[1], y[2]);
+        double dqr = 0.5*mc_slope(y[1], y[2], y[3]);
+        pl[v] = y[1] + dql;
+        pr[v] = y[2] - dqr;
+        if (d == 0 && P.rescale_r) {
+          double f = X.irscale[(3 + rs_slot(v))*P.n1t + i];
+          pl[v] *= f;
+          pr[v] *= f;
+        }
+      }
+    }
+  }
+
+  const Geom* g = geom_at(P, X, i, j, k, d);
+  const int dim = d + 1;
+  State ql, qr;
+  if (get_state(pl, g, P.gam, &ql)) fr.nf[NFAIL_GAMMA_CALC]++;
+  if (get_state(pr, g, P.gam, &qr)) fr.nf[NFAIL_GAMMA_CALC]++;
+  double Ul[NP], Ur[NP], Fl[NP], Fr[NP];
+  primtoflux(pl, &ql, 0, g, Ul);
+  primtoflux(pr, &qr, 0, g, Ur);
+  primtoflux(pl, &ql, dim, g, Fl);
+  primtoflux(pr, &qr, dim, g, Fr);
+  double cmaxl, cminl, cmaxr, cminr;
+  vchar_fast(&ql, g, dim, P.gam, &cmaxl, &cminl);
+  vchar_fast(&qr, g, dim, P.gam, &cmaxr, &cminr);
+  double cmax = fabs(dmax(dmax(0., cmaxl), cmaxr));
+  double cmin = fabs(dmax(dmax(0., -cminl), -cminr));
+  double ctop = dmax(cmin, cmax);
+
+  // fastest speed over interior faces (timestep(), src/step_ch.c:716-724)
+  if (i >= NG && i < NG + P.n1 && j >= NG && j < NG + P.n2 && k >= NG && k < NG + P.n3 && ctop > 0.)
+    atomic_max_u64(&X.status[ST_VMAX + dim], dbl_bits(ctop));
+
+  const long long c = cidx(P, i, j, k);
+  double* F = X.F;
+  if (P.flux == FLUX_HLLE) {
+    for (int l = 0; l < NP; l++)
+      F[(d*NFLUX + l)*NT + c] = (cmax*Fl[l] + cmin*Fr[l] - cmax*cmin*(Ur[l] - Ul[l])) / (cmax + cmin + SMALL);
+  } else {
+    for (int l = 0; l < NP; l++)
+      F[(d*NFLUX + l)*NT + c] = 0.5*(Fl[l] + Fr[l] - ctop*(Ur[l] - Ul[l]));
+  }
+
+  // entropy flux of this face for fixup_entropy_eq() (src/fixup.c:363-398),
+  // from the face data the reference stores in c_gf/ucon_L/ucon_R (src/step_ch.c:743-750)
+  {
+    double uL0 = g->g * ql.ucon[0], uL1 = g->g * ql.ucon[dim];
+    double uR0 = g->g * qr.ucon[0], uR1 = g->g * qr.ucon[dim];
+    double s = (P.gam - 1.) * pl[UU] * pow(pl[RHO], 1. - P.gam);
+    double Sl = s * uL0, Gl = s * uL1;
+    s = (P.gam - 1.) * pr[UU] * pow(pr[RHO], 1. - P.gam);
+    double Sr = s * uR0, Gr = s * uR1;
+    double Fs;
+    if (P.flux == FLUX_HLLE) Fs = (cmax*Gl + cmin*Gr - cmax*cmin*(Sr - Sl)) / (cmax + cmin + SMALL);
+    else Fs = 0.5*(Gl + Gr - ctop*(Sr - Sl));
+    F[(d*NFLUX + 8)*NT + c] = Fs;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Corner EMFs, flux_ct_para_fast2() (src/flux_ct.c:759-846) or flux_ct()
+// (src/flux_ct.c:26-41) for the edge (i,j,k).
+HD double ppm_line_pair(const double* __restrict__ base, long long c_center, long long stride) {
+  // returns (right edge of zone c-1) + (left edge of zone c) accumulated onto 0
+  double y[6];
+  for (int m = 0; m < 6; m++) y[m] = base[c_center + (m - 3)*stride];
+  double yl, yr;
+  ppm_interface(y, &yl, &yr);
+  double e = 0.0 + yl;
+  return e + yr;
+}
+
+HD void emf_edge(const Params& P, const Ptrs& X, int i, int j, int k) {
+  const long long NT = P.nt;
+  const long long c = cidx(P, i, j, k);
+  const long long si = (long long)P.n2t * P.n3t, sj = P.n3t, sk = 1;
+  const double* F = X.F;
+  const double* F0B2 = F + (0*NFLUX + B2)*NT;
+  const double* F0B3 = F + (0*NFLUX + B3)*NT;
+  const double* F1B1 = F + (1*NFLUX + B1)*NT;
+  const double* F1B3 = F + (1*NFLUX + B3)*NT;
+  const double* F2B1 = F + (2*NFLUX + B1)*NT;
+  const double* F2B2 = F + (2*NFLUX + B2)*NT;
+  double e0, e1, e2;
+  if (P.ct_para) {
+    double y[6], yl, yr;
+    // emf0: + PPM of F1[B3] along x3, - PPM of F2[B2] along x2
+    e0 = ppm_line_pair(F1B3, c, sk);
+    for (int m = 0; m < 6; m++) y[m] = F2B2[c + (m - 3)*sj];
+    ppm_interface(y, &yl, &yr);
+    e0 = (e0 - yl) - yr;
+    // emf1: + PPM of F2[B1] along x1, - PPM of F0[B3] along x3
+    e1 = ppm_line_pair(F2B1, c, si);
+    for (int m = 0; m < 6; m++) y[m] = F0B3[c + (m - 3)*sk];
+    ppm_interface(y, &yl, &yr);
+    e1 = (e1 - yl) - yr;
+    // emf2: + PPM of F0[B2] along x2, - PPM of F1[B1] along x1
+    e2 = ppm_line_pair(F0B2, c, sj);
+    for (int m = 0; m < 6; m++) y[m] = F1B1[c + (m - 3)*si];
+    ppm_interface(y, &yl, &yr);
+    e2 = (e2 - yl) - yr;
+  } else {
+    e0 = 0.25*(F1B3[c] + F1B3[c - sk] - F2B2[c] - F2B2[c - sj]);
+    e1 = 0.25*(F2B1[c] + F2B1[c - si] - F0B3[c] - F0B3[c - sk]);
+    e2 = 0.25*(F0B2[c] + F0B2[c - sj] - F1B1[c] - F1B1[c - si]);
+  }
+  X.emf[0*NT + c] = e0;
+  X.emf[1*NT + c] = e1;
+  X.emf[2*NT + c] = e2;
+}
+
+// Face flux of variable l through face (i,j,k) in direction d as the update
+// reads it: B components rebuilt from the corner EMFs (src/flux_ct.c:853-933),
+// the x1 mass flux limited by fix_flux() (src/bounds.c:2255-2270).
+HD double face_flux(const Params& P, const Ptrs& X, int d, int l, int i, int j, int k) {
+  const long long NT = P.nt;
+  const long long c = cidx(P, i, j, k);
+  if (l >= B1) {
+    const double cf = P.ct_para ? 0.125 : 0.5;
+    const long long si = (long long)P.n2t * P.n3t, sj = P.n3t, sk = 1;
+    const double* E0 = X.emf;
+    const double* E1 = X.emf + NT;
+    const double* E2 = X.emf + 2*NT;
+    if (d == 0) {
+      if (l == B1) return 0.;
+      if (l == B2) return 0. + cf*(E2[c] + E2[c + sj]);
+      return 0. + (-cf)*(E1[c] + E1[c + sk]);
+    } else if (d == 1) {
+      if (l == B1) return 0. + (-cf)*(E2[c] + E2[c + si]);
+      if (l == B2) return 0.;
+      return 0. + cf*(E0[c] + E0[c + sk]);
+    } else {
+      if (l == B1) return 0. + cf*(E1[c] + E1[c + si]);
+      if (l == B2) return 0. + (-cf)*(E0[c] + E0[c + sj]);
+      return 0.;
+    }
+  }
+  double f = X.F[(d*NFLUX + l)*NT + c];
+  if (l == RHO && d == 0 && P.bc == BC_SPHERICAL_OUTFLOW3) {
+    if (i == NG && f > 0) f = 0.;
+    if (i == NG + P.n1 && f < 0) f = 0.;
+  }
+  return f;
+}
+
+// --------------------------------------------------------------------------
+// source(), src/phys.c:121-187, with cooling_func_hr_disk(), src/phys.c:649-784
+HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, int i, int j, int k,
+              double* dU) {
+  for (int l = 0; l < NP; l++) dU[l] = 0.;
+  int gfail = 0;
+  State q;
+  bool have_q = false;
+  if (P.metric_dim > 0) {
+    gfail += get_state(ph, g, P.gam, &q);
+    have_q = true;
+    if (!P.flat) {
+      double mhd[4][4];
+      mhd_calc(&q, 0, mhd[0]); mhd_calc(&q, 1, mhd[1]);
+      mhd_calc(&q, 2, mhd[2]); mhd_calc(&q, 3, mhd[3]);
+      const double* cn = X.conn + (long long)conn_id(P, i, j, k) * 64;
+      for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) {
+        dU[U1] += mhd[a][b]*cn[(b*4 + 1)*4 + a];
+        dU[U2] += mhd[a][b]*cn[(b*4 + 2)*4 + a];
+        if (P.metric_dim > 2) {
+          dU[UU] += mhd[a][b]*cn[(b*4 + 0)*4 + a];
+          dU[U3] += mhd[a][b]*cn[(b*4 + 3)*4 + a];
+        }
+      }
+    }
+  }
+  if (P.cooling) {
+    if (!have_q) gfail += get_state(ph, g, P.gam, &q);
+    int bound = ((q.ucov[0]*(q.p + ph[UU] + ph[RHO] + q.bsq)) > (-ph[RHO])) ? 1 : 0;
+    double f_cool = 0.;
+    if (bound) {
+      double T_r = X.tr[i];
+      double Omega_r = X.omega[i];
+      double E_ratio = (P.gam - 1.)*ph[UU] / (ph[RHO] * T_r);
+      double dE = E_ratio - 1.;
+      double base = dE + fabs(dE);
+      double pw = (P.q_exp == 0.5) ? sqrt(base) : pow(base, P.q_exp);
+      f_cool = P.s_norm * ph[UU] * Omega_r * pw;
+    }
+    dU[UU] -= f_cool * q.ucov[0];
+    dU[U1] -= f_cool * q.ucov[1];
+    dU[U2] -= f_cool * q.ucov[2];
+    dU[U3] -= f_cool * q.ucov[3];
+  }
+  dU[UU] *= g->g;
+  dU[U1] *= g->g;
+  dU[U2] *= g->g;
+  dU[U3] *= g->g;
+  return gfail;
+}
+
+// --------------------------------------------------------------------------
+// fixup_entropy_eq(), src/fixup.c:335-454
+HD int fixup_entropy_eq(const Params& P, const Ptrs& X, int i, int j, int k, double Dt,
+                        const double* prim_old, double* prim, const Geom* g, const double* U,
+                        double* gamma_out, FailRec& fr) {
+  const long long NT = P.nt;
+  double prim_backup[NP];
+  for (int l = 0; l < NP; l++) prim_backup[l] = prim[l];
+  double gamma;
+  gamma_calc(prim_old, g, &gamma);
+  double S_old = g->g * (P.gam - 1.)*prim_old[UU]*gamma / (g->alpha * pow(prim_old[RHO], (P.gam - 1.)));
+  const long long c = cidx(P, i, j, k);
+  const long long si = (long long)P.n2t * P.n3t, sj = P.n3t, sk = 1;
+  const double Fm0 = X.F[(0*NFLUX + 8)*NT + c], Fp0 = X.F[(0*NFLUX + 8)*NT + c + si];
+  const double Fm1 = X.F[(1*NFLUX + 8)*NT + c], Fp1 = X.F[(1*NFLUX + 8)*NT + c + sj];
+  const double Fm2 = X.F[(2*NFLUX + 8)*NT + c], Fp2 = X.F[(2*NFLUX + 8)*NT + c + sk];
+  double S = S_old + Dt*(
+      - (Fp0 - Fm0) * P.invdx[1]
+      - (Fp1 - Fm1) * P.invdx[2]
+      - (Fp2 - Fm2) * P.invdx[3]);
+  int ret = utoprim_1d_ee(U, S, g, P.gam, prim, &gamma);
+  if ((!ret) && (check_gamma(gamma) == 2)) {
+    for (int l = 0; l < NP; l++) prim[l] = prim_backup[l];
+    ret = 1;
+    gamma = 1.;
+  }
+  if (ret) {
+    ret = utoprim_1d_ee2(U, S, g, P.gam, prim, &gamma);
+    if ((!ret) && (check_gamma(gamma) == 2)) {
+      for (int l = 0; l < NP; l++) prim[l] = prim_backup[l];
+      ret = 1;
+    }
+    if (ret) gamma = 1.;
+  }
+  *gamma_out = gamma;
+  if (!ret) fail_set(P, fr, NFAIL_USE_EE, 0, i, j, k);
+  return ret;
+}
+
+// --------------------------------------------------------------------------
+// recover_primitives(), src/step_ch.c:801-1024 (FIXUP_TREE 3, entropy fix on)
+HD void recover_primitives(const Params& P, const Ptrs& X, int i, int j, int k, double Dt,
+                           double* pf, const double* U, const double* p_old, const Geom* g,
+                           double rhoflr, double uuflr, double* p_gamma, FailRec& fr) {
+  double pg[NP], pf2[NP];
+  for (int l = 0; l < NP; l++) pg[l] = pf[l];
+  double gamma = 1., bsq = 0.;
+  int ret = utoprim_2d_fast(U, g, P.gam, P.gam_m1_o_gam, pf, &gamma, &bsq, nullptr);
+  if ((!ret) && (check_gamma(gamma) == 2)) {
+    for (int l = 0; l < B1; l++) pf[l] = pg[l];
+    ret = 1;
+    gamma = 1.;
+  }
+  if (ret) {
+    ret = utoprim_1d(U, g, P.gam, pf, &gamma, &bsq);
+    if ((!ret) && (check_gamma(gamma) == 2)) {
+      for (int l = 0; l < B1; l++) pf[l] = pg[l];
+      ret = 1;
+    }
+    if (ret) gamma = 1.;
+  }
+  *p_gamma = gamma;
+
+  int use_ee = 0;
+  if (ret || (pf[UU] < BETA_MIN * bsq) || (pf[RHO] < BETA_MIN * bsq)) use_ee = 1;
+
+  if (!use_ee) {
+    if (!ret) fail_set(P, fr, NFAIL_USE_FULL_INV, 0, i, j, k);
+    else fail_set(P, fr, NFAIL_UTOPRIM, 1, i, j, k);
+  } else {
+    if (!ret) { for (int l = 0; l < NP; l++) pf2[l] = pf[l]; }
+    else fail_set(P, fr, NFAIL_UTOPRIM, 0, i, j, k);
+    for (int l = 0; l < B1; l++) pf[l] = pg[l];
+    int ret_ee = fixup_entropy_eq(P, X, i, j, k, Dt, p_old, pf, g, U, &gamma, fr);
+    if (ret_ee) {
+      if (ret) {
+        fail_set(P, fr, NFAIL_UTOPRIM_EE, 1, i, j, k);
+        return;
+      } else {
+        fail_set(P, fr, NFAIL_UTOPRIM_EE, 0, i, j, k);
+        for (int l = 0; l < NP; l++) pf[l] = pf2[l];
+        gamma = *p_gamma;
+        fail_set(P, fr, NFAIL_USE_FULL_INV, 0, i, j, k);
+      }
+    }
+  }
+  *p_gamma = gamma;
+
+  // fixup_floor(), src/fixup.c:110-121
+  int fl = 0;
+  if (pf[RHO] < rhoflr) { pf[RHO] = rhoflr; fl = 1; }
+  if (pf[UU] < uuflr) { pf[UU] = uuflr; fl = 1; }
+  if (fl) fail_set(P, fr, NFAIL_FLOOR, 0, i, j, k);
+
+  if (check_gamma(gamma)) {
+    fail_set(P, fr, NFAIL_GAMMA_MAX, 2, i, j, k);
+    return;
+  }
+  *p_gamma = gamma;
+}
+
+HD double floor_rho(const Params& P, const Ptrs& X, int i, long long c) {
+  return (P.floor_dim == 1) ? X.rhoflr[i] : X.rhoflr[c];
+}
+HD double floor_uu(const Params& P, const Ptrs& X, int i, long long c) {
+  return (P.floor_dim == 1) ? X.uuflr[i] : X.uuflr[c];
+}
+
+// --------------------------------------------------------------------------
+// Per-zone part of advance(), src/step_ch.c:312-443: conserved update and
+// recovery of the primitives for interior zone (i,j,k).
+//   stage 0 (predictor): p_i = p_h = p, result into ph, guess = p before the
+//                        rescale round trip (the reference's ph copy)
+//   stage 1 (corrector): p_h = ph, U^n recomputed from p, guess = p, result into p
+HD void update_zone(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
+                    FailRec& fr) {
+  const long long NT = P.nt;
+  const long long c = cidx(P, i, j, k);
+  const double Dt = (stage == 0) ? 0.5*dt : dt;
+  const Geom* g = geom_at(P, X, i, j, k, CENT);
+
+  double p_raw[NP], pi[NP], phz[NP];
+  for (int v = 0; v < NP; v++) {
+    p_raw[v] = X.p[v*NT + c];
+    pi[v] = rt_val(P, X, v, i, p_raw[v]);
+  }
+  // U^n = primtoflux(p_i) (src/step_ch.c:339-341)
+  State q;
+  int gf = get_state(pi, g, P.gam, &q);
+  if (stage == 0 && gf) fr.nf[NFAIL_GAMMA_CALC]++;
+  double U[NP];
+  primtoflux(pi, &q, 0, g, U);
+
+  if (stage == 0) { for (int v = 0; v < NP; v++) phz[v] = pi[v]; }
+  else { for (int v = 0; v < NP; v++) phz[v] = rt_val(P, X, v, i, X.ph[v*NT + c]); }
+
+  double dU[NP];
+  fr.nf[NFAIL_GAMMA_CALC] += source(P, X, phz, g, i, j, k, dU);
+
+  for (int l = 0; l < NP; l++) {
+    double f0m = face_flux(P, X, 0, l, i, j, k), f0p = face_flux(P, X, 0, l, i + 1, j, k);
+    double f1m = face_flux(P, X, 1, l, i, j, k), f1p = face_flux(P, X, 1, l, i, j + 1, k);
+    double f2m = face_flux(P, X, 2, l, i, j, k), f2p = face_flux(P, X, 2, l, i, j, k + 1);
+    U[l] += Dt*(
+        - (f0p - f0m) * P.invdx[1]
+        - (f1p - f1m) * P.invdx[2]
+        - (f2p - f2m) * P.invdx[3]
+        + dU[l]);
+  }
+
+  double pf[NP];
+  if (stage == 0) { for (int v = 0; v < NP; v++) pf[v] = p_raw[v]; }
+  else { for (int v = 0; v < NP; v++) pf[v] = pi[v]; }
+
+  double pgam = 1.;
+  recover_primitives(P, X, i, j, k, Dt, pf, U, p_raw, g, floor_rho(P, X, i, c), floor_uu(P, X, i, c),
+                     &pgam, fr);
+
+  double* dst = (stage == 0) ? X.ph : X.p;
+  for (int v = 0; v < NP; v++) dst[v*NT + c] = pf[v];
+  X.gamma[c] = pgam;
+  if (fr.pflag) X.pflag[c] = fr.pflag;
+}
+
+// --------------------------------------------------------------------------
+// Boundary conditions, src/bounds.c:31-579 (BC_CARTESIAN_OUTFLOW :58-133,
+// BC_SPHERICAL_OUTFLOW3 :492-579) and inflow_check(), src/bounds.c:2304-2380.
+
+HD void inflow_check(const Params& P, const Ptrs& X, double* pr, int ii, int jj, int kk, int type,
+                     FailRec& fr) {
+  const int dir = 1;
+  const Geom* g = geom_at(P, X, ii, jj, kk, CENT);
+  double gamma;
+  if (gamma_calc(pr, g, &gamma)) fr.nf[NFAIL_GAMMA_CALC]++;
+  double ucon[4];
+  ucon[0] = gamma*g->ncon[0];
+  ucon[1] = pr[U1] - ucon[0]*g->beta[0];
+  ucon[2] = pr[U2] - ucon[0]*g->beta[1];
+  ucon[3] = pr[U3] - ucon[0]*g->beta[2];
+  if (((ucon[dir] > 0.) && (type == 0)) || ((ucon[dir] < 0.) && (type == 1))) {
+    gamma = g->alpha * ucon[0];
+    pr[U1] /= gamma;
+    pr[U2] /= gamma;
+    pr[U3] /= gamma;
+    pr[U1 + dir - 1] = g->beta[dir - 1]*g->ncon[0];
+    double vsq = 0.;
+    for (int a = 1; a < 4; a++) for (int b = 1; b < 4; b++)
+      vsq += g->gcov[a][b]*pr[U1 + a - 1]*pr[U1 + b - 1];
+    if (fabs(vsq) < 1.e-13) vsq = fabs(vsq);
+    if (vsq < 0.) vsq = 0.;
+    if (vsq >= 1.) vsq = 1. - 1./(GAMMAMAX*GAMMAMAX);
+    gamma = 1./sqrt(1. - vsq);
+    pr[U1] *= gamma;
+    pr[U2] *= gamma;
+    pr[U3] *= gamma;
+  }
+}
+
+// One ghost zone of direction dir.  t enumerates (g, transverse...) for the
+// lower (side 0) and upper (side 1) faces.
+HD void bound_zone(const Params& P, const Ptrs& X, double* S, int dir, int side, int g, int a, int b,
+                   int do_pflag, FailRec& fr) {
+  const long long NT = P.nt;
+  const int n1e = P.n1 + NG - 1, n2e = P.n2 + NG - 1, n3e = P.n3 + NG - 1;
+  int di, dj, dk, si, sj, sk;
+  bool reflect = false;
+  if (dir == 0) {
+    // a = j in [N2S,N2E], b = k in [N3S,N3E]
+    dj = sj = a; dk = sk = b;
+    di = side ? (n1e + 1 + g) : (NG - 1 - g);
+    si = side ? n1e : NG;
+  } else if (dir == 1) {
+    // a = i (all), b = k in [N3S,N3E]
+    di = si = a; dk = sk = b;
+    dj = side ? (n2e + 1 + g) : (NG - 1 - g);
+    if (P.bc == BC_SPHERICAL_OUTFLOW3) { sj = side ? (n2e - g) : (NG + g); reflect = true; }
+    else sj = side ? n2e : NG;
+  } else {
+    // a = i (all), b = j (all)
+    di = si = a; dj = sj = b;
+    dk = side ? (n3e + 1 + g) : (NG - 1 - g);
+    if (P.bc == BC_SPHERICAL_OUTFLOW3) sk = side ? (NG + g) : (n3e - g);
+    else sk = side ? n3e : NG;
+  }
+  const long long cd = cidx(P, di, dj, dk), cs = cidx(P, si, sj, sk);
+  double pr[NP];
+  for (int v = 0; v < NP; v++) pr[v] = S[v*NT + cs];
+  if (reflect) { pr[U2] *= -1; pr[B2] *= -1; }
+  if (dir == 0 && P.bc == BC_SPHERICAL_OUTFLOW3) inflow_check(P, X, pr, di, dj, dk, side, fr);
+  for (int v = 0; v < NP; v++) S[v*NT + cd] = pr[v];
+  if (do_pflag) X.pflag[cd] = X.pflag[cs];
+}
+
+// --------------------------------------------------------------------------
+// fixup_gamma(), src/fixup.c:148-158
+HD void fixup_gamma(double* pv, double* gamma, FailRec& fr) {
+  double f = sqrt((GAMMAMAX - 1.)*(GAMMAMAX + 1.) / ((*gamma - 1.)*(*gamma + 1.)));
+  pv[U1] *= f;
+  pv[U2] *= f;
+  pv[U3] *= f;
+  *gamma = GAMMAMAX;
+  fr.nf[NFAIL_GAMMA_MAX]++;
+}
+
+// fixup_interp_v() (which == PFLAG_INTERP_V, src/fixup.c:659-761) and
+// fixup_interp_prim() (which == PFLAG_INTERP_PRIM, src/fixup.c:499-610).
+HD void fixup_interp_zone(const Params& P, const Ptrs& X, double* pv, int which, int i, int j, int k,
+                          FailRec& fr) {
+  const long long NT = P.nt;
+  const long long c = cidx(P, i, j, k);
+  if (X.pflag[c] != which) return;
+  const int o = 13;
+  const int ngood_min[4] = {13, 9, 6, 4};
+  int pfl[27];
+  for (int ii = -1; ii <= 1; ii++) for (int jj = -1; jj <= 1; jj++) for (int kk = -1; kk <= 1; kk++) {
+    int ind1 = ii + 3*(jj + 3*kk);
+    pfl[o + ind1] = !X.pflag[cidx(P, i + ii, j + jj, k + kk)];
+  }
+  int ngood = 0;
+  for (int ii = 1; ii <= 13; ii++) {
+    if (pfl[o + ii] && pfl[o - ii]) ngood++;
+    else pfl[o + ii] = pfl[o - ii] = 0;
+  }
+  const Geom* g = geom_at(P, X, i, j, k, CENT);
+  const int lbeg = (which == PFLAG_INTERP_V) ? U1 : RHO;
+  double p[NP], pold[NP];
+  for (int v = 0; v < NP; v++) p[v] = pv[v*NT + c];
+  for (int v = 0; v < NP; v++) pold[v] = p[v];
+  double gamma;
+  if (ngood < ngood_min[P.n_spatial_dims]) {
+    if (which == PFLAG_INTERP_V) {
+      double gm = X.gamma[c];
+      fixup_gamma(p, &gm, fr);
+      X.gamma[c] = gm;
+      fr.nf[NFAIL_INTERP_V]++;
+    } else {
+      double gm;
+      gamma_calc(p, g, &gm);
+      X.gamma[c] = gm;
+      fr.nf[NFAIL_INTERP_PRIM]++;
+    }
+  } else {
+    for (int l = lbeg; l <= U3; l++) p[l] = 0.;
+    for (int ii = -1; ii <= 1; ii++) for (int jj = -1; jj <= 1; jj++) for (int kk = -1; kk <= 1; kk++) {
+      if (pfl[o + ii + 3*(jj + 3*kk)]) {
+        long long cn = cidx(P, i + ii, j + jj, k + kk);
+        for (int l = lbeg; l <= U3; l++) p[l] += pv[l*NT + cn];
+      }
+    }
+    ngood *= 2;
+    for (int l = lbeg; l <= U3; l++) p[l] /= ngood;
+    int i_ret = gamma_calc(p, g, &gamma);
+    if (check_gamma(gamma)) {
+      double gamma_old = gamma;
+      gamma = 0.;
+      int ngood2 = 0;
+      for (int ii = -1; ii <= 1; ii++) for (int jj = -1; jj <= 1; jj++) for (int kk = -1; kk <= 1; kk++) {
+        if (pfl[o + ii + 3*(jj + 3*kk)]) {
+          long long cn = cidx(P, i + ii, j + jj, k + kk);
+          double pn[NP];
+          for (int v = 0; v < NP; v++) pn[v] = pv[v*NT + cn];
+          double f;
+          i_ret = gamma_calc(pn, geom_at(P, X, i + ii, j + jj, k + kk, CENT), &f);
+          if (!i_ret) { gamma += f; ngood2++; }
+        }
+      }
+      gamma /= ngood2;
+      double f = sqrt((gamma - 1.)*(gamma + 1.) / ((gamma_old - 1.)*(gamma_old + 1.)));
+      p[U1] *= f;
+      p[U2] *= f;
+      p[U3] *= f;
+      i_ret = gamma_calc(p, g, &gamma);
+    }
+    if (which == PFLAG_INTERP_V) {
+      if (i_ret || check_gamma(gamma)) {
+        for (int l = U1; l <= U3; l++) p[l] = pold[l];
+        double gm = X.gamma[c];
+        fixup_gamma(p, &gm, fr);
+        X.gamma[c] = gm;
+        fr.nf[NFAIL_INTERP_V]++;
+      } else {
+        fr.nf[NFAIL_USE_INTERP_V]++;
+        X.gamma[c] = gamma;
+      }
+    } else {
+      if (i_ret) {
+        for (int l = U1; l <= U3; l++) p[l] = pold[l];
+        gamma_calc(p, g, &gamma);
+        fr.nf[NFAIL_INTERP_PRIM]++;
+      }
+      fr.nf[NFAIL_USE_INTERP_PRIM]++;
+      if ((!i_ret) && check_gamma(gamma)) fixup_gamma(p, &gamma, fr);
+      X.gamma[c] = gamma;
+    }
+  }
+  for (int v = 0; v < NP; v++) pv[v*NT + c] = p[v];
+}
+
+// --------------------------------------------------------------------------
+// fixup(), src/fixup.c:8-24 with fixup1zone(), :35-66 (initial data)
+HD void fixup_init_zone(const Params& P, const Ptrs& X, int i, int j, int k, FailRec& fr) {
+  const long long NT = P.nt;
+  const long long c = cidx(P, i, j, k);
+  const Geom* g = geom_at(P, X, i, j, k, CENT);
+  double p[NP];
+  for (int v = 0; v < NP; v++) p[v] = X.p[v*NT + c];
+  double gamma;
+  if (gamma_calc(p, g, &gamma)) { fr.nf[NFAIL_GAMMA_CALC]++; gamma = 1.; }
+  int fl = 0;
+  double rf = floor_rho(P, X, i, c), uf = floor_uu(P, X, i, c);
+  if (p[RHO] < rf) { p[RHO] = rf; fl = 1; }
+  if (p[UU] < uf) { p[UU] = uf; fl = 1; }
+  if (fl) fr.nf[NFAIL_FLOOR]++;
+  if (check_gamma(gamma)) fixup_gamma(p, &gamma, fr);
+  for (int v = 0; v < NP; v++) X.p[v*NT + c] = p[v];
+}
+
+// --------------------------------------------------------------------------
+// timestep(), src/step_ch.c:546-623 (TIMESTEP_ORIGINAL); also advances t as
+// step_ch() does at src/step_ch.c:218-223.  One thread.
+HD void timestep_update(const Params& P, const Ptrs& X) {
+  double* sc = X.scal;
+  sc[SC_T_OLD] = sc[SC_T];
+  sc[SC_T] += sc[SC_DT];
+  sc[SC_DT_OLD] = sc[SC_DT];
+  double c1 = bits_dbl(X.status[ST_VMAX + 1]);
+  double c2 = bits_dbl(X.status[ST_VMAX + 2]);
+  double c3 = bits_dbl(X.status[ST_VMAX + 3]);
+  double dtsqij = sqrt((1./3.)/(
+      (c1*c1)/(P.dx[1]*P.dx[1]) +
+      (c2*c2)/(P.dx[2]*P.dx[2]) +
+      (c3*c3)/(P.dx[3]*P.dx[3])));
+  double dtt = P.cour*dtsqij;
+  if (dtt > 1.3*sc[SC_DT]) dtt = 1.3*sc[SC_DT];
+  if (sc[SC_T] + dtt > P.t_final) dtt = P.t_final - sc[SC_T];
+  sc[SC_DT] = dtt;
+}
+
+// --------------------------------------------------------------------------
+// divb_calc(), src/diag.c:503-602: |div B| at the corner of zone (i,j,k)
+HD double divb_corner(const Params& P, const Ptrs& X, const double* S, int i, int j, int k) {
+  const long long NT = P.nt;
+  double det_g[8], b1[8], b2[8], b3[8];
+  int id = 0;
+  for (int di = -1; di <= 0; di++) for (int dj = -1; dj <= 0; dj++) for (int dk = -1; dk <= 0; dk++) {
+    long long c = cidx(P, i + di, j + dj, k + dk);
+    det_g[id] = geom_at(P, X, i + di, j + dj, k + dk, CENT)->g;
+    double B1v = S[B1*NT + c], B2v = S[B2*NT + c], B3v = S[B3*NT + c];
+    b1[id] = (di == 0) ? B1v : -B1v;
+    b2[id] = (dj == 0) ? B2v : -B2v;
+    b3[id] = (dk == 0) ? B3v : -B3v;
+    id++;
+  }
+  double divb = 0., ftmp = 0.;
+  for (id = 0; id < 8; id++) ftmp += b1[id]*det_g[id];
+  divb += ftmp*P.invdx[1];
+  ftmp = 0.;
+  for (id = 0; id < 8; id++) ftmp += b2[id]*det_g[id];
+  divb += ftmp*P.invdx[2];
+  ftmp = 0.;
+  for (id = 0; id < 8; id++) ftmp += b3[id]*det_g[id];
+  divb += ftmp*P.invdx[3];
+  divb *= 0.25;
+  return fabs(divb);
+}
+
+// --------------------------------------------------------------------------
+// Index boxes of the kernels (inclusive bounds)
+struct Box {
+  int lo[3], hi[3];
+  HD long long count() const {
+    return (long long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+  }
+  HD void at(long long t, int& i, int& j, int& k) const {
+    const int nk = hi[2] - lo[2] + 1, nj = hi[1] - lo[1] + 1;
+    k = lo[2] + (int)(t % nk);
+    t /= nk;
+    j = lo[1] + (int)(t % nj);
+    i = lo[0] + (int)(t / nj);
+  }
+};
+
+HD void dim_n(const Params& P, int* n) { n[0] = P.n1; n[1] = P.n2; n[2] = P.n3; }
+
+// numerical_flux() face range, src/step_ch.c:656-664
+HD Box flux_box(const Params& P, int d) {
+  int n[3]; dim_n(P, n);
+  Box b;
+  for (int e = 0; e < 3; e++) {
+    if (e == d) { b.lo[e] = NG; b.hi[e] = NG + n[e]; }
+    else if (P.ct_para) { b.lo[e] = 0; b.hi[e] = n[e] + 2*NG - 1; }
+    else { b.lo[e] = NG - 1; b.hi[e] = NG + n[e]; }
+  }
+  return b;
+}
+// corner EMFs, src/flux_ct.c:31 and :780-846
+HD Box edge_box(const Params& P) {
+  Box b;
+  b.lo[0] = b.lo[1] = b.lo[2] = NG;
+  b.hi[0] = NG + P.n1; b.hi[1] = NG + P.n2; b.hi[2] = NG + P.n3;
+  return b;
+}
+HD Box interior_box(const Params& P) {
+  Box b;
+  b.lo[0] = b.lo[1] = b.lo[2] = NG;
+  b.hi[0] = NG + P.n1 - 1; b.hi[1] = NG + P.n2 - 1; b.hi[2] = NG + P.n3 - 1;
+  return b;
+}
+// ghost layers of direction dir, both sides: (side, g, a, b) flattened as
+// lo = {0, 0, a0, b0}: index box over (side*NG + g, a, b)
+HD Box ghost_box(const Params& P, int dir) {
+  Box b;
+  b.lo[0] = 0; b.hi[0] = 2*NG - 1;
+  if (dir == 0) { b.lo[1] = NG; b.hi[1] = NG + P.n2 - 1; b.lo[2] = NG; b.hi[2] = NG + P.n3 - 1; }
+  else if (dir == 1) { b.lo[1] = 0; b.hi[1] = P.n1t - 1; b.lo[2] = NG; b.hi[2] = NG + P.n3 - 1; }
+  else { b.lo[1] = 0; b.hi[1] = P.n1t - 1; b.lo[2] = 0; b.hi[2] = P.n2t - 1; }
+  return b;
+}
+// interior corners for div B: (N1S+1..N1E, ...) so that all 8 zones are interior
+HD Box divb_box(const Params& P) {
+  Box b;
+  b.lo[0] = NG + (P.n1 > 1); b.hi[0] = NG + P.n1 - 1;
+  b.lo[1] = NG + (P.n2 > 1); b.hi[1] = NG + P.n2 - 1;
+  b.lo[2] = NG + (P.n3 > 1); b.hi[2] = NG + P.n3 - 1;
+  return b;
+}
+
+}  // namespace hgpu

@@ -1,0 +1,332 @@
+// hgpu_core.h -- context, set-up and step sequencing shared by the GPU
+// library (hgpu_kernels.hip) and the host check build of the kernel bodies
+// (hgpu_hostcheck.cpp).  BE is the backend: it owns memory and launches.
+#pragma once
+#include "../../include/hgpu.h"
+#include "hgpu_bodies.h"
+#include "hgpu_geom.h"
+#include <string>
+#include <vector>
+#include <map>
+#include <cstring>
+#include <cstdio>
+#include <cstdlib>
+
+namespace hgpu {
+
+inline thread_local std::string g_last_error;
+inline int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+}  // namespace hgpu
+
+struct hgpu_ctx {
+  hgpu_config cfg;
+  hgpu::Params P;
+  hgpu::Ptrs X;
+  long long n_geom = 0, n_conn = 0;
+  int n1_conn = 0;
+  std::vector<double> stage;       // host staging buffer
+  void* be = nullptr;              // backend-private data
+  unsigned long long host_status[hgpu::ST_WORDS];
+  double host_scal[hgpu::SC_WORDS];
+};
+
+namespace hgpu {
+
+// Fills Params from the configuration (set_global_domain_variables(),
+// src/decomp.c:20-130, and global_init(), src/main.c:283-300).
+inline int make_params(const hgpu_config& c, Params& P) {
+  if (c.n1 < 1 || c.n2 < 1 || c.n3 < 1) return set_error(-2, "grid sizes must be >= 1");
+  if (c.metric != 0 && c.metric != 2) return set_error(-2, "metric must be 0 (Minkowski Cartesian) or 2 (Kerr-Schild)");
+  if (c.coords != 0 && c.coords != 1) return set_error(-2, "coords must be 0 (identity) or 1 (modified Kerr-Schild)");
+  if (c.recon != RECON_MC && c.recon != RECON_PPM_MC) return set_error(-2, "recon must be 2 (MC) or 5 (PPM+MC)");
+  if (c.flux != FLUX_LF && c.flux != FLUX_HLLE) return set_error(-2, "flux must be 0 (LF) or 1 (HLLE)");
+  if (c.bc != BC_CARTESIAN_OUTFLOW && c.bc != BC_SPHERICAL_OUTFLOW3) return set_error(-2, "bc must be 0 or 10");
+  if (c.ct_para && c.recon != RECON_PPM_MC)
+    return set_error(-2, "PPM EMFs (ct_para) use the PPM limiter of the reconstruction: recon must be 5");
+  if (c.cooling && c.metric != 2) return set_error(-2, "the h/r cooling function needs the Kerr-Schild metric");
+  if (c.bc == BC_SPHERICAL_OUTFLOW3 && (c.n2 < NG || c.n3 < NG))
+    return set_error(-2, "reflecting/periodic boundaries need at least NG interior zones in x2 and x3");
+  std::memset(&P, 0, sizeof(P));
+  P.n1 = c.n1; P.n2 = c.n2; P.n3 = c.n3;
+  P.n1t = c.n1 + 2*NG; P.n2t = c.n2 + 2*NG; P.n3t = c.n3 + 2*NG;
+  P.nt = (long long)P.n1t * P.n2t * P.n3t;
+  P.metric_dim = (c.metric == 2) ? 2 : 0;
+  P.flat = (c.metric == 0);
+  P.recon = c.recon;
+  P.flux = c.flux;
+  P.ct_para = c.ct_para;
+  P.cooling = c.cooling;
+  P.degen[0] = (c.n1 == 1); P.degen[1] = (c.n2 == 1); P.degen[2] = (c.n3 == 1);
+  P.rescale_r = c.rescale_r && !P.degen[0];
+  P.bc = c.bc;
+  P.floor_dim = (c.coords == 1) ? 1 : 3;
+  P.n_spatial_dims = (c.n1 > 1) + (c.n2 > 1) + (c.n3 > 1);
+  P.track_nfail = c.track_nfail;
+  for (int l = 0; l < 4; l++) { P.dx[l] = c.dx[l]; P.invdx[l] = 1./c.dx[l]; }
+  P.gam = c.gam;
+  P.gam_m1_o_gam = (c.gam - 1.)/c.gam;
+  P.cour = c.cour;
+  P.t_final = c.t_final;
+  P.q_exp = 0.5;
+  P.s_norm = 1.;
+  return 0;
+}
+
+inline CoordParams coord_params(const hgpu_config& c) {
+  CoordParams C;
+  C.metric = c.metric;
+  C.coords = c.coords;
+  for (int l = 0; l < 4; l++) { C.startx[l] = c.startx[l]; C.dx[l] = c.dx[l]; }
+  C.a = c.a; C.asq = c.a*c.a; C.M = 1.; C.R0 = c.R0;
+  C.h_slope = c.h_slope; C.th_beg = c.th_beg; C.th_length = c.th_length;
+  return C;
+}
+
+// Host tables (computed with the host libm as the reference computes them).
+struct HostTables {
+  std::vector<double> geom, conn, omega, tr, rscale, irscale, rhoflr, uuflr;
+};
+
+inline void build_tables(const hgpu_config& c, const Params& P, long long n_geom, long long n_conn, HostTables& T) {
+  CoordParams C = coord_params(c);
+  T.geom.assign(n_geom * GEOM_DOUBLES, 0.);
+  if (P.metric_dim == 2) {
+    for (int pos = 0; pos < NPOS; pos++)
+      for (int i = 0; i < P.n1t; i++)
+        for (int j = 0; j < P.n2t; j++)
+          geom_record(C, i, j, NG, pos, reinterpret_cast<Geom*>(&T.geom[(long long)geom_id(P, i, j, NG, pos) * GEOM_DOUBLES]));
+  } else {
+    geom_record(C, NG, NG, NG, CENT, reinterpret_cast<Geom*>(&T.geom[0]));
+  }
+  T.conn.assign(n_conn * 64, 0.);
+  if (P.metric_dim == 2 && !P.flat) {
+    for (int i = NG; i < NG + P.n1; i++)
+      for (int j = NG; j < NG + P.n2; j++)
+        ks_connection(C, i, j, &T.conn[(long long)conn_id(P, i, j, NG) * 64]);
+  }
+  // per-x1 tables
+  T.omega.assign(P.n1t, 0.); T.tr.assign(P.n1t, 0.);
+  T.rscale.assign(6 * P.n1t, 1.); T.irscale.assign(6 * P.n1t, 1.);
+  std::vector<double> rc(P.n1t), rf(P.n1t);
+  for (int i = 0; i < P.n1t; i++) {
+    double xp[4], x[4], J[4];
+    coord_xp(C, i, NG, NG, CENT, xp); x_of_xp(C, xp, x, J); rc[i] = x[1];
+    coord_xp(C, i, NG, NG, FACE1, xp); x_of_xp(C, xp, x, J); rf[i] = x[1];
+  }
+  if (c.cooling) {
+    CoolTables CT;
+    cool_setup(C, c.r_isco, c.h_o_r, &CT);
+    for (int i = 0; i < P.n1t; i++) {
+      T.omega[i] = omega_circular_equatorial(C, CT, rc[i]);
+      T.tr[i] = target_temperature(C, CT, rc[i]);
+    }
+  }
+  if (P.rescale_r) {
+    // rescale_prims_r() first call, src/recon.c:1141-1167
+    for (int i = 0; i < P.n1t; i++) {
+      for (int pz = 0; pz < 2; pz++) {
+        double rloc = pz ? rf[i] : rc[i];
+        double rf2 = rloc*rloc, rf0 = sqrt(rf2 * rloc), rf1 = rf2*rloc;
+        T.rscale[(3*pz + 0)*P.n1t + i] = rf0;
+        T.rscale[(3*pz + 1)*P.n1t + i] = rf1;
+        T.rscale[(3*pz + 2)*P.n1t + i] = rloc;
+      }
+      for (int s = 0; s < 6; s++) T.irscale[s*P.n1t + i] = 1/T.rscale[s*P.n1t + i];
+    }
+  }
+  // floors: coord_of_xp(), src/coord.c:3790-3797 (r = x^1 for spherical
+  // topology, |x| for Cartesian)
+  if (P.floor_dim == 1) {
+    T.rhoflr.resize(P.n1t); T.uuflr.resize(P.n1t);
+    for (int i = 0; i < P.n1t; i++) {
+      T.rhoflr[i] = c.rhomin*pow(rc[i], c.rhopower);
+      T.uuflr[i] = c.uumin*pow(rc[i], c.uupower);
+    }
+  } else {
+    T.rhoflr.resize(P.nt); T.uuflr.resize(P.nt);
+    for (int i = 0; i < P.n1t; i++) for (int j = 0; j < P.n2t; j++) for (int k = 0; k < P.n3t; k++) {
+      double xp[4], x[4], J[4];
+      coord_xp(C, i, j, k, CENT, xp); x_of_xp(C, xp, x, J);
+      double r = sqrt(x[1]*x[1] + x[2]*x[2] + x[3]*x[3]);
+      long long cc = cidx(P, i, j, k);
+      T.rhoflr[cc] = c.rhomin*pow(r, c.rhopower);
+      T.uuflr[cc] = c.uumin*pow(r, c.uupower);
+    }
+  }
+}
+
+// AoS [zone][8] <-> SoA [8][zone]
+inline void aos_to_soa(const double* aos, double* soa, long long nt, int nv) {
+  for (long long c = 0; c < nt; c++) for (int v = 0; v < nv; v++) soa[v*nt + c] = aos[c*nv + v];
+}
+inline void soa_to_aos(const double* soa, double* aos, long long nt, int nv) {
+  for (long long c = 0; c < nt; c++) for (int v = 0; v < nv; v++) aos[c*nv + v] = soa[v*nt + c];
+}
+
+template <class BE>
+struct Api {
+  static int create(const hgpu_config* cfg, hgpu_ctx** out) {
+    if (!cfg || !out) return set_error(-1, "null argument");
+    hgpu_ctx* c = new hgpu_ctx();
+    c->cfg = *cfg;
+    int rc = make_params(*cfg, c->P);
+    if (rc) { delete c; return rc; }
+    const Params& P = c->P;
+    c->n_geom = (P.metric_dim == 2) ? (long long)NPOS * P.n1t * P.n2t : 1;
+    c->n_conn = (P.metric_dim == 2) ? (long long)P.n1 * P.n2 : 1;
+    rc = BE::init(c);
+    if (rc) { delete c; return rc; }
+    HostTables T;
+    build_tables(*cfg, P, c->n_geom, c->n_conn, T);
+    Ptrs& X = c->X;
+    const long long NT = P.nt;
+    rc = 0;
+    rc |= BE::alloc(c, (void**)&X.p, sizeof(double) * NP * NT);
+    rc |= BE::alloc(c, (void**)&X.ph, sizeof(double) * NP * NT);
+    rc |= BE::alloc(c, (void**)&X.F, sizeof(double) * 3 * NFLUX * NT);
+    rc |= BE::alloc(c, (void**)&X.emf, sizeof(double) * 3 * NT);
+    rc |= BE::alloc(c, (void**)&X.geom, sizeof(double) * GEOM_DOUBLES * c->n_geom);
+    rc |= BE::alloc(c, (void**)&X.conn, sizeof(double) * 64 * c->n_conn);
+    rc |= BE::alloc(c, (void**)&X.omega, sizeof(double) * P.n1t);
+    rc |= BE::alloc(c, (void**)&X.tr, sizeof(double) * P.n1t);
+    rc |= BE::alloc(c, (void**)&X.rscale, sizeof(double) * 6 * P.n1t);
+    rc |= BE::alloc(c, (void**)&X.irscale, sizeof(double) * 6 * P.n1t);
+    rc |= BE::alloc(c, (void**)&X.rhoflr, sizeof(double) * T.rhoflr.size());
+    rc |= BE::alloc(c, (void**)&X.uuflr, sizeof(double) * T.uuflr.size());
+    rc |= BE::alloc(c, (void**)&X.gamma, sizeof(double) * NT);
+    rc |= BE::alloc(c, (void**)&X.pflag, sizeof(int) * NT);
+    rc |= BE::alloc(c, (void**)&X.nfail, sizeof(int) * (P.track_nfail ? N_NFAIL * NT : 1));
+    rc |= BE::alloc(c, (void**)&X.status, sizeof(unsigned long long) * ST_WORDS);
+    rc |= BE::alloc(c, (void**)&X.scal, sizeof(double) * SC_WORDS);
+    if (rc) { destroy(c); return set_error(-3, "device allocation failed: " + g_last_error); }
+    rc |= BE::h2d(c, (void*)X.geom, T.geom.data(), sizeof(double) * T.geom.size());
+    rc |= BE::h2d(c, (void*)X.conn, T.conn.data(), sizeof(double) * T.conn.size());
+    rc |= BE::h2d(c, (void*)X.omega, T.omega.data(), sizeof(double) * T.omega.size());
+    rc |= BE::h2d(c, (void*)X.tr, T.tr.data(), sizeof(double) * T.tr.size());
+    rc |= BE::h2d(c, (void*)X.rscale, T.rscale.data(), sizeof(double) * T.rscale.size());
+    rc |= BE::h2d(c, (void*)X.irscale, T.irscale.data(), sizeof(double) * T.irscale.size());
+    rc |= BE::h2d(c, (void*)X.rhoflr, T.rhoflr.data(), sizeof(double) * T.rhoflr.size());
+    rc |= BE::h2d(c, (void*)X.uuflr, T.uuflr.data(), sizeof(double) * T.uuflr.size());
+    rc |= BE::zero(c, X.p, sizeof(double) * NP * NT);
+    rc |= BE::zero(c, X.ph, sizeof(double) * NP * NT);
+    rc |= BE::zero(c, X.F, sizeof(double) * 3 * NFLUX * NT);
+    rc |= BE::zero(c, X.emf, sizeof(double) * 3 * NT);
+    rc |= BE::zero(c, X.gamma, sizeof(double) * NT);
+    rc |= BE::zero(c, X.pflag, sizeof(int) * NT);
+    rc |= BE::zero(c, X.nfail, sizeof(int) * (P.track_nfail ? N_NFAIL * NT : 1));
+    rc |= BE::zero(c, X.status, sizeof(unsigned long long) * ST_WORDS);
+    double sc[SC_WORDS] = {0};
+    sc[SC_T] = cfg->startx[0];
+    sc[SC_DT] = cfg->dx[0];
+    rc |= BE::h2d(c, X.scal, sc, sizeof(sc));
+    rc |= BE::sync(c);
+    if (rc) { destroy(c); return set_error(-3, "device initialisation failed: " + g_last_error); }
+    *out = c;
+    return 0;
+  }
+
+  static int destroy(hgpu_ctx* c) {
+    if (!c) return 0;
+    Ptrs& X = c->X;
+    void* ptrs[] = {X.p, X.ph, X.F, X.emf, (void*)X.geom, (void*)X.conn, (void*)X.omega, (void*)X.tr,
+                    (void*)X.rscale, (void*)X.irscale, (void*)X.rhoflr, (void*)X.uuflr, X.gamma,
+                    X.pflag, X.nfail, X.status, X.scal};
+    for (void* p : ptrs) if (p) BE::release(c, p);
+    BE::fini(c);
+    delete c;
+    return 0;
+  }
+
+  static int upload_prims(hgpu_ctx* c, int which, const double* host) {
+    if (!c || !host) return set_error(-1, "null argument");
+    const long long NT = c->P.nt;
+    c->stage.resize(NP * NT);
+    aos_to_soa(host, c->stage.data(), NT, NP);
+    double* dst = which ? c->X.ph : c->X.p;
+    int rc = BE::h2d(c, dst, c->stage.data(), sizeof(double) * NP * NT);
+    return rc ? rc : BE::sync(c);
+  }
+
+  static int download_prims(hgpu_ctx* c, int which, double* host) {
+    if (!c || !host) return set_error(-1, "null argument");
+    const long long NT = c->P.nt;
+    c->stage.resize(NP * NT);
+    const double* src = which ? c->X.ph : c->X.p;
+    int rc = BE::d2h(c, c->stage.data(), src, sizeof(double) * NP * NT);
+    if (rc) return rc;
+    soa_to_aos(c->stage.data(), host, NT, NP);
+    return 0;
+  }
+
+  static int set_time(hgpu_ctx* c, double t, double dt) {
+    double sc[SC_WORDS] = {0};
+    sc[SC_T] = t; sc[SC_DT] = dt; sc[SC_T_OLD] = t; sc[SC_DT_OLD] = dt;
+    int rc = BE::h2d(c, c->X.scal, sc, sizeof(sc));
+    return rc ? rc : BE::sync(c);
+  }
+
+  static int get_time(hgpu_ctx* c, double* t, double* dt, double* dt_old) {
+    double sc[SC_WORDS];
+    int rc = BE::d2h(c, sc, c->X.scal, sizeof(sc));
+    if (rc) return rc;
+    if (t) *t = sc[SC_T];
+    if (dt) *dt = sc[SC_DT];
+    if (dt_old) *dt_old = sc[SC_DT_OLD];
+    return 0;
+  }
+
+  // bounds(prim, skip_mpi): skip_mpi = 0 always applies the physical BCs;
+  // skip_mpi = 1 only if boundary_phys_pflag (src/bounds.c:56)
+  static int bounds(hgpu_ctx* c, double* S, int skip_mpi) {
+    int rc = 0;
+    for (int dir = 0; dir < 3; dir++) rc |= BE::bounds_dir(c, S, dir, skip_mpi);
+    return rc;
+  }
+
+  // One advance() stage plus its boundaries and fixups (src/step_ch.c:92-121,
+  // :159-180).
+  static int substep(hgpu_ctx* c, int stage) {
+    Ptrs& X = c->X;
+    double* src = (stage == 0) ? X.p : X.ph;
+    double* dst = (stage == 0) ? X.ph : X.p;
+    int rc = 0;
+    rc |= BE::reset_vmax(c);
+    for (int d = 0; d < 3; d++) rc |= BE::flux(c, src, d);
+    rc |= BE::emf(c);
+    rc |= BE::update(c, stage);
+    rc |= bounds(c, dst, 0);
+    rc |= BE::fixup_interp(c, dst, PFLAG_INTERP_V);
+    rc |= BE::fixup_interp(c, dst, PFLAG_INTERP_PRIM);
+    rc |= bounds(c, dst, 1);
+    rc |= BE::clear_boundary_flag(c);
+    return rc;
+  }
+
+  static int step(hgpu_ctx* c, int nsteps) {
+    if (!c) return set_error(-1, "null context");
+    int rc = 0;
+    for (int s = 0; s < nsteps && !rc; s++) {
+      rc |= BE::mark(c, "step", 0);
+      rc |= BE::clear_failures(c);      // src/step_ch.c:65-69
+      rc |= substep(c, 0);
+      rc |= BE::clear_failures(c);      // src/step_ch.c:117-121
+      rc |= substep(c, 1);
+      rc |= BE::timestep(c);            // src/step_ch.c:218-223
+      rc |= BE::mark(c, "step", 1);
+    }
+    return rc;
+  }
+
+  static int fixup_bounds(hgpu_ctx* c) {
+    int rc = BE::fixup_init(c);
+    rc |= bounds(c, c->X.p, 0);
+    rc |= BE::sync(c);
+    return rc;
+  }
+};
+
+}  // namespace hgpu

@@ -1,0 +1,168 @@
+// hgpu_hostcheck.cpp -- host build of the kernel bodies (libhgpu_hostcheck.so).
+//
+// Runs exactly the per-thread bodies the GPU kernels run (hgpu_bodies.h), in
+// sequential loops over the same index boxes, behind the same C ABI.  It exists
+// so the kernels' arithmetic can be compared with the reference build on a
+// machine without a GPU; it is never loaded by the product path (the Python
+// wrapper loads libhgpu.so and fails if that is missing).
+#include "hgpu_core.h"
+
+namespace hgpu {
+
+static void host_commit(const Params& P, const Ptrs& X, const FailRec& fr, long long c, bool count_zone) {
+  const long long NT = P.nt;
+  for (int l = 0; l < N_NFAIL; l++) {
+    if (!fr.nf[l]) continue;
+    if (count_zone && P.track_nfail) X.nfail[l*NT + c] += fr.nf[l];
+    X.status[ST_NFAILTOT + l] += (unsigned long long)fr.nf[l];
+  }
+  if (fr.pflag) {
+    X.status[ST_FAILURE_EXISTS] = 1;
+    X.status[(fr.pflag == PFLAG_INTERP_PRIM) ? ST_FAILTYPE_PRIM : ST_FAILTYPE_V] = 1;
+    if (fr.boundary) X.status[ST_BOUNDARY_PHYS] = 1;
+  }
+}
+
+struct HostBE {
+  static int init(hgpu_ctx*) { return 0; }
+  static void fini(hgpu_ctx*) {}
+  static int alloc(hgpu_ctx*, void** p, size_t bytes) {
+    *p = calloc(1, bytes ? bytes : 8);
+    return *p ? 0 : set_error(-3, "calloc failed");
+  }
+  static void release(hgpu_ctx*, void* p) { free(p); }
+  static int h2d(hgpu_ctx*, void* d, const void* h, size_t bytes) { memcpy(d, h, bytes); return 0; }
+  static int d2h(hgpu_ctx*, void* h, const void* d, size_t bytes) { memcpy(h, d, bytes); return 0; }
+  static int zero(hgpu_ctx*, void* p, size_t bytes) { memset(p, 0, bytes); return 0; }
+  static int sync(hgpu_ctx*) { return 0; }
+  static int mark(hgpu_ctx*, const char*, int) { return 0; }
+  static int profile_enable(hgpu_ctx*, int) { return 0; }
+  static int profile_read(hgpu_ctx*, const char*, double* ms, long long* n) {
+    if (ms) *ms = 0.;
+    if (n) *n = 0;
+    return 0;
+  }
+  static int reset_vmax(hgpu_ctx* c) {
+    for (int l = 0; l < 4; l++) c->X.status[ST_VMAX + l] = 0;
+    return 0;
+  }
+  static int reset_nfailtot(hgpu_ctx* c) {
+    for (int l = 0; l < N_NFAIL; l++) c->X.status[ST_NFAILTOT + l] = 0;
+    return 0;
+  }
+  static int flux(hgpu_ctx* c, const double* src, int d) {
+    Box b = flux_box(c->P, d);
+    for (long long t = 0; t < b.count(); t++) {
+      int i, j, k;
+      b.at(t, i, j, k);
+      FailRec fr; fr.clear();
+      flux_face(c->P, c->X, src, d, i, j, k, fr);
+      host_commit(c->P, c->X, fr, cidx(c->P, i, j, k), true);
+    }
+    return 0;
+  }
+  static int emf(hgpu_ctx* c) {
+    Box b = edge_box(c->P);
+    for (long long t = 0; t < b.count(); t++) {
+      int i, j, k;
+      b.at(t, i, j, k);
+      emf_edge(c->P, c->X, i, j, k);
+    }
+    return 0;
+  }
+  static int update(hgpu_ctx* c, int stage) {
+    Box b = interior_box(c->P);
+    const double dt = c->X.scal[SC_DT];
+    for (long long t = 0; t < b.count(); t++) {
+      int i, j, k;
+      b.at(t, i, j, k);
+      FailRec fr; fr.clear();
+      update_zone(c->P, c->X, stage, dt, i, j, k, fr);
+      host_commit(c->P, c->X, fr, cidx(c->P, i, j, k), true);
+    }
+    return 0;
+  }
+  static int bounds_dir(hgpu_ctx* c, double* S, int dir, int skip_mpi) {
+    if (skip_mpi && !c->X.status[ST_BOUNDARY_PHYS]) return 0;
+    const int do_pflag = c->X.status[ST_FAILURE_EXISTS] ? 1 : 0;
+    Box b = ghost_box(c->P, dir);
+    const int n1e = c->P.n1 + NG - 1;
+    for (long long t = 0; t < b.count(); t++) {
+      int sg, a, bb;
+      b.at(t, sg, a, bb);
+      const int side = sg / NG, g = sg % NG;
+      FailRec fr; fr.clear();
+      bound_zone(c->P, c->X, S, dir, side, g, a, bb, do_pflag, fr);
+      if (dir == 0) host_commit(c->P, c->X, fr, cidx(c->P, side ? (n1e + 1 + g) : (NG - 1 - g), a, bb), true);
+    }
+    return 0;
+  }
+  static int fixup_interp(hgpu_ctx* c, double* S, int which) {
+    if (!c->X.status[ST_FAILURE_EXISTS]) return 0;
+    if (!c->X.status[(which == PFLAG_INTERP_PRIM) ? ST_FAILTYPE_PRIM : ST_FAILTYPE_V]) return 0;
+    Box b = interior_box(c->P);
+    for (long long t = 0; t < b.count(); t++) {
+      int i, j, k;
+      b.at(t, i, j, k);
+      FailRec fr; fr.clear();
+      fixup_interp_zone(c->P, c->X, S, which, i, j, k, fr);
+      host_commit(c->P, c->X, fr, cidx(c->P, i, j, k), true);
+    }
+    return 0;
+  }
+  static int clear_boundary_flag(hgpu_ctx* c) { c->X.status[ST_BOUNDARY_PHYS] = 0; return 0; }
+  static int clear_failures(hgpu_ctx* c) {
+    if (c->X.status[ST_FAILURE_EXISTS]) {
+      for (long long t = 0; t < c->P.nt; t++) c->X.pflag[t] = 0;
+    }
+    c->X.status[ST_FAILURE_EXISTS] = 0;
+    c->X.status[ST_FAILTYPE_V] = 0;
+    c->X.status[ST_FAILTYPE_PRIM] = 0;
+    return 0;
+  }
+  static int timestep(hgpu_ctx* c) { timestep_update(c->P, c->X); return 0; }
+  static int fixup_init(hgpu_ctx* c) {
+    Box b = interior_box(c->P);
+    for (long long t = 0; t < b.count(); t++) {
+      int i, j, k;
+      b.at(t, i, j, k);
+      FailRec fr; fr.clear();
+      fixup_init_zone(c->P, c->X, i, j, k, fr);
+      host_commit(c->P, c->X, fr, cidx(c->P, i, j, k), true);
+    }
+    return 0;
+  }
+  static int divb_max(hgpu_ctx* c, double* out) {
+    Box b = divb_box(c->P);
+    double m = 0.;
+    for (long long t = 0; t < b.count(); t++) {
+      int i, j, k;
+      b.at(t, i, j, k);
+      double v = divb_corner(c->P, c->X, c->X.p, i, j, k);
+      if (v > m) m = v;
+    }
+    *out = m;
+    return 0;
+  }
+  static int con2prim_batch(hgpu_ctx* c, long long n, const double* U, const double* geom, double* prim,
+                            double* gamma, int* code, int* iters) {
+    const Geom* G = reinterpret_cast<const Geom*>(geom);
+    for (long long t = 0; t < n; t++) {
+      double u[NP], pr[NP];
+      for (int l = 0; l < NP; l++) { u[l] = U[l*n + t]; pr[l] = prim[l*n + t]; }
+      double gm = 1., bsq = 0.;
+      int it = 0;
+      int ret = utoprim_2d_fast(u, &G[t], c->P.gam, c->P.gam_m1_o_gam, pr, &gm, &bsq, &it);
+      for (int l = 0; l < NP; l++) prim[l*n + t] = pr[l];
+      gamma[t] = gm;
+      code[t] = ret;
+      if (iters) iters[t] = it;
+    }
+    return 0;
+  }
+};
+
+}  // namespace hgpu
+
+using BE = hgpu::HostBE;
+#include "hgpu_capi.inc"

@@ -1,0 +1,394 @@
+// hgpu_kernels.hip -- MI355X (gfx950) backend of the per-substep GRMHD advance.
+//
+// Kernels are thin wrappers over the per-thread bodies of hgpu_bodies.h: a
+// flattened 1D launch over each kernel's index box with k fastest, so the 64
+// lanes of a wavefront walk contiguous zones of one x3 pencil (coalesced SoA
+// loads; the 2D geometry record is a wavefront-wide broadcast).  Failure
+// counters are summed across the wavefront before one atomic per wave.
+#include "hgpu_core.h"
+#include <hip/hip_runtime.h>
+
+namespace hgpu {
+
+constexpr int BLOCK = 256;
+
+struct ProfEvent { std::string name; hipEvent_t a, b; };
+
+struct GpuPriv {
+  hipStream_t stream = nullptr;
+  bool prof = false;
+  std::vector<ProfEvent> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long long>> acc;
+  std::map<std::string, hipEvent_t> open;   // mark() begin events
+};
+
+static inline GpuPriv* priv(hgpu_ctx* c) { return static_cast<GpuPriv*>(c->be); }
+
+static int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  return set_error(-4, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// --------------------------------------------------------------------------
+__device__ __forceinline__ int wave_sum(int v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Writes a zone's fail() tallies: per-zone nfail, wave-summed nfailtot, and the
+// failure_exists / failure_type / boundary_phys_pflag words.
+__device__ __forceinline__ void commit_fail(const Params& P, const Ptrs& X, const FailRec& fr,
+                                            bool active, long long c, unsigned mask) {
+  const long long NT = P.nt;
+  const int lane = threadIdx.x & 63;
+  for (int l = 0; l < N_NFAIL; l++) {
+    if (!(mask & (1u << l))) continue;
+    int v = active ? fr.nf[l] : 0;
+    if (active && v && P.track_nfail) X.nfail[l*NT + c] += v;
+    v = wave_sum(v);
+    if (lane == 0 && v) atomicAdd(&X.status[ST_NFAILTOT + l], (unsigned long long)v);
+  }
+  if (active && fr.pflag) {
+    atomicOr(&X.status[ST_FAILURE_EXISTS], 1ull);
+    atomicOr(&X.status[(fr.pflag == PFLAG_INTERP_PRIM) ? ST_FAILTYPE_PRIM : ST_FAILTYPE_V], 1ull);
+    if (fr.boundary) atomicOr(&X.status[ST_BOUNDARY_PHYS], 1ull);
+  }
+}
+
+constexpr unsigned MASK_GAMMA = 1u << NFAIL_GAMMA_CALC;
+constexpr unsigned MASK_ALL = (1u << N_NFAIL) - 1u;
+
+__global__ __launch_bounds__(BLOCK) void k_flux(Params P, Ptrs X, const double* __restrict__ src, int d, Box b) {
+  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = t < b.count();
+  FailRec fr; fr.clear();
+  long long c = 0;
+  if (active) {
+    int i, j, k;
+    b.at(t, i, j, k);
+    flux_face(P, X, src, d, i, j, k, fr);
+    c = cidx(P, i, j, k);
+  }
+  commit_fail(P, X, fr, active, c, MASK_GAMMA);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_emf(Params P, Ptrs X, Box b) {
+  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= b.count()) return;
+  int i, j, k;
+  b.at(t, i, j, k);
+  emf_edge(P, X, i, j, k);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_update(Params P, Ptrs X, int stage, Box b) {
+  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = t < b.count();
+  FailRec fr; fr.clear();
+  long long c = 0;
+  if (active) {
+    int i, j, k;
+    b.at(t, i, j, k);
+    const double dt = X.scal[SC_DT];
+    update_zone(P, X, stage, dt, i, j, k, fr);
+    c = cidx(P, i, j, k);
+  }
+  commit_fail(P, X, fr, active, c, MASK_ALL);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bounds(Params P, Ptrs X, double* S, int dir, int skip_mpi, Box b) {
+  if (skip_mpi && !X.status[ST_BOUNDARY_PHYS]) return;
+  const int do_pflag = X.status[ST_FAILURE_EXISTS] ? 1 : 0;
+  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = t < b.count();
+  FailRec fr; fr.clear();
+  long long c = 0;
+  if (active) {
+    int sg, a, bb;
+    b.at(t, sg, a, bb);
+    const int side = sg / NG, g = sg % NG;
+    bound_zone(P, X, S, dir, side, g, a, bb, do_pflag, fr);
+    // nfail of inflow_check() belongs to the ghost zone
+    const int n1e = P.n1 + NG - 1;
+    if (dir == 0) c = cidx(P, side ? (n1e + 1 + g) : (NG - 1 - g), a, bb);
+  }
+  commit_fail(P, X, fr, active && dir == 0, c, MASK_GAMMA);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fixup_interp(Params P, Ptrs X, double* S, int which, Box b) {
+  if (!X.status[ST_FAILURE_EXISTS]) return;
+  if (!X.status[(which == PFLAG_INTERP_PRIM) ? ST_FAILTYPE_PRIM : ST_FAILTYPE_V]) return;
+  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = t < b.count();
+  FailRec fr; fr.clear();
+  long long c = 0;
+  if (active) {
+    int i, j, k;
+    b.at(t, i, j, k);
+    fixup_interp_zone(P, X, S, which, i, j, k, fr);
+    c = cidx(P, i, j, k);
+  }
+  commit_fail(P, X, fr, active, c, MASK_ALL);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fixup_init(Params P, Ptrs X, Box b) {
+  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = t < b.count();
+  FailRec fr; fr.clear();
+  long long c = 0;
+  if (active) {
+    int i, j, k;
+    b.at(t, i, j, k);
+    fixup_init_zone(P, X, i, j, k, fr);
+    c = cidx(P, i, j, k);
+  }
+  commit_fail(P, X, fr, active, c, MASK_ALL);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_clear_pflag(Ptrs X, long long nt) {
+  if (!X.status[ST_FAILURE_EXISTS]) return;
+  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (t < nt) X.pflag[t] = 0;
+}
+
+__global__ void k_reset_failure_words(Ptrs X) {
+  if (threadIdx.x == 0) {
+    X.status[ST_FAILURE_EXISTS] = 0;
+    X.status[ST_FAILTYPE_V] = 0;
+    X.status[ST_FAILTYPE_PRIM] = 0;
+  }
+}
+
+__global__ void k_reset_words(Ptrs X, int first, int n) {
+  if ((int)threadIdx.x < n) X.status[first + threadIdx.x] = 0;
+}
+
+__global__ void k_timestep(Params P, Ptrs X) {
+  if (threadIdx.x == 0) timestep_update(P, X);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_zero(unsigned int* p, long long n) {
+  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (t < n) p[t] = 0u;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_divb(Params P, Ptrs X, Box b, unsigned long long* out) {
+  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= b.count()) return;
+  int i, j, k;
+  b.at(t, i, j, k);
+  double v = divb_corner(P, X, X.p, i, j, k);
+  if (v == v) atomicMax(out, dbl_bits(v));
+}
+
+// Batched Utoprim_2d_fast() on independent zones (SoA buffers).
+__global__ __launch_bounds__(BLOCK) void k_con2prim(Params P, long long n, const double* __restrict__ U,
+                                                   const Geom* __restrict__ geom, double* prim,
+                                                   double* gamma, int* code, int* iters) {
+  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= n) return;
+  double u[NP], pr[NP];
+  for (int l = 0; l < NP; l++) { u[l] = U[l*n + t]; pr[l] = prim[l*n + t]; }
+  double gm = 1., bsq = 0.;
+  int it = 0;
+  int ret = utoprim_2d_fast(u, &geom[t], P.gam, P.gam_m1_o_gam, pr, &gm, &bsq, &it);
+  for (int l = 0; l < NP; l++) prim[l*n + t] = pr[l];
+  gamma[t] = gm;
+  code[t] = ret;
+  if (iters) iters[t] = it;
+}
+
+// --------------------------------------------------------------------------
+struct GpuBE {
+  static int init(hgpu_ctx* c) {
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev < 1)
+      return set_error(-5, "no HIP device available (the hgpu library runs only on a GPU)");
+    GpuPriv* g = new GpuPriv();
+    c->be = g;
+    return hip_check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
+  }
+  static void fini(hgpu_ctx* c) {
+    GpuPriv* g = priv(c);
+    if (!g) return;
+    if (g->stream) { (void)hipStreamSynchronize(g->stream); (void)hipStreamDestroy(g->stream); }
+    for (auto& pe : g->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
+    for (auto ev : g->pool) (void)hipEventDestroy(ev);
+    for (auto& kv : g->open) (void)hipEventDestroy(kv.second);
+    delete g;
+    c->be = nullptr;
+  }
+  static int alloc(hgpu_ctx*, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    return hip_check(hipMalloc(p, bytes), "hipMalloc");
+  }
+  static void release(hgpu_ctx*, void* p) { (void)hipFree(p); }
+  static int h2d(hgpu_ctx* c, void* d, const void* h, size_t bytes) {
+    return hip_check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, priv(c)->stream), "hipMemcpy H2D") |
+           hip_check(hipStreamSynchronize(priv(c)->stream), "hipStreamSynchronize");
+  }
+  static int d2h(hgpu_ctx* c, void* h, const void* d, size_t bytes) {
+    return hip_check(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, priv(c)->stream), "hipMemcpy D2H") |
+           hip_check(hipStreamSynchronize(priv(c)->stream), "hipStreamSynchronize");
+  }
+  // zero fill with a kernel (no memset nodes on the stream)
+  static int zero(hgpu_ctx* c, void* p, size_t bytes) {
+    if (bytes % 4) return set_error(-2, "zero(): size not a multiple of 4");
+    long long n = (long long)(bytes / 4);
+    long long nb = (n + BLOCK - 1) / BLOCK;
+    if (nb > 0) hipLaunchKernelGGL(k_zero, dim3((unsigned)nb), dim3(BLOCK), 0, priv(c)->stream, (unsigned int*)p, n);
+    return hip_check(hipGetLastError(), "k_zero");
+  }
+  static int sync(hgpu_ctx* c) { return hip_check(hipStreamSynchronize(priv(c)->stream), "hipStreamSynchronize"); }
+
+  // profiling ---------------------------------------------------------------
+  static hipEvent_t ev(GpuPriv* g) {
+    if (!g->pool.empty()) { hipEvent_t e = g->pool.back(); g->pool.pop_back(); return e; }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  struct Scope {
+    GpuPriv* g; const char* name; hipEvent_t a = nullptr;
+    Scope(hgpu_ctx* c, const char* n) : g(priv(c)), name(n) {
+      if (g->prof) { a = ev(g); (void)hipEventRecord(a, g->stream); }
+    }
+    ~Scope() {
+      if (g->prof) { hipEvent_t b = ev(g); (void)hipEventRecord(b, g->stream); g->pending.push_back({name, a, b}); }
+    }
+  };
+  static int mark(hgpu_ctx* c, const char* name, int end) {
+    GpuPriv* g = priv(c);
+    if (!g->prof) return 0;
+    if (!end) { hipEvent_t a = ev(g); (void)hipEventRecord(a, g->stream); g->open[name] = a; }
+    else {
+      auto it = g->open.find(name);
+      if (it == g->open.end()) return 0;
+      hipEvent_t b = ev(g);
+      (void)hipEventRecord(b, g->stream);
+      g->pending.push_back({name, it->second, b});
+      g->open.erase(it);
+    }
+    return 0;
+  }
+  static int profile_enable(hgpu_ctx* c, int on) {
+    GpuPriv* g = priv(c);
+    int rc = sync(c);
+    drain(g);
+    g->acc.clear();
+    g->prof = on != 0;
+    return rc;
+  }
+  static void drain(GpuPriv* g) {
+    for (auto& pe : g->pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess) {
+        auto& a = g->acc[pe.name];
+        a.first += ms;
+        a.second += 1;
+      }
+      g->pool.push_back(pe.a);
+      g->pool.push_back(pe.b);
+    }
+    g->pending.clear();
+  }
+  static int profile_read(hgpu_ctx* c, const char* name, double* ms, long long* cnt) {
+    GpuPriv* g = priv(c);
+    int rc = sync(c);
+    drain(g);
+    auto it = g->acc.find(name);
+    if (ms) *ms = (it == g->acc.end()) ? 0. : it->second.first;
+    if (cnt) *cnt = (it == g->acc.end()) ? 0 : it->second.second;
+    return rc;
+  }
+
+  // launches ------------------------------------------------------------------
+  static unsigned nblocks(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+  static int reset_vmax(hgpu_ctx* c) {
+    hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_VMAX, 4);
+    return hip_check(hipGetLastError(), "k_reset_words");
+  }
+  static int reset_nfailtot(hgpu_ctx* c) {
+    hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_NFAILTOT, N_NFAIL);
+    return hip_check(hipGetLastError(), "k_reset_words");
+  }
+  static int flux(hgpu_ctx* c, const double* src, int d) {
+    Box b = flux_box(c->P, d);
+    Scope s(c, "flux");
+    hipLaunchKernelGGL(k_flux, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, src, d, b);
+    return hip_check(hipGetLastError(), "k_flux");
+  }
+  static int emf(hgpu_ctx* c) {
+    Box b = edge_box(c->P);
+    Scope s(c, "emf");
+    hipLaunchKernelGGL(k_emf, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, b);
+    return hip_check(hipGetLastError(), "k_emf");
+  }
+  static int update(hgpu_ctx* c, int stage) {
+    Box b = interior_box(c->P);
+    Scope s(c, "update");
+    hipLaunchKernelGGL(k_update, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage, b);
+    return hip_check(hipGetLastError(), "k_update");
+  }
+  static int bounds_dir(hgpu_ctx* c, double* S, int dir, int skip_mpi) {
+    Box b = ghost_box(c->P, dir);
+    Scope s(c, "bounds");
+    hipLaunchKernelGGL(k_bounds, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, dir, skip_mpi, b);
+    return hip_check(hipGetLastError(), "k_bounds");
+  }
+  static int fixup_interp(hgpu_ctx* c, double* S, int which) {
+    Box b = interior_box(c->P);
+    Scope s(c, "fixup");
+    hipLaunchKernelGGL(k_fixup_interp, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, which, b);
+    return hip_check(hipGetLastError(), "k_fixup_interp");
+  }
+  static int clear_boundary_flag(hgpu_ctx* c) {
+    hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_BOUNDARY_PHYS, 1);
+    return hip_check(hipGetLastError(), "k_reset_words");
+  }
+  static int clear_failures(hgpu_ctx* c) {
+    hipLaunchKernelGGL(k_clear_pflag, dim3(nblocks(c->P.nt)), dim3(BLOCK), 0, priv(c)->stream, c->X, c->P.nt);
+    hipLaunchKernelGGL(k_reset_failure_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X);
+    return hip_check(hipGetLastError(), "k_clear_pflag");
+  }
+  static int timestep(hgpu_ctx* c) {
+    hipLaunchKernelGGL(k_timestep, dim3(1), dim3(64), 0, priv(c)->stream, c->P, c->X);
+    return hip_check(hipGetLastError(), "k_timestep");
+  }
+  static int fixup_init(hgpu_ctx* c) {
+    Box b = interior_box(c->P);
+    hipLaunchKernelGGL(k_fixup_init, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, b);
+    return hip_check(hipGetLastError(), "k_fixup_init");
+  }
+  static int divb_max(hgpu_ctx* c, double* out) {
+    unsigned long long* d = nullptr;
+    int rc = alloc(c, (void**)&d, 8);
+    if (rc) return rc;
+    rc |= zero(c, d, 8);
+    Box b = divb_box(c->P);
+    if (b.count() > 0)
+      hipLaunchKernelGGL(k_divb, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, b, d);
+    rc |= hip_check(hipGetLastError(), "k_divb");
+    unsigned long long h = 0;
+    rc |= d2h(c, &h, d, 8);
+    release(c, d);
+    *out = bits_dbl(h);
+    return rc;
+  }
+  static int con2prim_batch(hgpu_ctx* c, long long n, const double* U, const double* geom, double* prim,
+                            double* gamma, int* code, int* iters) {
+    if (n == 0) return 0;
+    Scope s(c, "con2prim");
+    hipLaunchKernelGGL(k_con2prim, dim3(nblocks(n)), dim3(BLOCK), 0, priv(c)->stream, c->P, n, U,
+                       reinterpret_cast<const Geom*>(geom), prim, gamma, code, iters);
+    return hip_check(hipGetLastError(), "k_con2prim");
+  }
+};
+
+}  // namespace hgpu
+
+using BE = hgpu::GpuBE;
+#include "hgpu_capi.inc"

@@ -1,0 +1,222 @@
+"""Host-side mirror of coolharm3d's step interface for the GPU library.
+
+:class:`Harm3D` owns one grid resident in HBM and exposes the reference's
+per-step operations under the reference's names:
+
+* ``step_ch()`` -- one full midpoint-RK2 step (``src/step_ch.c:50-234``);
+* ``advance(stage)`` -- one ``advance()`` stage with boundaries and fixups;
+* ``fixup_bounds()`` -- ``fixup(p); bounds(p,0)`` on initial data;
+* ``pflag()``, ``nfail()``, ``nfailtot()`` -- the failure maps ``fail()`` writes
+  (``src/diag.c:159-492``).
+
+Primitive arrays cross the boundary in the reference's layout
+``[i][j][k][var]`` over all zones including ``NG = 3`` ghosts.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from .problems import GridSpec, NG
+
+RECON_MC, RECON_PPM_MC = 2, 5
+FLUX_LF, FLUX_HLLE = 0, 1
+BC_CARTESIAN_OUTFLOW, BC_SPHERICAL_OUTFLOW3 = 0, 10
+N_NFAIL = 15
+NFAIL_NAMES = ["FLOOR", "TMAX", "GAMMA_MAX", "GAMMA_CALC", "USE_FULL_INV", "USE_EE", "UTOPRIM",
+               "UTOPRIM_EE", "USE_INTERP_PRIM", "USE_INTERP_V", "INTERP_PRIM", "INTERP_V",
+               "CUTOUT_PRESSURE", "CUTOUT_GAMMA", "CUTOUT_BC"]
+
+
+def risco_prograde(a: float) -> float:
+    """risco_calc(1), src/metric.c:458-472, with asq = a^2."""
+    t1, t2 = (1.0 + a) ** (1.0 / 3.0), (1.0 - a) ** (1.0 / 3.0)
+    z1 = 1.0 + t1 * t2 * (t1 + t2)
+    z2 = math.sqrt(3.0 * a * a + z1 * z1)
+    return 3.0 + z2 - math.sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2))
+
+
+def config_for(grid: GridSpec, **kw) -> _lib.HgpuConfig:
+    """hgpu_config for a GridSpec.  Defaults follow the two problem families:
+    Minkowski/identity -> MC, Toth CT, Cartesian outflow, no cooling;
+    Kerr-Schild/MKS -> PPM+MC, PPM EMFs, spherical outflow, h/r cooling."""
+    c = _lib.HgpuConfig()
+    c.n1, c.n2, c.n3 = grid.n1, grid.n2, grid.n3
+    ks = grid.metric == "kerr_schild"
+    c.metric = 2 if ks else 0
+    c.coords = 1 if grid.coords == "mks" else 0
+    c.recon = kw.get("recon", RECON_PPM_MC if ks else RECON_MC)
+    c.flux = kw.get("flux", FLUX_HLLE)
+    c.ct_para = kw.get("ct_para", 1 if ks else 0)
+    c.cooling = kw.get("cooling", 1 if ks else 0)
+    c.rescale_r = kw.get("rescale_r", 1 if ks else 0)
+    c.bc = kw.get("bc", BC_SPHERICAL_OUTFLOW3 if ks else BC_CARTESIAN_OUTFLOW)
+    c.track_nfail = kw.get("track_nfail", 1)
+    for l in range(4):
+        c.startx[l] = grid.startx[l]
+        c.dx[l] = grid.dx[l]
+    c.gam, c.cour = grid.gam, grid.cour
+    c.t_final = grid.t_end + grid.startx[0]
+    c.a, c.R0, c.h_slope = grid.a, grid.R0, grid.h_slope
+    c.th_beg, c.th_length = grid.th_beg, grid.th_length
+    c.r_isco = kw.get("r_isco", risco_prograde(grid.a) if ks else 0.0)
+    c.h_o_r = kw.get("h_o_r", 0.1 if ks else 0.05)
+    if ks:
+        c.rhomin, c.uumin = kw.get("rhomin", 1.0e-7), kw.get("uumin", 1.0e-9)
+    else:
+        c.rhomin, c.uumin = kw.get("rhomin", 2.0e-10), kw.get("uumin", 2.0e-12)
+    c.rhopower, c.uupower = kw.get("rhopower", -1.5), kw.get("uupower", -2.5)
+    return c
+
+
+def _dp(a):
+    return a.ctypes.data_as(_lib.c_double_p)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_lib.c_int_p)
+
+
+class Harm3D:
+    """One grid of the per-substep GRMHD advance resident on the GPU."""
+
+    def __init__(self, grid: GridSpec, hostcheck: bool = False, **kw):
+        self.grid = grid
+        self.lib = _lib.load(hostcheck=hostcheck)
+        self.cfg = config_for(grid, **kw)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib, self.lib.hgpu_create(ctypes.byref(self.cfg), ctypes.byref(h)), "hgpu_create")
+        self.h = h
+        d = np.zeros(7, np.int32)
+        self.lib.hgpu_dims(self.h, _ip(d))
+        self.shape = (int(d[3]), int(d[4]), int(d[5]))
+        self.nt = self.shape[0] * self.shape[1] * self.shape[2]
+        self.nstep = 0
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.hgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        _lib.check(self.lib, rc, what)
+
+    # -- state ---------------------------------------------------------------
+    def set_prims(self, p, which=0):
+        a = np.ascontiguousarray(p, dtype=np.float64)
+        if a.shape != self.shape + (8,):
+            raise ValueError(f"prims must have shape {self.shape + (8,)}, got {a.shape}")
+        self._chk(self.lib.hgpu_upload_prims(self.h, which, _dp(a)), "hgpu_upload_prims")
+
+    def prims(self, which=0):
+        a = np.empty(self.shape + (8,))
+        self._chk(self.lib.hgpu_download_prims(self.h, which, _dp(a)), "hgpu_download_prims")
+        return a
+
+    def set_time(self, t, dt):
+        self._chk(self.lib.hgpu_set_time(self.h, t, dt), "hgpu_set_time")
+
+    def time(self):
+        t, dt, dto = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._chk(self.lib.hgpu_get_time(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(dto)),
+                  "hgpu_get_time")
+        return t.value, dt.value, dto.value
+
+    def fixup_bounds(self):
+        self._chk(self.lib.hgpu_fixup_bounds(self.h), "hgpu_fixup_bounds")
+
+    def step_ch(self, nsteps=1):
+        self._chk(self.lib.hgpu_step(self.h, nsteps), "hgpu_step")
+        self.nstep += nsteps
+
+    def advance(self, stage):
+        self._chk(self.lib.hgpu_substep(self.h, stage), "hgpu_substep")
+
+    def sync(self):
+        self._chk(self.lib.hgpu_sync(self.h), "hgpu_sync")
+
+    # -- failure bookkeeping -------------------------------------------------
+    def pflag(self):
+        a = np.empty(self.shape, np.int32)
+        self._chk(self.lib.hgpu_download_pflag(self.h, _ip(a)), "hgpu_download_pflag")
+        return a
+
+    def gamma(self):
+        a = np.empty(self.shape)
+        self._chk(self.lib.hgpu_download_gamma(self.h, _dp(a)), "hgpu_download_gamma")
+        return a
+
+    def nfail(self):
+        a = np.empty((N_NFAIL,) + self.shape, np.int32)
+        self._chk(self.lib.hgpu_download_nfail(self.h, _ip(a)), "hgpu_download_nfail")
+        return a
+
+    def nfailtot(self):
+        a = np.zeros(N_NFAIL, np.int64)
+        self._chk(self.lib.hgpu_get_nfailtot(self.h, a.ctypes.data_as(_lib.c_ll_p)), "hgpu_get_nfailtot")
+        return a
+
+    def reset_nfail(self):
+        self._chk(self.lib.hgpu_reset_nfail(self.h), "hgpu_reset_nfail")
+
+    def status(self):
+        fe = ctypes.c_int()
+        v = np.zeros(4)
+        self._chk(self.lib.hgpu_get_status(self.h, ctypes.byref(fe), _dp(v)), "hgpu_get_status")
+        return fe.value, v
+
+    def fluxes(self):
+        a = np.empty((3, 9) + self.shape)
+        self._chk(self.lib.hgpu_download_fluxes(self.h, _dp(a)), "hgpu_download_fluxes")
+        return a
+
+    def divb_max(self):
+        v = ctypes.c_double()
+        self._chk(self.lib.hgpu_divb_max(self.h, ctypes.byref(v)), "hgpu_divb_max")
+        return v.value
+
+    # -- geometry --------------------------------------------------------------
+    def geometry_counts(self):
+        ng, nc = ctypes.c_longlong(), ctypes.c_longlong()
+        self._chk(self.lib.hgpu_geometry_counts(self.h, ctypes.byref(ng), ctypes.byref(nc)), "counts")
+        return ng.value, nc.value
+
+    def geometry(self):
+        ng, nc = self.geometry_counts()
+        g = np.empty((ng, 42))
+        c = np.empty((nc, 4, 4, 4))
+        self._chk(self.lib.hgpu_download_geometry(self.h, _dp(g), _dp(c)), "hgpu_download_geometry")
+        return g, c
+
+    def set_geometry(self, geom=None, conn=None):
+        ng, nc = self.geometry_counts()
+        gp = cp = None
+        if geom is not None:
+            geom = np.ascontiguousarray(geom, np.float64)
+            assert geom.size == ng * 42
+            gp = _dp(geom)
+        if conn is not None:
+            conn = np.ascontiguousarray(conn, np.float64)
+            assert conn.size == nc * 64
+            cp = _dp(conn)
+        self._chk(self.lib.hgpu_upload_geometry(self.h, gp, cp), "hgpu_upload_geometry")
+
+    # -- timing ----------------------------------------------------------------
+    def profile_enable(self, on=True):
+        self._chk(self.lib.hgpu_profile_enable(self.h, 1 if on else 0), "hgpu_profile_enable")
+
+    def profile_read(self, name):
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        self._chk(self.lib.hgpu_profile_read(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)),
+                  "hgpu_profile_read")
+        return ms.value, n.value

@@ -1,0 +1,128 @@
+/*
+ * hgpu.h -- C ABI of the MI355X GRMHD substep library (libhgpu.so).
+ *
+ * Drop-in boundary for coolharm3d's per-step advance: the reference driver
+ * calls step_ch() (src/globals.h:898, src/main.c:81), which reads and writes the
+ * primitive state p[][][][] and the failure bookkeeping through process
+ * globals.  This library keeps the state resident in HBM and exposes the same
+ * step as hgpu_step(); a C shim for the reference tree maps
+ * step_ch() onto these calls.  All entry points are plain C: pointers, sizes,
+ * int status (0 = ok, < 0 = error, message in hgpu_last_error()).  Numerical
+ * failures are never returned as errors: they are recorded in the pflag/nfail
+ * maps exactly as fail() does (src/diag.c:159-492).
+ *
+ * Array layout at the boundary is the reference's own: all zones including
+ * NG = 3 ghost layers, ordering [i][j][k][var] with var fastest (8 doubles per
+ * zone), i.e. what the pointer trees p[i][j][k][l] hold (src/decomp.c:148-175).
+ */
+#ifndef HGPU_H
+#define HGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Run configuration: the compile-time switches of src/options.h the advance
+ * honours, as run-time values, and the grid parameters of init_base(). */
+typedef struct hgpu_config {
+  int n1, n2, n3;          /* interior zones N1, N2, N3 (src/harm_mpi.c:1470-1472)          */
+  int metric;              /* 0: METRIC_MINK_CARTESIAN, 2: METRIC_KS_SPHERICAL (src/options.h:73-83) */
+  int coords;              /* 0: COORD_IDENTITY, 1: COORD_DIAGONAL (src/options.h:136-147)           */
+  int recon;               /* 2: RECON_MC, 5: RECON_PPM_MC (src/options.h:151-157)                   */
+  int flux;                /* 0: FLUX_LF, 1: FLUX_HLLE (src/options.h:169-172)                        */
+  int ct_para;             /* USE_FLUX_CT_PARA (src/options.h:165)                                    */
+  int cooling;             /* USE_COOLING_FUNCTION == 1 (src/options.h:194)                           */
+  int rescale_r;           /* RESCALE_R (src/options.h:267)                                           */
+  int bc;                  /* 0: BC_CARTESIAN_OUTFLOW, 10: BC_SPHERICAL_OUTFLOW3 (src/options.h:108-132) */
+  int track_nfail;         /* keep the per-zone nfail[15][] counters (src/globals.h:310)              */
+  double startx[4];        /* startx[] incl. the -NG*dx ghost offset (src/init.torus.c:100-110)       */
+  double dx[4];            /* dx[1..3]; dx[0] is the initial time step                                */
+  double gam;              /* adiabatic index                                                          */
+  double cour;             /* Courant factor                                                           */
+  double t_final;          /* GridLength[0] + startx[0] (timestep clamp, src/step_ch.c:615)           */
+  double a, R0, h_slope, th_beg, th_length;  /* spin and coordinate parameters (src/init.torus.c:174-200) */
+  double r_isco;           /* ISCO radius used by the cooling function (src/phys.c:815-845)           */
+  double h_o_r;            /* cooling target H/R (src/phys.c:655)                                     */
+  double rhomin, uumin, rhopower, uupower;  /* floor model (src/options.h:232-241)                 */
+} hgpu_config;
+
+typedef struct hgpu_ctx hgpu_ctx;
+
+/* Returns the last error message of the calling thread ("" if none). */
+const char *hgpu_last_error(void);
+
+/* Allocates the device state for one grid and computes its static geometry
+ * (calc_all_geom(), src/metric.c:4111) and the x1 tables of cooling, radial
+ * rescaling and floors.  Replaces set_arrays()+init_base() for the device. */
+int hgpu_create(const hgpu_config *cfg, hgpu_ctx **out);
+int hgpu_destroy(hgpu_ctx *ctx);
+
+/* Grid sizes: out[0..6] = N1, N2, N3, N1TOT, N2TOT, N3TOT, NG */
+int hgpu_dims(const hgpu_ctx *ctx, int *out);
+
+/* Primitive state transfer, which = 0: p, 1: ph.  Host layout [i][j][k][8]. */
+int hgpu_upload_prims(hgpu_ctx *ctx, int which, const double *host);
+int hgpu_download_prims(hgpu_ctx *ctx, int which, double *host);
+
+/* Replaces the computed geometry by the host's geom_arr / conn records
+ * (struct of_geom layout, src/globals.h:90-98; conn[N_CONN][4][4][4]).
+ * n_geom / n_conn must equal the counts hgpu_geometry_counts() reports. */
+int hgpu_geometry_counts(const hgpu_ctx *ctx, long long *n_geom, long long *n_conn);
+int hgpu_upload_geometry(hgpu_ctx *ctx, const double *geom, const double *conn);
+int hgpu_download_geometry(hgpu_ctx *ctx, double *geom, double *conn);
+
+/* Integrator clock: t and dt (= dx[0]), and dt_old. */
+int hgpu_set_time(hgpu_ctx *ctx, double t, double dt);
+int hgpu_get_time(hgpu_ctx *ctx, double *t, double *dt, double *dt_old);
+
+/* fixup(p) + bounds(p,0) of the initial data (src/fixup.c:8, src/bounds.c:31). */
+int hgpu_fixup_bounds(hgpu_ctx *ctx);
+
+/* step_ch() (src/step_ch.c:50-234) repeated nsteps times: predictor and
+ * corrector advance() with boundaries and fixups, then t += dt and the new
+ * dt = timestep().  Enqueued on the library stream; returns without waiting. */
+int hgpu_step(hgpu_ctx *ctx, int nsteps);
+
+/* One advance() stage (0: predictor into ph, 1: corrector into p) followed by
+ * bounds and fixups, as step_ch() sequences them (src/step_ch.c:92-180). */
+int hgpu_substep(hgpu_ctx *ctx, int stage);
+
+/* Failure bookkeeping: pflag[NT] (src/macros.h:256-259), p_gamma[NT],
+ * nfail[15][NT] (src/macros.h:236-251), nfailtot[15]. */
+int hgpu_download_pflag(hgpu_ctx *ctx, int *host);
+int hgpu_download_gamma(hgpu_ctx *ctx, double *host);
+int hgpu_download_nfail(hgpu_ctx *ctx, int *host);
+int hgpu_get_nfailtot(hgpu_ctx *ctx, long long *out15);
+int hgpu_reset_nfail(hgpu_ctx *ctx);
+
+/* failure_exists and max_char_speed[1..3] of the last numerical_flux(). */
+int hgpu_get_status(hgpu_ctx *ctx, int *failure_exists, double *vmax4);
+
+/* Face fluxes of the last advance(): F[3][9][NT] (8 conserved + entropy). */
+int hgpu_download_fluxes(hgpu_ctx *ctx, double *host);
+
+/* Maximum |div B| at zone corners with the constrained-transport stencil
+ * (divb_calc(), src/diag.c:503-602) over the interior corners, from p. */
+int hgpu_divb_max(hgpu_ctx *ctx, double *out);
+
+/* Waits for all work enqueued by this context. */
+int hgpu_sync(hgpu_ctx *ctx);
+
+/* Kernel timing with HIP events on the library stream.  Names:
+ * "flux", "emf", "update", "bounds", "fixup", "step". */
+int hgpu_profile_enable(hgpu_ctx *ctx, int enable);
+int hgpu_profile_read(hgpu_ctx *ctx, const char *name, double *total_ms, long long *count);
+
+/* Batched con-to-prim of independent zones (Utoprim_2d_fast(),
+ * src/utoprim_2d_fast.c:135, called by recover_primitives(), src/step_ch.c:832)
+ * on device buffers laid out structure-of-arrays: U[8][n], geometry records
+ * geom[n] (struct of_geom), prim[8][n] (guess in, solution out), gamma[n],
+ * code[n] (return code), iters[n] (Newton iterations).  Pointers are device
+ * addresses; the call is enqueued on the library stream. */
+int hgpu_con2prim_batch(hgpu_ctx *ctx, long long n, const double *U, const double *geom,
+                        double *prim, double *gamma, int *code, int *iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HGPU_H */
