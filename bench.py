@@ -42,6 +42,19 @@ HBM_PEAK_GBS = 8000.0
 ALGO_BYTES = {"flux": 136.0, "emf": 72.0, "update": 0.5 * (376.0 + 440.0)}
 
 
+def measured_traffic(workload):
+    """HBM bytes per dispatch of each kernel, from the committed counter summary
+    profiles/hbm_traffic_<workload>.md (two rocprofv3 --pmc runs, FETCH_SIZE and
+    WRITE_SIZE, calibrated on a plain 1 GiB device copy); None when absent."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        if ln.startswith("hbm_bytes_per_dispatch "):
+            return json.loads(ln[len("hbm_bytes_per_dispatch "):])
+    return None
+
+
 def units_per_launch(n1, n2, n3):
     ng = 3
     n1t, n2t, n3t = n1 + 2 * ng, n2 + 2 * ng, n3 + 2 * ng
@@ -50,17 +63,25 @@ def units_per_launch(n1, n2, n3):
 
 
 def cpu_worker(args):
-    """One reference process on an x3 slab: prints zone-cycles and seconds."""
+    """One CPU process on an x3 slab (the reference build, or the oracle when
+    the reference build is absent): prints zone-cycles and seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from refwrap import RefRun, torus_params
     from coolharm3d_amd.problems import torus_grid, torus
     n1, n2, n3 = args.n1, args.n2, args.slab
     g = torus_grid(n1, n2, n3, x3_extent=0.5 * math.pi * n3 / args.n3)
-    R = RefRun("c1", n1, n2, n3, params=torus_params(g))
-    R.set_prims(torus(g, seed=1234 + args.worker))
-    R.fixup_bounds()
-    R.set_time(0.0, 1.0e-5, g.cour)
-    R.step(1)  # first step also builds the reference's cooling tables
+    if args.cpu_kind == "reference":
+        from refwrap import RefRun, torus_params
+        R = RefRun("c1", n1, n2, n3, params=torus_params(g))
+        R.set_prims(torus(g, seed=1234 + args.worker))
+        R.fixup_bounds()
+        R.set_time(0.0, 1.0e-5, g.cour)
+    else:
+        from pyoracle import OracleRun
+        R = OracleRun(g)
+        R.set_prims(torus(g, seed=1234 + args.worker))
+        R.fixup_bounds()
+        R.set_time(0.0, 1.0e-5)
+    R.step(1)  # untimed first step (the reference also builds its cooling tables here)
     t0 = time.perf_counter()
     R.step(args.slab_steps)
     el = time.perf_counter() - t0
@@ -72,14 +93,17 @@ def cpu_worker(args):
 def cpu_baseline(args):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from refwrap import have_ref
-    if not have_ref("c1"):
-        return {"value": None, "unit": "zone-cycles/s", "cores": 0, "kind": "reference",
-                "sample": "oracle/_ref/c1 not built on this machine"}
+    from pyoracle import have_oracle
+    kind = "reference" if have_ref("c1") else "port"
+    if kind == "port" and not have_oracle():
+        return {"value": None, "unit": "zone-cycles/s", "cores": 0, "kind": kind,
+                "sample": "neither oracle/_ref/c1 nor oracle/_build/liboracle.so is built on this machine"}
     ncores = min(16, os.cpu_count() or 1)
     slab = 4
     procs = []
     for w in range(ncores):
         cmd = [sys.executable, os.path.abspath(__file__), "--_cpu-worker", "--worker", str(w),
+               "--cpu-kind", kind,
                "--n1", str(args.n1), "--n2", str(args.n2), "--n3", str(args.n3),
                "--slab", str(slab), "--slab-steps", str(args.cpu_steps)]
         procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
@@ -92,8 +116,9 @@ def cpu_baseline(args):
                 r = json.loads(line)
                 rate += r["zc"] / r["s"]
                 n_ok += 1
-    return {"value": rate if n_ok else None, "unit": "zone-cycles/s", "cores": n_ok, "kind": "reference",
-            "sample": (f"reference step_ch() (gcc -O2), {n_ok} independent processes, each a "
+    what = "reference step_ch() (gcc -O2)" if kind == "reference" else "oracle restatement of step_ch() (gcc -O2)"
+    return {"value": rate if n_ok else None, "unit": "zone-cycles/s", "cores": n_ok, "kind": kind,
+            "sample": (f"{what}, {n_ok} independent processes, each a "
                        f"{args.n1}x{args.n2}x{slab} x3-slab of the {args.n1}x{args.n2}x{args.n3} torus, "
                        f"{args.cpu_steps} timed steps after 1 untimed; rates summed (no halo exchange: "
                        f"an upper bound on an MPI run)")}
@@ -113,6 +138,10 @@ def main():
     ap.add_argument("--worker", type=int, default=0)
     ap.add_argument("--slab", type=int, default=4)
     ap.add_argument("--slab-steps", type=int, default=4)
+    ap.add_argument("--cpu-kind", choices=["reference", "port"], default="reference")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="launch one plain device copy of this many MiB before the run "
+                         "(calibrates the HBM byte counters of a profiler run)")
     args = ap.parse_args()
 
     if args.cpu_worker:
@@ -128,6 +157,14 @@ def main():
     from coolharm3d_amd import Harm3D, torus_grid, torus
 
     n1, n2, n3 = args.n1, args.n2, args.n3
+    if args.copy_probe_mib > 0:
+        import torch
+        nel = args.copy_probe_mib * (1 << 20) // 8
+        a = torch.ones(nel, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     g = torus_grid(n1, n2, n3)
     H = Harm3D(g, track_nfail=1)
     H.set_prims(torus(g))
@@ -157,8 +194,12 @@ def main():
     dom = max(("flux", "emf", "update"), key=lambda k: kern[k][0])
     avg_s = kern[dom][0] / max(kern[dom][1], 1) / 1e3
     achieved = ALGO_BYTES[dom] * units[dom] / avg_s / 1e9
+    workload = f"torus{n1}x{n2}x{n3}_fp64"
+    traffic_tab = measured_traffic(workload)
+    traffic = traffic_tab.get("k_" + dom) if traffic_tab else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                "traffic_source": f"profiles/hbm_traffic_{workload}.md" if traffic else None,
                 "avg_launch_ms": avg_s * 1e3, "algo_bytes_per_launch": ALGO_BYTES[dom] * units[dom],
                 "kernel_ms_per_step": {k: kern[k][0] / args.steps for k in ("flux", "emf", "update", "bounds", "fixup")},
                 "state_roofline_frac": value * 320.0 / (HBM_PEAK_GBS * 1e9)}

@@ -1,0 +1,185 @@
+/*
+ * step_ch_hgpu.c -- drop-in step_ch() for the coolharm3d C tree that runs the
+ * per-step advance on the GPU through libhgpu.so (include/hgpu.h).
+ *
+ * The reference driver calls step_ch() once per time step (src/main.c:81)
+ * and reads the state and the failure bookkeeping through process globals.
+ * This file provides that step_ch() with the same observable effect on the
+ * globals it writes (src/step_ch.c:50-234): p, p_gamma, pflag, nfail,
+ * nfailtot, failure_exists, t, t_old, dx[0], dt_old.
+ *
+ * Build, inside the reference's own build (no other source changes):
+ *   - compile src/step_ch.c with -Dstep_ch=step_ch_cpu (keeps advance(),
+ *     check_for_crash() and the rest of that file; the CPU step is renamed);
+ *   - compile this file with -I<reference src> -I<this repository>/include;
+ *   - link with -L<this repository>/coolharm3d_amd -lhgpu.
+ *
+ * On its first call the shim creates the device context from the run's
+ * globals and options, uploads the geometry the reference computed
+ * (geom_arr[0] and conn, so the device uses the reference's own metric and
+ * connection) and the primitive state.  After every step it copies back the
+ * clock, failure_exists and nfailtot (check_for_crash() reads them,
+ * src/step_ch.c:1244) and, with HGPU_HOST_SYNC = 1 (default), the state and
+ * maps.  With HGPU_HOST_SYNC = 0 the host arrays are refreshed only by
+ * step_ch_hgpu_sync_host(), which the caller invokes before diag()/dump().
+ * After the host edits p (restart, re-initialisation), step_ch_hgpu_upload()
+ * sends it to the device again.
+ *
+ * Compile-time inputs the reference keeps inside function bodies:
+ *   HGPU_H_O_R   cooling target H/R of cooling_func_hr_disk() (src/phys.c:655).
+ */
+#include "decs.h"
+#include "hgpu.h"
+
+#ifndef HGPU_H_O_R
+#define HGPU_H_O_R (5.e-2)
+#endif
+#ifndef HGPU_HOST_SYNC
+#define HGPU_HOST_SYNC (1)
+#endif
+
+void step_ch_hgpu_sync_host(void);
+void step_ch_hgpu_upload(void);
+
+static hgpu_ctx *hg = NULL;
+static double *hg_stage = NULL;        /* [N1TOT][N2TOT][N3TOT][8] */
+static int *hg_istage = NULL;          /* [15][N1TOT][N2TOT][N3TOT] */
+static int *hg_nfail_base = NULL;      /* nfail counts recorded before the first step */
+static ulint hg_nfailtot_base[N_NFAIL];
+
+static void hg_check(int rc, const char *what) {
+  if (rc) {
+    fprintf(stderr, "step_ch_hgpu: %s failed (%d): %s\n", what, rc, hgpu_last_error());
+    fflush(stderr);
+    myexit(1);
+  }
+}
+
+static long hg_ncell(void) { return (long)N1TOT * N2TOT * N3TOT; }
+
+static void hg_setup(void) {
+  int i, j, k, l, m0, m1, m2;   /* (a is the global spin, src/globals.h:668) */
+  hgpu_config cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.n1 = N1; cfg.n2 = N2; cfg.n3 = N3;
+#if( METRIC_TYPE_CHOICE == METRIC_MINK_CARTESIAN )
+  cfg.metric = 0;
+#elif( METRIC_TYPE_CHOICE == METRIC_KS_SPHERICAL )
+  cfg.metric = 2;
+#else
+  fprintf(stderr, "step_ch_hgpu: METRIC_TYPE_CHOICE %d is not supported on the GPU\n", METRIC_TYPE_CHOICE);
+  myexit(1);
+#endif
+#if( COORD_TYPE_CHOICE == COORD_IDENTITY )
+  cfg.coords = 0;
+#elif( COORD_TYPE_CHOICE == COORD_DIAGONAL )
+  cfg.coords = 1;
+#else
+  fprintf(stderr, "step_ch_hgpu: COORD_TYPE_CHOICE %d is not supported on the GPU\n", COORD_TYPE_CHOICE);
+  myexit(1);
+#endif
+  cfg.recon = RECON_TYPE_CHOICE;
+  cfg.flux = FLUX_TYPE_CHOICE;
+  cfg.ct_para = USE_FLUX_CT_PARA;
+  cfg.cooling = (USE_COOLING_FUNCTION == 1);
+  cfg.rescale_r = RESCALE_R;
+  cfg.bc = BC_TYPE_CHOICE;
+  cfg.track_nfail = 1;
+  for (l = 0; l < NDIM; l++) { cfg.startx[l] = startx[l]; cfg.dx[l] = dx[l]; }
+  cfg.gam = gam;
+  cfg.cour = cour;
+  cfg.t_final = GridLength[0] + startx[0];
+  cfg.a = a; cfg.R0 = R0; cfg.h_slope = h_slope; cfg.th_beg = th_beg; cfg.th_length = th_length;
+  cfg.r_isco = r_isco;
+  cfg.h_o_r = HGPU_H_O_R;
+  cfg.rhomin = RHOMIN; cfg.uumin = UUMIN;
+#if( FLOOR_DIM == 0 )
+  cfg.rhopower = 0.; cfg.uupower = 0.;
+#else
+  cfg.rhopower = RHOPOWER; cfg.uupower = UUPOWER;
+#endif
+  if (getenv("HGPU_SHIM_VERBOSE")) {
+    fprintf(stdout, "step_ch_hgpu: n=%d,%d,%d metric %d coords %d recon %d flux %d ct_para %d cooling %d "
+            "rescale_r %d bc %d gam %.17g cour %.17g t_final %.17g a %.17g R0 %.17g h_slope %.17g "
+            "th_beg %.17g th_length %.17g r_isco %.17g h_o_r %.17g floors %g %g %g %g\n",
+            cfg.n1, cfg.n2, cfg.n3, cfg.metric, cfg.coords, cfg.recon, cfg.flux, cfg.ct_para, cfg.cooling,
+            cfg.rescale_r, cfg.bc, cfg.gam, cfg.cour, cfg.t_final, cfg.a, cfg.R0, cfg.h_slope, cfg.th_beg,
+            cfg.th_length, cfg.r_isco, cfg.h_o_r, cfg.rhomin, cfg.uumin, cfg.rhopower, cfg.uupower);
+    fflush(stdout);
+  }
+  hg_check(hgpu_create(&cfg, &hg), "hgpu_create");
+
+  hg_stage = (double *)malloc(sizeof(double) * NP * hg_ncell());
+  hg_istage = (int *)malloc(sizeof(int) * N_NFAIL * hg_ncell());
+  hg_nfail_base = (int *)malloc(sizeof(int) * N_NFAIL * hg_ncell());
+  if (!hg_stage || !hg_istage || !hg_nfail_base) { fprintf(stderr, "step_ch_hgpu: out of host memory\n"); myexit(1); }
+
+  /* the reference's geometry records and connection */
+  {
+    long long ng, nc;
+    hg_check(hgpu_geometry_counts(hg, &ng, &nc), "hgpu_geometry_counts");
+    if (ng != (long long)N_GEOM || nc != (long long)N_CONN) {
+      fprintf(stderr, "step_ch_hgpu: geometry counts differ (%lld/%d, %lld/%d)\n", ng, N_GEOM, nc, N_CONN);
+      myexit(1);
+    }
+    double *cn = (double *)malloc(sizeof(double) * 64 * N_CONN);
+    for (l = 0; l < N_CONN; l++) for (m0 = 0; m0 < NDIM; m0++) for (m1 = 0; m1 < NDIM; m1++) for (m2 = 0; m2 < NDIM; m2++)
+      cn[((l * 4 + m0) * 4 + m1) * 4 + m2] = conn[l][m0][m1][m2];
+    hg_check(hgpu_upload_geometry(hg, (const double *)geom_arr[0], cn), "hgpu_upload_geometry");
+    free(cn);
+  }
+  for (l = 0; l < N_NFAIL; l++) {
+    hg_nfailtot_base[l] = nfailtot[l];
+    ALL_LOOP hg_nfail_base[(long)l * hg_ncell() + ((long)i * N2TOT + j) * N3TOT + k] = nfail[l][i][j][k];
+  }
+  step_ch_hgpu_upload();
+}
+
+/* Sends the host primitive state and clock to the device. */
+void step_ch_hgpu_upload(void) {
+  int i, j, k, l;
+  long c = 0;
+  ALL_LOOP PLOOP hg_stage[c++] = p[i][j][k][l];
+  hg_check(hgpu_upload_prims(hg, 0, hg_stage), "hgpu_upload_prims");
+  hg_check(hgpu_set_time(hg, t, dx[0]), "hgpu_set_time");
+}
+
+/* Copies the device state and failure maps into the host arrays. */
+void step_ch_hgpu_sync_host(void) {
+  int i, j, k, l;
+  long c = 0, n = hg_ncell();
+  if (!hg) return;
+  hg_check(hgpu_download_prims(hg, 0, hg_stage), "hgpu_download_prims");
+  ALL_LOOP PLOOP p[i][j][k][l] = hg_stage[c++];
+  hg_check(hgpu_download_gamma(hg, hg_stage), "hgpu_download_gamma");
+  c = 0;
+  ALL_LOOP p_gamma[i][j][k] = hg_stage[c++];
+  hg_check(hgpu_download_pflag(hg, hg_istage), "hgpu_download_pflag");
+  c = 0;
+  ALL_LOOP pflag[i][j][k] = hg_istage[c++];
+  hg_check(hgpu_download_nfail(hg, hg_istage), "hgpu_download_nfail");
+  for (l = 0; l < N_NFAIL; l++) {
+    c = (long)l * n;
+    ALL_LOOP { nfail[l][i][j][k] = hg_nfail_base[c] + hg_istage[c]; c++; }
+  }
+}
+
+void step_ch(void) {
+  double tt, dtt, dto, vmax[4];
+  long long nft[N_NFAIL];
+  int fe, l;
+  if (!hg) hg_setup();
+  hg_check(hgpu_step(hg, 1), "hgpu_step");
+  hg_check(hgpu_get_time(hg, &tt, &dtt, &dto), "hgpu_get_time");
+  t_old = t;
+  t = tt;
+  dt_old = dto;
+  dx[0] = dtt;
+  hg_check(hgpu_get_status(hg, &fe, vmax), "hgpu_get_status");
+  failure_exists = fe;
+  hg_check(hgpu_get_nfailtot(hg, nft), "hgpu_get_nfailtot");
+  for (l = 0; l < N_NFAIL; l++) nfailtot[l] = hg_nfailtot_base[l] + (ulint)nft[l];
+#if( HGPU_HOST_SYNC )
+  step_ch_hgpu_sync_host();
+#endif
+}
