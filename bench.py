@@ -11,12 +11,18 @@ CFL time step.  value = interior zones x steps / wall time of the timed steps
 resident in HBM.
 
 Also reported:
-  roofline      -- the dominant kernel's algorithmic HBM bytes per launch over
-                   its average launch time (HIP events on the library stream,
-                   during the timed steps), against 8 TB/s;
-  cpu_baseline  -- the reference C (oracle/_ref/c1, built from the reference
-                   sources) on the host cores: independent processes on x3
-                   slabs of the same grid, timed around step_ch() only.
+  roofline          -- the dominant kernel's algorithmic HBM bytes per launch
+                       over its average launch time (HIP events on the library
+                       stream, during the timed steps), against 8 TB/s, and the
+                       measured HBM bytes per launch (counter summary in profiles/);
+  mixed_precision   -- the same grid and steps with FP32 reconstruction, face
+                       fluxes and EMFs (FP64 update, con-to-prim, metric, cooling),
+                       and its speed-up over the FP64 step;
+  precision_variant -- per-primitive relative L1 of the mixed step against the
+                       FP64 step on the 256x256x192 torus after 1, 10, 100 steps;
+  cpu_baseline      -- the reference C (oracle/_ref/c1, built from the reference
+                       sources) on the host cores: independent processes on x3
+                       slabs of the same grid, timed around step_ch() only.
 """
 from __future__ import annotations
 
@@ -40,6 +46,9 @@ HBM_PEAK_GBS = 8000.0
 #   update : per zone   -- p read (64 B), 27 face values + 3 EMFs read (240 B), result written (64 B),
 #                          gamma written (8 B); the corrector also reads ph (64 B): 376 / 440 B
 ALGO_BYTES = {"flux": 136.0, "emf": 72.0, "update": 0.5 * (376.0 + 440.0)}
+# mixed precision: face values and EMFs are FP32 (flux writes 36 B/face, EMF 24+12 B/edge,
+# update reads 27 face values + 3 EMFs at 4 B)
+ALGO_BYTES_MIXED = {"flux": 100.0, "emf": 36.0, "update": 0.5 * (376.0 + 440.0) - 120.0}
 
 
 def measured_traffic(workload):
@@ -124,6 +133,60 @@ def cpu_baseline(args):
                        f"an upper bound on an MPI run)")}
 
 
+def timed_run(args, g, p0, precision):
+    """Warm-up + timed step_ch() of one grid; returns timings and per-kernel figures."""
+    from coolharm3d_amd import Harm3D
+    H = Harm3D(g, track_nfail=1, precision=precision)
+    H.set_prims(p0)
+    H.fixup_bounds()
+    H.set_time(0.0, 1.0e-5)
+    H.step_ch(args.warmup)
+    H.sync()
+    H.profile_enable(True)
+    H.sync()
+    t0 = time.perf_counter()
+    H.step_ch(args.steps)
+    H.sync()
+    el = time.perf_counter() - t0
+    kern = {nm: H.profile_read(nm) for nm in ("flux", "emf", "update", "bounds", "fixup", "step")}
+    H.profile_enable(False)
+    import numpy as np
+    finite = bool(np.isfinite(H.prims()).all())
+    H.close()
+    return el, kern, finite
+
+
+def precision_report(shape, steps=(1, 10, 100)):
+    """Mixed-precision variant vs the FP64 step on the same torus: per-primitive
+    relative L1 (interior zones) after each of `steps`."""
+    import numpy as np
+    from coolharm3d_amd import Harm3D, torus_grid, torus
+    g = torus_grid(*shape)
+    p0 = torus(g)
+    states = {}
+    for prec in ("fp64", "mixed"):
+        H = Harm3D(g, track_nfail=0, precision=prec)
+        H.set_prims(p0)
+        H.fixup_bounds()
+        H.set_time(0.0, 1.0e-5)
+        done, out = 0, {}
+        for s in steps:
+            H.step_ch(s - done)
+            done = s
+            out[s] = H.prims()[3:-3, 3:-3, 3:-3].reshape(-1, 8)
+        states[prec] = out
+        H.close()
+    names = ["rho", "u", "u1", "u2", "u3", "B1", "B2", "B3"]
+    rep = {}
+    for s in steps:
+        a, b = states["fp64"][s], states["mixed"][s]
+        num = np.abs(b - a).sum(axis=0)
+        den = np.abs(a).sum(axis=0)
+        rep[str(s)] = {names[v]: float(num[v] / den[v]) if den[v] > 0 else float(num[v]) for v in range(8)}
+    return {"grid": list(shape), "steps": list(steps), "rel_l1_vs_fp64": rep,
+            "reconstruction_flux_emf": "fp32", "update_con2prim_metric_cooling": "fp64"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +195,13 @@ def main():
     ap.add_argument("--n1", type=int, default=192)
     ap.add_argument("--n2", type=int, default=192)
     ap.add_argument("--n3", type=int, default=128)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="precision of the headline run (default: the plain FP64 step)")
+    ap.add_argument("--mixed", choices=["auto", "off"], default="auto",
+                    help="also time the mixed-precision step on the same grid")
+    ap.add_argument("--precision-report", choices=["auto", "off"], default="auto",
+                    help="per-primitive L1 of the mixed step vs FP64 on the 256x256x192 torus after 1, 10, 100 steps")
+    ap.add_argument("--report-grid", default="256x256x192")
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--_cpu-worker", action="store_true", dest="cpu_worker")
@@ -153,8 +223,7 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
 
-    import numpy as np
-    from coolharm3d_amd import Harm3D, torus_grid, torus
+    from coolharm3d_amd import torus_grid, torus
 
     n1, n2, n3 = args.n1, args.n2, args.n3
     if args.copy_probe_mib > 0:
@@ -166,43 +235,40 @@ def main():
         torch.cuda.synchronize()
         del a, b
     g = torus_grid(n1, n2, n3)
-    H = Harm3D(g, track_nfail=1)
-    H.set_prims(torus(g))
-    H.fixup_bounds()
-    H.set_time(0.0, 1.0e-5)
-    H.step_ch(args.warmup)
-    H.sync()
-
-    H.profile_enable(True)
-    H.sync()
-    t0 = time.perf_counter()
-    H.step_ch(args.steps)
-    H.sync()
-    el = time.perf_counter() - t0
+    p0 = torus(g)
+    el, kern, finite = timed_run(args, g, p0, args.precision)
+    assert finite, "non-finite state after the timed steps"
     ncell = n1 * n2 * n3
     value = ncell * args.steps / el
 
-    kern = {}
-    for nm in ("flux", "emf", "update", "bounds", "fixup", "step"):
-        ms, cnt = H.profile_read(nm)
-        kern[nm] = (ms, cnt)
-    H.profile_enable(False)
-    p = H.prims()
-    assert np.isfinite(p).all(), "non-finite state after the timed steps"
-
+    tag = "fp64" if args.precision == "fp64" else "mixed"
     units = units_per_launch(n1, n2, n3)
     dom = max(("flux", "emf", "update"), key=lambda k: kern[k][0])
     avg_s = kern[dom][0] / max(kern[dom][1], 1) / 1e3
-    achieved = ALGO_BYTES[dom] * units[dom] / avg_s / 1e9
-    workload = f"torus{n1}x{n2}x{n3}_fp64"
+    algo = ALGO_BYTES[dom] if tag == "fp64" else ALGO_BYTES_MIXED[dom]
+    achieved = algo * units[dom] / avg_s / 1e9
+    workload = f"torus{n1}x{n2}x{n3}_{tag}"
     traffic_tab = measured_traffic(workload)
-    traffic = traffic_tab.get("k_" + dom) if traffic_tab else None
+    kname = {"flux": "k_flux", "emf": "k_emf", "update": "k_update"}[dom]
+    traffic = traffic_tab.get(kname) if traffic_tab else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                 "traffic_source": f"profiles/hbm_traffic_{workload}.md" if traffic else None,
-                "avg_launch_ms": avg_s * 1e3, "algo_bytes_per_launch": ALGO_BYTES[dom] * units[dom],
+                "avg_launch_ms": avg_s * 1e3, "algo_bytes_per_launch": algo * units[dom],
                 "kernel_ms_per_step": {k: kern[k][0] / args.steps for k in ("flux", "emf", "update", "bounds", "fixup")},
                 "state_roofline_frac": value * 320.0 / (HBM_PEAK_GBS * 1e9)}
+
+    mixed = None
+    if args.mixed == "auto" and args.precision == "fp64":
+        el_m, kern_m, fin_m = timed_run(args, g, p0, "mixed")
+        v_m = ncell * args.steps / el_m
+        mixed = {"value": v_m, "unit": "zone-cycles/s", "ms_per_step": el_m / args.steps * 1e3,
+                 "speedup_vs_fp64": v_m / value, "finite": fin_m,
+                 "kernel_ms_per_step": {k: kern_m[k][0] / args.steps for k in ("flux", "emf", "update", "bounds", "fixup")}}
+
+    prec_rep = None
+    if args.precision_report == "auto":
+        prec_rep = precision_report(tuple(int(x) for x in args.report_grid.split("x")))
 
     cpu = None
     if rank == 0 and args.cpu_baseline == "auto":
@@ -211,13 +277,16 @@ def main():
     line = {
         "metric": METRIC, "value": value, "unit": "zone-cycles/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if tag == "fp64" else "f64+f32",
         "data": "synthetic (Fishbone-Moncrief torus, seeded +-2% u perturbation)",
-        "config": {"workload": f"torus_{n1}x{n2}x{n3}_fp64", "grid": [n1, n2, n3], "spin": 0.9,
+        "config": {"workload": f"torus_{n1}x{n2}x{n3}_{tag}", "grid": [n1, n2, n3], "spin": 0.9,
                    "recon": "PPM+MC", "flux": "HLL", "ct": "PPM EMFs", "cooling": "h/r=0.1",
-                   "precision": "fp64 (mixed-precision variant not built)"},
+                   "precision": tag},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
+        "precision_variant": prec_rep,
     }
     if rank == 0:
         print(json.dumps(line), flush=True)

@@ -31,6 +31,7 @@ class HgpuConfig(ctypes.Structure):
         ("r_isco", ctypes.c_double), ("h_o_r", ctypes.c_double),
         ("rhomin", ctypes.c_double), ("uumin", ctypes.c_double),
         ("rhopower", ctypes.c_double), ("uupower", ctypes.c_double),
+        ("precision", ctypes.c_int),
     ]
 
 

@@ -6,74 +6,22 @@
 // arithmetic can be compared with the reference on a machine without a GPU.
 #pragma once
 #include "hgpu_phys.h"
+#include "hgpu_mixed.h"
 
 namespace hgpu {
 
 // --------------------------------------------------------------------------
-// small helpers
-
-HD double dmax(double a, double b) { return (a > b) ? a : b; }   // MAX(), src/macros.h:474
-
-HD unsigned long long dbl_bits(double x) {
-  union { double d; unsigned long long u; } c;
-  c.d = x;
-  return c.u;
-}
-HD double bits_dbl(unsigned long long u) {
-  union { double d; unsigned long long u; } c;
-  c.u = u;
-  return c.d;
-}
-
-// Index of the x1 rescaling factor per variable: rho -> r^1.5, u -> r, rest -> r^3
-// (rscale_prims_r(), src/recon.c:1173-1189).
-HD int rs_slot(int v) { return (v == RHO) ? 0 : ((v == UU) ? 2 : 1); }
-
-// Value of a primitive after the in-place x1 rescale/unscale round trip that
-// reconstruct_fast() applies to every zone (src/recon.c:996, :1008, :1173-1210).
-HD double rt_val(const Params& P, const Ptrs& X, int v, int i, double x) {
-  if (!P.rescale_r) return x;
-  int s = rs_slot(v);
-  return (x * X.rscale[s*P.n1t + i]) * X.irscale[s*P.n1t + i];
-}
-
-// Tally of fail() calls of one zone; written out by the kernel wrapper.
-struct FailRec {
-  int nf[N_NFAIL];
-  int pflag;        // last pflag response (0: none)
-  int boundary;     // check_boundary_pflag() hit a physical boundary
-  HD void clear() {
-    for (int l = 0; l < N_NFAIL; l++) nf[l] = 0;
-    pflag = 0;
-    boundary = 0;
-  }
-};
-
-// fail(type, state) bookkeeping, src/diag.c:159-300: state 1 -> INTERP_PRIM,
-// state 2 -> INTERP_V; check_boundary_pflag(), src/harm_mpi.c:1388-1430.
-HD void fail_set(const Params& P, FailRec& r, int nfail_id, int state, int i, int j, int k) {
-  r.nf[nfail_id]++;
-  if (state) {
-    r.pflag = (state == 1) ? PFLAG_INTERP_PRIM : PFLAG_INTERP_V;
-    int n1e = P.n1 + NG - 1, n2e = P.n2 + NG - 1, n3e = P.n3 + NG - 1;
-    if (i < NG + NG || i > n1e - NG || j < NG + NG || j > n2e - NG || k < NG + NG || k > n3e - NG)
-      r.boundary = 1;
-  }
-}
-
-HD const Geom* geom_at(const Params& P, const Ptrs& X, int i, int j, int k, int pos) {
-  return X.geom + geom_id(P, i, j, k, pos);
-}
-
-// --------------------------------------------------------------------------
 // numerical_flux() for one face, src/step_ch.c:636-790 (with the face states of
 // reconstruct_fast(), src/recon.c:894-1103).  src is the state being
-// reconstructed (p in the predictor, ph in the corrector).
-HD void flux_face(const Params& P, const Ptrs& X, const double* __restrict__ src, int d,
+// reconstructed (p in the predictor, ph in the corrector).  The direction is a
+// template parameter so every 4-vector component the face selects (u^dir,
+// b^dir, g^{t dir}) is a compile-time index and stays in registers.
+template <int d>
+HD void flux_face(const Params& P, const Ptrs& X, const double* __restrict__ src,
                   int i, int j, int k, FailRec& fr) {
   const long long NT = P.nt;
   double pl[NP], pr[NP];
-  const int di = (d == 0), dj = (d == 1), dk = (d == 2);
+  constexpr int di = (d == 0), dj = (d == 1), dk = (d == 2);
 
   if (P.degen[d]) {
     // degenerate direction: p_L = p_R = p_h of the zone at the direction's
@@ -140,7 +88,7 @@ HD void flux_face(const Params& P, const Ptrs& X, const double* __restrict__ src
   }
 
   const Geom* g = geom_at(P, X, i, j, k, d);
-  const int dim = d + 1;
+  constexpr int dim = d + 1;
   State ql, qr;
   if (get_state(pl, g, P.gam, &ql)) fr.nf[NFAIL_GAMMA_CALC]++;
   if (get_state(pr, g, P.gam, &qr)) fr.nf[NFAIL_GAMMA_CALC]++;
@@ -156,9 +104,10 @@ HD void flux_face(const Params& P, const Ptrs& X, const double* __restrict__ src
   double cmin = fabs(dmax(dmax(0., -cminl), -cminr));
   double ctop = dmax(cmin, cmax);
 
-  // fastest speed over interior faces (timestep(), src/step_ch.c:716-724)
+  // fastest speed over interior faces (timestep(), src/step_ch.c:716-724),
+  // reduced over the launch by the caller
   if (i >= NG && i < NG + P.n1 && j >= NG && j < NG + P.n2 && k >= NG && k < NG + P.n3 && ctop > 0.)
-    atomic_max_u64(&X.status[ST_VMAX + dim], dbl_bits(ctop));
+    fr.vmax = ctop;
 
   const long long c = cidx(P, i, j, k);
   double* F = X.F;
@@ -324,7 +273,15 @@ HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, i
 }
 
 // --------------------------------------------------------------------------
+// Entropy face flux of face (d, c) from the FP64 or the FP32 flux array
+template <bool MIXED>
+HD double entropy_flux(const Ptrs& X, long long NT, int d, long long c) {
+  if (MIXED) return (double)X.Ff[(d*NFLUX + 8)*NT + c];
+  return X.F[(d*NFLUX + 8)*NT + c];
+}
+
 // fixup_entropy_eq(), src/fixup.c:335-454
+template <bool MIXED>
 HD int fixup_entropy_eq(const Params& P, const Ptrs& X, int i, int j, int k, double Dt,
                         const double* prim_old, double* prim, const Geom* g, const double* U,
                         double* gamma_out, FailRec& fr) {
@@ -336,9 +293,9 @@ HD int fixup_entropy_eq(const Params& P, const Ptrs& X, int i, int j, int k, dou
   double S_old = g->g * (P.gam - 1.)*prim_old[UU]*gamma / (g->alpha * pow(prim_old[RHO], (P.gam - 1.)));
   const long long c = cidx(P, i, j, k);
   const long long si = (long long)P.n2t * P.n3t, sj = P.n3t, sk = 1;
-  const double Fm0 = X.F[(0*NFLUX + 8)*NT + c], Fp0 = X.F[(0*NFLUX + 8)*NT + c + si];
-  const double Fm1 = X.F[(1*NFLUX + 8)*NT + c], Fp1 = X.F[(1*NFLUX + 8)*NT + c + sj];
-  const double Fm2 = X.F[(2*NFLUX + 8)*NT + c], Fp2 = X.F[(2*NFLUX + 8)*NT + c + sk];
+  const double Fm0 = entropy_flux<MIXED>(X, NT, 0, c), Fp0 = entropy_flux<MIXED>(X, NT, 0, c + si);
+  const double Fm1 = entropy_flux<MIXED>(X, NT, 1, c), Fp1 = entropy_flux<MIXED>(X, NT, 1, c + sj);
+  const double Fm2 = entropy_flux<MIXED>(X, NT, 2, c), Fp2 = entropy_flux<MIXED>(X, NT, 2, c + sk);
   double S = S_old + Dt*(
       - (Fp0 - Fm0) * P.invdx[1]
       - (Fp1 - Fm1) * P.invdx[2]
@@ -364,6 +321,7 @@ HD int fixup_entropy_eq(const Params& P, const Ptrs& X, int i, int j, int k, dou
 
 // --------------------------------------------------------------------------
 // recover_primitives(), src/step_ch.c:801-1024 (FIXUP_TREE 3, entropy fix on)
+template <bool MIXED>
 HD void recover_primitives(const Params& P, const Ptrs& X, int i, int j, int k, double Dt,
                            double* pf, const double* U, const double* p_old, const Geom* g,
                            double rhoflr, double uuflr, double* p_gamma, FailRec& fr) {
@@ -396,7 +354,7 @@ HD void recover_primitives(const Params& P, const Ptrs& X, int i, int j, int k, 
     if (!ret) { for (int l = 0; l < NP; l++) pf2[l] = pf[l]; }
     else fail_set(P, fr, NFAIL_UTOPRIM, 0, i, j, k);
     for (int l = 0; l < B1; l++) pf[l] = pg[l];
-    int ret_ee = fixup_entropy_eq(P, X, i, j, k, Dt, p_old, pf, g, U, &gamma, fr);
+    int ret_ee = fixup_entropy_eq<MIXED>(P, X, i, j, k, Dt, p_old, pf, g, U, &gamma, fr);
     if (ret_ee) {
       if (ret) {
         fail_set(P, fr, NFAIL_UTOPRIM_EE, 1, i, j, k);
@@ -437,6 +395,9 @@ HD double floor_uu(const Params& P, const Ptrs& X, int i, long long c) {
 //   stage 0 (predictor): p_i = p_h = p, result into ph, guess = p before the
 //                        rescale round trip (the reference's ph copy)
 //   stage 1 (corrector): p_h = ph, U^n recomputed from p, guess = p, result into p
+// MIXED reads the FP32 face fluxes and EMFs and sums the divergence with
+// compensation; otherwise the FP64 arrays with the reference's sum.
+template <bool MIXED>
 HD void update_zone(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
                     FailRec& fr) {
   const long long NT = P.nt;
@@ -462,15 +423,28 @@ HD void update_zone(const Params& P, const Ptrs& X, int stage, double dt, int i,
   double dU[NP];
   fr.nf[NFAIL_GAMMA_CALC] += source(P, X, phz, g, i, j, k, dU);
 
-  for (int l = 0; l < NP; l++) {
-    double f0m = face_flux(P, X, 0, l, i, j, k), f0p = face_flux(P, X, 0, l, i + 1, j, k);
-    double f1m = face_flux(P, X, 1, l, i, j, k), f1p = face_flux(P, X, 1, l, i, j + 1, k);
-    double f2m = face_flux(P, X, 2, l, i, j, k), f2p = face_flux(P, X, 2, l, i, j, k + 1);
-    U[l] += Dt*(
-        - (f0p - f0m) * P.invdx[1]
-        - (f1p - f1m) * P.invdx[2]
-        - (f2p - f2m) * P.invdx[3]
-        + dU[l]);
+  if (MIXED) {
+#pragma unroll
+    for (int l = 0; l < NP; l++) {
+      CompSum acc;
+      acc.add(-(face_flux_mixed(P, X, 0, l, i + 1, j, k) - face_flux_mixed(P, X, 0, l, i, j, k)) * P.invdx[1]);
+      acc.add(-(face_flux_mixed(P, X, 1, l, i, j + 1, k) - face_flux_mixed(P, X, 1, l, i, j, k)) * P.invdx[2]);
+      acc.add(-(face_flux_mixed(P, X, 2, l, i, j, k + 1) - face_flux_mixed(P, X, 2, l, i, j, k)) * P.invdx[3]);
+      acc.add(dU[l]);
+      U[l] += Dt*acc.value();
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < NP; l++) {
+      double f0m = face_flux(P, X, 0, l, i, j, k), f0p = face_flux(P, X, 0, l, i + 1, j, k);
+      double f1m = face_flux(P, X, 1, l, i, j, k), f1p = face_flux(P, X, 1, l, i, j + 1, k);
+      double f2m = face_flux(P, X, 2, l, i, j, k), f2p = face_flux(P, X, 2, l, i, j, k + 1);
+      U[l] += Dt*(
+          - (f0p - f0m) * P.invdx[1]
+          - (f1p - f1m) * P.invdx[2]
+          - (f2p - f2m) * P.invdx[3]
+          + dU[l]);
+    }
   }
 
   double pf[NP];
@@ -478,8 +452,8 @@ HD void update_zone(const Params& P, const Ptrs& X, int stage, double dt, int i,
   else { for (int v = 0; v < NP; v++) pf[v] = pi[v]; }
 
   double pgam = 1.;
-  recover_primitives(P, X, i, j, k, Dt, pf, U, p_raw, g, floor_rho(P, X, i, c), floor_uu(P, X, i, c),
-                     &pgam, fr);
+  recover_primitives<MIXED>(P, X, i, j, k, Dt, pf, U, p_raw, g, floor_rho(P, X, i, c), floor_uu(P, X, i, c),
+                            &pgam, fr);
 
   double* dst = (stage == 0) ? X.ph : X.p;
   for (int v = 0; v < NP; v++) dst[v*NT + c] = pf[v];

@@ -66,6 +66,8 @@ inline int make_params(const hgpu_config& c, Params& P) {
   P.floor_dim = (c.coords == 1) ? 1 : 3;
   P.n_spatial_dims = (c.n1 > 1) + (c.n2 > 1) + (c.n3 > 1);
   P.track_nfail = c.track_nfail;
+  if (c.precision != 0 && c.precision != 1) return set_error(-2, "precision must be 0 (FP64) or 1 (mixed)");
+  P.mixed = c.precision;
   for (int l = 0; l < 4; l++) { P.dx[l] = c.dx[l]; P.invdx[l] = 1./c.dx[l]; }
   P.gam = c.gam;
   P.gam_m1_o_gam = (c.gam - 1.)/c.gam;
@@ -159,6 +161,23 @@ inline void build_tables(const hgpu_config& c, const Params& P, long long n_geom
   }
 }
 
+// FP32 copies of geometry records (mixed precision)
+inline void geom_to_f32(const double* g42, long long n, std::vector<GeomF>& out) {
+  out.resize(n);
+  for (long long r = 0; r < n; r++) {
+    const Geom* g = reinterpret_cast<const Geom*>(g42 + r * GEOM_DOUBLES);
+    GeomF& f = out[r];
+    for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) {
+      f.gcon[a][b] = (float)g->gcon[a][b];
+      f.gcov[a][b] = (float)g->gcov[a][b];
+    }
+    f.g = (float)g->g; f.g_inv = (float)g->g_inv; f.alpha = (float)g->alpha;
+    for (int a = 0; a < 3; a++) f.beta[a] = (float)g->beta[a];
+    for (int a = 0; a < 4; a++) f.ncon[a] = (float)g->ncon[a];
+    f.om_alpha = (float)(1. - g->alpha);
+  }
+}
+
 // AoS [zone][8] <-> SoA [8][zone]
 inline void aos_to_soa(const double* aos, double* soa, long long nt, int nv) {
   for (long long c = 0; c < nt; c++) for (int v = 0; v < nv; v++) soa[v*nt + c] = aos[c*nv + v];
@@ -187,8 +206,12 @@ struct Api {
     rc = 0;
     rc |= BE::alloc(c, (void**)&X.p, sizeof(double) * NP * NT);
     rc |= BE::alloc(c, (void**)&X.ph, sizeof(double) * NP * NT);
-    rc |= BE::alloc(c, (void**)&X.F, sizeof(double) * 3 * NFLUX * NT);
-    rc |= BE::alloc(c, (void**)&X.emf, sizeof(double) * 3 * NT);
+    const long long nF = P.mixed ? 1 : 3 * NFLUX * NT, nE = P.mixed ? 1 : 3 * NT;
+    rc |= BE::alloc(c, (void**)&X.F, sizeof(double) * nF);
+    rc |= BE::alloc(c, (void**)&X.emf, sizeof(double) * nE);
+    rc |= BE::alloc(c, (void**)&X.Ff, sizeof(float) * (P.mixed ? 3 * NFLUX * NT : 2));
+    rc |= BE::alloc(c, (void**)&X.emff, sizeof(float) * (P.mixed ? 3 * NT : 2));
+    rc |= BE::alloc(c, (void**)&X.geomf, sizeof(GeomF) * (P.mixed ? c->n_geom : 1));
     rc |= BE::alloc(c, (void**)&X.geom, sizeof(double) * GEOM_DOUBLES * c->n_geom);
     rc |= BE::alloc(c, (void**)&X.conn, sizeof(double) * 64 * c->n_conn);
     rc |= BE::alloc(c, (void**)&X.omega, sizeof(double) * P.n1t);
@@ -213,8 +236,13 @@ struct Api {
     rc |= BE::h2d(c, (void*)X.uuflr, T.uuflr.data(), sizeof(double) * T.uuflr.size());
     rc |= BE::zero(c, X.p, sizeof(double) * NP * NT);
     rc |= BE::zero(c, X.ph, sizeof(double) * NP * NT);
-    rc |= BE::zero(c, X.F, sizeof(double) * 3 * NFLUX * NT);
-    rc |= BE::zero(c, X.emf, sizeof(double) * 3 * NT);
+    rc |= BE::zero(c, X.F, sizeof(double) * nF);
+    rc |= BE::zero(c, X.emf, sizeof(double) * nE);
+    if (P.mixed) {
+      rc |= BE::zero(c, X.Ff, sizeof(float) * 3 * NFLUX * NT);
+      rc |= BE::zero(c, X.emff, sizeof(float) * 3 * NT);
+      rc |= upload_geomf(c, T.geom.data());
+    }
     rc |= BE::zero(c, X.gamma, sizeof(double) * NT);
     rc |= BE::zero(c, X.pflag, sizeof(int) * NT);
     rc |= BE::zero(c, X.nfail, sizeof(int) * (P.track_nfail ? N_NFAIL * NT : 1));
@@ -234,11 +262,19 @@ struct Api {
     Ptrs& X = c->X;
     void* ptrs[] = {X.p, X.ph, X.F, X.emf, (void*)X.geom, (void*)X.conn, (void*)X.omega, (void*)X.tr,
                     (void*)X.rscale, (void*)X.irscale, (void*)X.rhoflr, (void*)X.uuflr, X.gamma,
-                    X.pflag, X.nfail, X.status, X.scal};
+                    X.pflag, X.nfail, X.status, X.scal, X.Ff, X.emff, (void*)X.geomf};
     for (void* p : ptrs) if (p) BE::release(c, p);
     BE::fini(c);
     delete c;
     return 0;
+  }
+
+  // FP32 geometry copy for the mixed-precision face fluxes
+  static int upload_geomf(hgpu_ctx* c, const double* geom_host) {
+    if (!c->P.mixed) return 0;
+    std::vector<GeomF> gf;
+    geom_to_f32(geom_host, c->n_geom, gf);
+    return BE::h2d(c, (void*)c->X.geomf, gf.data(), sizeof(GeomF) * gf.size());
   }
 
   static int upload_prims(hgpu_ctx* c, int which, const double* host) {

@@ -56,8 +56,20 @@ struct HostBE {
       int i, j, k;
       b.at(t, i, j, k);
       FailRec fr; fr.clear();
-      flux_face(c->P, c->X, src, d, i, j, k, fr);
+      if (c->P.mixed) {
+        if (d == 0) flux_face_mixed<0>(c->P, c->X, src, i, j, k, fr);
+        else if (d == 1) flux_face_mixed<1>(c->P, c->X, src, i, j, k, fr);
+        else flux_face_mixed<2>(c->P, c->X, src, i, j, k, fr);
+      } else {
+        if (d == 0) flux_face<0>(c->P, c->X, src, i, j, k, fr);
+        else if (d == 1) flux_face<1>(c->P, c->X, src, i, j, k, fr);
+        else flux_face<2>(c->P, c->X, src, i, j, k, fr);
+      }
       host_commit(c->P, c->X, fr, cidx(c->P, i, j, k), true);
+      if (fr.vmax > 0.) {
+        unsigned long long& w = c->X.status[ST_VMAX + d + 1];
+        if (dbl_bits(fr.vmax) > w) w = dbl_bits(fr.vmax);
+      }
     }
     return 0;
   }
@@ -66,7 +78,8 @@ struct HostBE {
     for (long long t = 0; t < b.count(); t++) {
       int i, j, k;
       b.at(t, i, j, k);
-      emf_edge(c->P, c->X, i, j, k);
+      if (c->P.mixed) emf_edge_mixed(c->P, c->X, i, j, k);
+      else emf_edge(c->P, c->X, i, j, k);
     }
     return 0;
   }
@@ -77,7 +90,8 @@ struct HostBE {
       int i, j, k;
       b.at(t, i, j, k);
       FailRec fr; fr.clear();
-      update_zone(c->P, c->X, stage, dt, i, j, k, fr);
+      if (c->P.mixed) update_zone<true>(c->P, c->X, stage, dt, i, j, k, fr);
+      else update_zone<false>(c->P, c->X, stage, dt, i, j, k, fr);
       host_commit(c->P, c->X, fr, cidx(c->P, i, j, k), true);
     }
     return 0;

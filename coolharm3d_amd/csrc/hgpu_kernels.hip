@@ -1,10 +1,10 @@
 // hgpu_kernels.hip -- MI355X (gfx950) backend of the per-substep GRMHD advance.
 //
-// Kernels are thin wrappers over the per-thread bodies of hgpu_bodies.h: a
-// flattened 1D launch over each kernel's index box with k fastest, so the 64
-// lanes of a wavefront walk contiguous zones of one x3 pencil (coalesced SoA
-// loads; the 2D geometry record is a wavefront-wide broadcast).  Failure
-// counters are summed across the wavefront before one atomic per wave.
+// Kernels are thin wrappers over the per-thread bodies of hgpu_bodies.h.  The
+// 64 lanes of a wavefront walk contiguous zones of one x3 pencil (k fastest:
+// coalesced SoA loads; the 2D geometry record is a wavefront-wide broadcast);
+// the stencil kernels map workgroups to tiles XCD by XCD (Tiling below).
+// Failure counters are summed across the wavefront before one atomic per wave.
 #include "hgpu_core.h"
 #include <hip/hip_runtime.h>
 
@@ -36,61 +36,148 @@ __device__ __forceinline__ int wave_sum(int v) {
   return v;
 }
 
-// Writes a zone's fail() tallies: per-zone nfail, wave-summed nfailtot, and the
-// failure_exists / failure_type / boundary_phys_pflag words.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off, 64);
+    v = (o > v) ? o : v;
+  }
+  return v;
+}
+
+// Writes the fail() tallies of a workgroup: per-zone nfail, and block totals of
+// nfailtot, the failure words and the fastest face speed (vmax_slot >= 0),
+// reduced in LDS so each status word receives at most one atomic per
+// workgroup.  Every thread of the block must call it.
 __device__ __forceinline__ void commit_fail(const Params& P, const Ptrs& X, const FailRec& fr,
-                                            bool active, long long c, unsigned mask) {
+                                            bool active, long long c, unsigned mask, int vmax_slot = -1) {
+  __shared__ unsigned s_cnt[N_NFAIL];
+  __shared__ unsigned s_flags;
+  __shared__ unsigned long long s_vmax;
   const long long NT = P.nt;
-  const int lane = threadIdx.x & 63;
-  for (int l = 0; l < N_NFAIL; l++) {
-    if (!(mask & (1u << l))) continue;
-    int v = active ? fr.nf[l] : 0;
-    if (active && v && P.track_nfail) X.nfail[l*NT + c] += v;
-    v = wave_sum(v);
-    if (lane == 0 && v) atomicAdd(&X.status[ST_NFAILTOT + l], (unsigned long long)v);
+  const int tid = threadIdx.x;
+  if (tid < N_NFAIL) s_cnt[tid] = 0;
+  if (tid == 0) { s_flags = 0; s_vmax = 0; }
+  __syncthreads();
+  if (active) {
+    for (int l = 0; l < N_NFAIL; l++) {
+      if (!(mask & (1u << l))) continue;
+      const int v = fr.nf[l];
+      if (v) {
+        if (P.track_nfail) X.nfail[l*NT + c] += v;
+        atomicAdd(&s_cnt[l], (unsigned)v);
+      }
+    }
+    if (fr.pflag) {
+      unsigned f = 1u | ((fr.pflag == PFLAG_INTERP_PRIM) ? 2u : 4u) | (fr.boundary ? 8u : 0u);
+      atomicOr(&s_flags, f);
+    }
   }
-  if (active && fr.pflag) {
-    atomicOr(&X.status[ST_FAILURE_EXISTS], 1ull);
-    atomicOr(&X.status[(fr.pflag == PFLAG_INTERP_PRIM) ? ST_FAILTYPE_PRIM : ST_FAILTYPE_V], 1ull);
-    if (fr.boundary) atomicOr(&X.status[ST_BOUNDARY_PHYS], 1ull);
+  if (vmax_slot >= 0) {
+    const unsigned long long m = wave_max_u64(active ? dbl_bits(fr.vmax) : 0ull);
+    if ((tid & 63) == 0 && m) atomicMax(&s_vmax, m);
   }
+  __syncthreads();
+  if (tid < N_NFAIL && s_cnt[tid]) atomicAdd(&X.status[ST_NFAILTOT + tid], (unsigned long long)s_cnt[tid]);
+  if (tid == 0) {
+    const unsigned f = s_flags;
+    if (f) {
+      atomicOr(&X.status[ST_FAILURE_EXISTS], 1ull);
+      if (f & 2u) atomicOr(&X.status[ST_FAILTYPE_PRIM], 1ull);
+      if (f & 4u) atomicOr(&X.status[ST_FAILTYPE_V], 1ull);
+      if (f & 8u) atomicOr(&X.status[ST_BOUNDARY_PHYS], 1ull);
+    }
+    if (vmax_slot >= 0 && s_vmax) atomicMax(&X.status[ST_VMAX + vmax_slot], s_vmax);
+  }
+}
+
+// Work decomposition of the stencil kernels.  The index box is cut into
+// x1-planes of (j,k) and each plane into tiles of BLOCK consecutive (j,k)
+// (k fastest).  Hardware dispatches workgroup p to XCD p % 8, and each XCD has
+// its own L2; the logical tile of workgroup p is chosen so that every XCD gets
+// one contiguous range of tiles (whole x1-slabs), processed in order, so the
+// +-3-zone stencils of neighbouring tiles hit the same L2.
+struct Tiling {
+  unsigned X;         // tiles per plane
+  unsigned nplane;    // (j,k) points per plane
+  unsigned nk;        // k extent
+  unsigned ni;        // number of planes
+  unsigned nblocks;   // X * ni rounded up to a multiple of 8
+};
+
+static inline Tiling make_tiling(const Box& b) {
+  Tiling T;
+  T.nk = (unsigned)(b.hi[2] - b.lo[2] + 1);
+  T.nplane = (unsigned)(b.hi[1] - b.lo[1] + 1) * T.nk;
+  T.ni = (unsigned)(b.hi[0] - b.lo[0] + 1);
+  T.X = (T.nplane + BLOCK - 1) / BLOCK;
+  T.nblocks = ((T.X * T.ni + 7u) / 8u) * 8u;
+  return T;
+}
+
+__device__ __forceinline__ bool tile_index(const Box& b, const Tiling& T, int& i, int& j, int& k) {
+  const unsigned p = blockIdx.x;
+  const unsigned L = (p & 7u) * (T.nblocks >> 3) + (p >> 3);
+  const unsigned ii = L / T.X;
+  const unsigned t = (L - ii * T.X) * BLOCK + threadIdx.x;
+  if (ii >= T.ni || t >= T.nplane) return false;
+  const unsigned jj = t / T.nk;
+  i = b.lo[0] + (int)ii;
+  j = b.lo[1] + (int)jj;
+  k = b.lo[2] + (int)(t - jj * T.nk);
+  return true;
 }
 
 constexpr unsigned MASK_GAMMA = 1u << NFAIL_GAMMA_CALC;
 constexpr unsigned MASK_ALL = (1u << N_NFAIL) - 1u;
 
-__global__ __launch_bounds__(BLOCK) void k_flux(Params P, Ptrs X, const double* __restrict__ src, int d, Box b) {
-  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
-  const bool active = t < b.count();
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_flux(Params P, Ptrs X, const double* __restrict__ src, Box b, Tiling T) {
+  int i, j, k;
+  const bool active = tile_index(b, T, i, j, k);
   FailRec fr; fr.clear();
   long long c = 0;
   if (active) {
-    int i, j, k;
-    b.at(t, i, j, k);
-    flux_face(P, X, src, d, i, j, k, fr);
+    flux_face<D>(P, X, src, i, j, k, fr);
     c = cidx(P, i, j, k);
   }
-  commit_fail(P, X, fr, active, c, MASK_GAMMA);
+  commit_fail(P, X, fr, active, c, MASK_GAMMA, D + 1);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_emf(Params P, Ptrs X, Box b) {
-  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
-  if (t >= b.count()) return;
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_flux_mixed(Params P, Ptrs X, const double* __restrict__ src, Box b,
+                                                      Tiling T) {
   int i, j, k;
-  b.at(t, i, j, k);
+  const bool active = tile_index(b, T, i, j, k);
+  FailRec fr; fr.clear();
+  long long c = 0;
+  if (active) {
+    flux_face_mixed<D>(P, X, src, i, j, k, fr);
+    c = cidx(P, i, j, k);
+  }
+  commit_fail(P, X, fr, active, c, MASK_GAMMA, D + 1);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_emf(Params P, Ptrs X, Box b, Tiling T) {
+  int i, j, k;
+  if (!tile_index(b, T, i, j, k)) return;
   emf_edge(P, X, i, j, k);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_update(Params P, Ptrs X, int stage, Box b) {
-  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
-  const bool active = t < b.count();
+__global__ __launch_bounds__(BLOCK) void k_emf_mixed(Params P, Ptrs X, Box b, Tiling T) {
+  int i, j, k;
+  if (!tile_index(b, T, i, j, k)) return;
+  emf_edge_mixed(P, X, i, j, k);
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(BLOCK) void k_update(Params P, Ptrs X, int stage, Box b, Tiling T) {
+  int i, j, k;
+  const bool active = tile_index(b, T, i, j, k);
   FailRec fr; fr.clear();
   long long c = 0;
   if (active) {
-    int i, j, k;
-    b.at(t, i, j, k);
     const double dt = X.scal[SC_DT];
-    update_zone(P, X, stage, dt, i, j, k, fr);
+    update_zone<MIXED>(P, X, stage, dt, i, j, k, fr);
     c = cidx(P, i, j, k);
   }
   commit_fail(P, X, fr, active, c, MASK_ALL);
@@ -318,19 +405,34 @@ struct GpuBE {
   static int flux(hgpu_ctx* c, const double* src, int d) {
     Box b = flux_box(c->P, d);
     Scope s(c, "flux");
-    hipLaunchKernelGGL(k_flux, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, src, d, b);
+    const Tiling T = make_tiling(b);
+    const dim3 grid(T.nblocks), blk(BLOCK);
+    hipStream_t st = priv(c)->stream;
+    if (c->P.mixed) {
+      if (d == 0) hipLaunchKernelGGL(k_flux_mixed<0>, grid, blk, 0, st, c->P, c->X, src, b, T);
+      else if (d == 1) hipLaunchKernelGGL(k_flux_mixed<1>, grid, blk, 0, st, c->P, c->X, src, b, T);
+      else hipLaunchKernelGGL(k_flux_mixed<2>, grid, blk, 0, st, c->P, c->X, src, b, T);
+    } else {
+      if (d == 0) hipLaunchKernelGGL(k_flux<0>, grid, blk, 0, st, c->P, c->X, src, b, T);
+      else if (d == 1) hipLaunchKernelGGL(k_flux<1>, grid, blk, 0, st, c->P, c->X, src, b, T);
+      else hipLaunchKernelGGL(k_flux<2>, grid, blk, 0, st, c->P, c->X, src, b, T);
+    }
     return hip_check(hipGetLastError(), "k_flux");
   }
   static int emf(hgpu_ctx* c) {
     Box b = edge_box(c->P);
     Scope s(c, "emf");
-    hipLaunchKernelGGL(k_emf, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, b);
+    const Tiling T = make_tiling(b);
+    if (c->P.mixed) hipLaunchKernelGGL(k_emf_mixed, dim3(T.nblocks), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, b, T);
+    else hipLaunchKernelGGL(k_emf, dim3(T.nblocks), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, b, T);
     return hip_check(hipGetLastError(), "k_emf");
   }
   static int update(hgpu_ctx* c, int stage) {
     Box b = interior_box(c->P);
     Scope s(c, "update");
-    hipLaunchKernelGGL(k_update, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage, b);
+    const Tiling T = make_tiling(b);
+    if (c->P.mixed) hipLaunchKernelGGL(k_update<true>, dim3(T.nblocks), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage, b, T);
+    else hipLaunchKernelGGL(k_update<false>, dim3(T.nblocks), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage, b, T);
     return hip_check(hipGetLastError(), "k_update");
   }
   static int bounds_dir(hgpu_ctx* c, double* S, int dir, int skip_mpi) {

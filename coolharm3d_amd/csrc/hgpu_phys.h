@@ -543,8 +543,10 @@ struct Uee {
   double t2, t4, t7, t24, t300, t400, s100, s200;
 };
 
-// func_rho() of src/utoprim_1d_ee.c:818-838
-HD void uee_func_rho(Uee& s, const double* x, double* dx, double* f, double* df) {
+// func_rho() of src/utoprim_1d_ee.c:818-838; like the reference it leaves the
+// residual norms f, df untouched (they stay 1, so only a non-finite x stops
+// the solver as divergent)
+HD void uee_func_rho(Uee& s, const double* x, double* dx, double*, double*) {
   double rho = x[0];
   double W = s.W_for_gnr2;
   double t40 = pow(rho, s.gm1);
@@ -553,8 +555,6 @@ HD void uee_func_rho(Uee& s, const double* x, double* dx, double* f, double* df)
   double t14 = t40/rho;
   double j00 = -t14*s.s200 + W;
   dx[0] = -r0/j00;
-  *df = -r0*r0;
-  *f = -0.5*(*df);
 }
 
 // gnr2(), src/utoprim_1d_ee.c:628-700 (never enters the extra-iteration mode)
@@ -577,8 +577,8 @@ HD int uee_gnr2(Uee& s, double* x) {
   return 1;
 }
 
-// func_W(), src/utoprim_1d_ee.c:742-795
-HD void uee_func_W(Uee& s, const double* x, double* dx, double* f, double* df) {
+// func_W(), src/utoprim_1d_ee.c:742-795 (f, df untouched, as func_rho())
+HD void uee_func_W(Uee& s, const double* x, double* dx, double*, double*) {
   double W = x[0];
   s.W_for_gnr2_old = s.W_for_gnr2;
   s.W_for_gnr2 = W;
@@ -599,8 +599,6 @@ HD void uee_func_W(Uee& s, const double* x, double* dx, double* f, double* df) {
   double r0 = (s.t300 + (s.t4 + s.t4 + (s.t400 + t15*(s.t7 + (t200)*W))*W)*W)*s.t24;
   double j00 = 2*(s.t4 + (s.t400 + t15*s.t7 + (3.0*t15*s.Bsq + s.t7*t1000 + (t15 + t15 + t1000*(t200))*W)*W)*W)*s.t24;
   dx[0] = -r0/j00;
-  *df = -r0*r0;
-  *f = -0.5*(*df);
 }
 
 // general_newton_raphson(), src/utoprim_1d_ee.c:534-618
@@ -859,6 +857,64 @@ HD int check_gamma(double gamma) {
   if (gamma <= GAMMAMAX) return 0;
   if (gamma > GAMMAMAX2) return 2;
   return 1;
+}
+
+// --------------------------------------------------------------------------
+// small helpers
+
+HD double dmax(double a, double b) { return (a > b) ? a : b; }   // MAX(), src/macros.h:474
+
+HD unsigned long long dbl_bits(double x) {
+  union { double d; unsigned long long u; } c;
+  c.d = x;
+  return c.u;
+}
+HD double bits_dbl(unsigned long long u) {
+  union { double d; unsigned long long u; } c;
+  c.u = u;
+  return c.d;
+}
+
+// Index of the x1 rescaling factor per variable: rho -> r^1.5, u -> r, rest -> r^3
+// (rscale_prims_r(), src/recon.c:1173-1189).
+HD int rs_slot(int v) { return (v == RHO) ? 0 : ((v == UU) ? 2 : 1); }
+
+// Value of a primitive after the in-place x1 rescale/unscale round trip that
+// reconstruct_fast() applies to every zone (src/recon.c:996, :1008, :1173-1210).
+HD double rt_val(const Params& P, const Ptrs& X, int v, int i, double x) {
+  if (!P.rescale_r) return x;
+  int s = rs_slot(v);
+  return (x * X.rscale[s*P.n1t + i]) * X.irscale[s*P.n1t + i];
+}
+
+// Tally of fail() calls of one zone; written out by the kernel wrapper.
+struct FailRec {
+  int nf[N_NFAIL];
+  int pflag;        // last pflag response (0: none)
+  int boundary;     // check_boundary_pflag() hit a physical boundary
+  double vmax;      // fastest characteristic speed of an interior face (0: none)
+  HD void clear() {
+    for (int l = 0; l < N_NFAIL; l++) nf[l] = 0;
+    pflag = 0;
+    boundary = 0;
+    vmax = 0.;
+  }
+};
+
+// fail(type, state) bookkeeping, src/diag.c:159-300: state 1 -> INTERP_PRIM,
+// state 2 -> INTERP_V; check_boundary_pflag(), src/harm_mpi.c:1388-1430.
+HD void fail_set(const Params& P, FailRec& r, int nfail_id, int state, int i, int j, int k) {
+  r.nf[nfail_id]++;
+  if (state) {
+    r.pflag = (state == 1) ? PFLAG_INTERP_PRIM : PFLAG_INTERP_V;
+    int n1e = P.n1 + NG - 1, n2e = P.n2 + NG - 1, n3e = P.n3 + NG - 1;
+    if (i < NG + NG || i > n1e - NG || j < NG + NG || j > n2e - NG || k < NG + NG || k > n3e - NG)
+      r.boundary = 1;
+  }
+}
+
+HD const Geom* geom_at(const Params& P, const Ptrs& X, int i, int j, int k, int pos) {
+  return X.geom + geom_id(P, i, j, k, pos);
 }
 
 }  // namespace hgpu

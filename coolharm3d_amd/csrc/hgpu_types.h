@@ -47,6 +47,19 @@ struct Geom {
 };
 static_assert(sizeof(Geom) == GEOM_DOUBLES * sizeof(double), "Geom layout");
 
+// Single-precision copy of a geometry record, used by the mixed-precision
+// reconstruction and face fluxes.
+struct GeomF {
+  float gcon[4][4];
+  float gcov[4][4];
+  float g;
+  float g_inv;
+  float alpha;
+  float beta[3];
+  float ncon[4];
+  float om_alpha;          // 1 - alpha, formed in FP64 (energy flux without the rest-mass cancellation)
+};
+
 struct State {   // struct of_state, src/globals.h:100-109
   double ucon[4], ucov[4], bcon[4], bcov[4];
   double bsq, p, eta, ptot;
@@ -82,6 +95,7 @@ struct Params {
   int degen[3];            // N_d == 1
   int n_spatial_dims;
   int track_nfail;         // keep nfail[15][NT] per-cell counters
+  int mixed;               // 1: FP32 reconstruction, face fluxes and EMFs
   double dx[4], invdx[4];
   double gam, gam_m1_o_gam;
   double cour;
@@ -96,6 +110,9 @@ struct Ptrs {
   double* F;               // face fluxes [3][NFLUX][NT]
   double* emf;             // corner EMFs [3][NT]
   const Geom* geom;        // geometry records
+  const GeomF* geomf;      // their FP32 copy (mixed precision)
+  float* Ff;               // FP32 face fluxes [3][NFLUX][NT] (mixed precision)
+  float* emff;             // FP32 corner EMFs [3][NT] (mixed precision)
   const double* conn;      // connection [N_CONN][64]
   const double* omega;     // cooling: Omega(r) per x1 index
   const double* tr;        // cooling: T(r)/(gam-1) per x1 index

@@ -69,6 +69,7 @@ def config_for(grid: GridSpec, **kw) -> _lib.HgpuConfig:
     else:
         c.rhomin, c.uumin = kw.get("rhomin", 2.0e-10), kw.get("uumin", 2.0e-12)
     c.rhopower, c.uupower = kw.get("rhopower", -1.5), kw.get("uupower", -2.5)
+    c.precision = {"fp64": 0, "mixed": 1}[kw.get("precision", "fp64")]
     return c
 
 

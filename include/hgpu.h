@@ -44,6 +44,12 @@ typedef struct hgpu_config {
   double r_isco;           /* ISCO radius used by the cooling function (src/phys.c:815-845)           */
   double h_o_r;            /* cooling target H/R (src/phys.c:655)                                     */
   double rhomin, uumin, rhopower, uupower;  /* floor model (src/options.h:232-241)                 */
+  int precision;           /* 0: all FP64 (the reference's arithmetic);
+                              1: mixed -- reconstruction, face fluxes and EMF averaging in FP32
+                              (face states formed from FP32 differences around the FP64 zone
+                              values), FP64 conserved update with compensated flux-divergence
+                              sums, FP64 con-to-prim, metric, connection and cooling.
+                              No counterpart in the reference.                                     */
 } hgpu_config;
 
 typedef struct hgpu_ctx hgpu_ctx;
