@@ -16,6 +16,7 @@ HOSTCHECK_LIB = os.path.join(PKG, "libhgpu_hostcheck.so")
 c_double_p = ctypes.POINTER(ctypes.c_double)
 c_int_p = ctypes.POINTER(ctypes.c_int)
 c_ll_p = ctypes.POINTER(ctypes.c_longlong)
+_vp, _ll, _d = ctypes.c_void_p, ctypes.c_longlong, ctypes.c_double
 
 
 class HgpuConfig(ctypes.Structure):
@@ -64,6 +65,19 @@ SIGNATURES = {
     "hgpu_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c_double_p, c_ll_p]),
     "hgpu_con2prim_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    # batched con-to-prim and its network (pointers are addresses: ints)
+    "hgpu_c2p_samples": (ctypes.c_int, [_vp, _ll, ctypes.c_ulonglong, _d, _d, _vp, _vp, _vp, _vp]),
+    "hgpu_c2p_workspace_bytes": (_ll, [_ll, ctypes.c_int]),
+    "hgpu_c2p_solve": (ctypes.c_int, [_vp, _ll, _d, _d, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, _vp, _ll, _vp, _vp, _vp]),
+    "hgpu_mlp_sizes": (ctypes.c_int, [c_int_p]),
+    "hgpu_mlp_features": (ctypes.c_int, [_vp, _ll, _d, _d, _vp, _vp, _vp, _vp, _vp]),
+    "hgpu_mlp_forward": (ctypes.c_int, [_vp, _ll, _vp, _vp, _vp]),
+    "hgpu_mlp_workspace_bytes": (_ll, [_ll]),
+    "hgpu_mlp_grad": (ctypes.c_int, [_vp, _ll, _vp, _vp, _vp, _vp, _ll, _vp, _vp]),
+    "hgpu_mlp_init": (ctypes.c_int, [_vp, ctypes.c_ulonglong, _ll, _vp, _vp, _vp]),
+    "hgpu_mlp_train": (ctypes.c_int, [_vp, _ll, _vp, _vp, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_float, _vp, _ll, _vp, _vp]),
 }
 
 _cache = {}

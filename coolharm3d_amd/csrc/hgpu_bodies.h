@@ -397,32 +397,38 @@ HD double floor_uu(const Params& P, const Ptrs& X, int i, long long c) {
 //   stage 1 (corrector): p_h = ph, U^n recomputed from p, guess = p, result into p
 // MIXED reads the FP32 face fluxes and EMFs and sums the divergence with
 // compensation; otherwise the FP64 arrays with the reference's sum.
+//
+// The work is split so the device can run it in two passes: zone_conserved()
+// + recover_fast() finish every zone whose 2D Newton inversion succeeds
+// without the entropy fix; the others keep U^{n+1} and go through the full
+// recover_primitives() tree in a second, compacted pass (recover_zone()).
+
+// U^{n+1} of the zone (src/step_ch.c:330-375)
 template <bool MIXED>
-HD void update_zone(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
-                    FailRec& fr) {
+HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int i, int j, int k,
+                       const Geom* g, FailRec& fr, double* U) {
   const long long NT = P.nt;
   const long long c = cidx(P, i, j, k);
-  const double Dt = (stage == 0) ? 0.5*dt : dt;
-  const Geom* g = geom_at(P, X, i, j, k, CENT);
-
-  double p_raw[NP], pi[NP], phz[NP];
-  for (int v = 0; v < NP; v++) {
-    p_raw[v] = X.p[v*NT + c];
-    pi[v] = rt_val(P, X, v, i, p_raw[v]);
+  {
+    // U^n = primtoflux(p_i) (src/step_ch.c:339-341)
+    double pi[NP];
+#pragma unroll
+    for (int v = 0; v < NP; v++) pi[v] = rt_val(P, X, v, i, X.p[v*NT + c]);
+    State q;
+    int gf = get_state(pi, g, P.gam, &q);
+    if (stage == 0 && gf) fr.nf[NFAIL_GAMMA_CALC]++;
+    primtoflux(pi, &q, 0, g, U);
   }
-  // U^n = primtoflux(p_i) (src/step_ch.c:339-341)
-  State q;
-  int gf = get_state(pi, g, P.gam, &q);
-  if (stage == 0 && gf) fr.nf[NFAIL_GAMMA_CALC]++;
-  double U[NP];
-  primtoflux(pi, &q, 0, g, U);
-
-  if (stage == 0) { for (int v = 0; v < NP; v++) phz[v] = pi[v]; }
-  else { for (int v = 0; v < NP; v++) phz[v] = rt_val(P, X, v, i, X.ph[v*NT + c]); }
-
+  phase_fence();
   double dU[NP];
-  fr.nf[NFAIL_GAMMA_CALC] += source(P, X, phz, g, i, j, k, dU);
-
+  {
+    double phz[NP];
+    const double* src = (stage == 0) ? X.p : X.ph;
+#pragma unroll
+    for (int v = 0; v < NP; v++) phz[v] = rt_val(P, X, v, i, src[v*NT + c]);
+    fr.nf[NFAIL_GAMMA_CALC] += source(P, X, phz, g, i, j, k, dU);
+  }
+  phase_fence();
   if (MIXED) {
 #pragma unroll
     for (int l = 0; l < NP; l++) {
@@ -446,19 +452,97 @@ HD void update_zone(const Params& P, const Ptrs& X, int stage, double dt, int i,
           + dU[l]);
     }
   }
+}
 
-  double pf[NP];
-  if (stage == 0) { for (int v = 0; v < NP; v++) pf[v] = p_raw[v]; }
-  else { for (int v = 0; v < NP; v++) pf[v] = pi[v]; }
+// Newton guess of recover_primitives(): the predictor's result array holds p
+// before the rescale round trip, the corrector's holds p after it.
+HD void zone_guess(const Params& P, const Ptrs& X, int stage, int i, long long c, double* pf) {
+  const long long NT = P.nt;
+#pragma unroll
+  for (int v = 0; v < NP; v++) {
+    const double x = X.p[v*NT + c];
+    pf[v] = (stage == 0) ? x : rt_val(P, X, v, i, x);
+  }
+}
 
-  double pgam = 1.;
-  recover_primitives<MIXED>(P, X, i, j, k, Dt, pf, U, p_raw, g, floor_rho(P, X, i, c), floor_uu(P, X, i, c),
-                            &pgam, fr);
+// The common branch of recover_primitives() (src/step_ch.c:832-1020): the 2D
+// inversion succeeds with an acceptable gamma and the entropy fix is not
+// needed.  Returns false, with pf, fr and the zone untouched in memory, when
+// the zone needs the full tree.
+HD bool recover_fast(const Params& P, const Ptrs& X, int i, int j, int k, double* pf, const double* U,
+                     const Geom* g, double rhoflr, double uuflr, double* p_gamma, FailRec& fr) {
+  double pt[NP];
+#pragma unroll
+  for (int l = 0; l < NP; l++) pt[l] = pf[l];
+  double gamma = 1., bsq = 0.;
+  int ret = utoprim_2d_fast(U, g, P.gam, P.gam_m1_o_gam, pt, &gamma, &bsq, nullptr);
+  if (ret || check_gamma(gamma) == 2) return false;
+  if ((pt[UU] < BETA_MIN * bsq) || (pt[RHO] < BETA_MIN * bsq)) return false;
+#pragma unroll
+  for (int l = 0; l < NP; l++) pf[l] = pt[l];
+  fail_set(P, fr, NFAIL_USE_FULL_INV, 0, i, j, k);
+  int fl = 0;
+  if (pf[RHO] < rhoflr) { pf[RHO] = rhoflr; fl = 1; }
+  if (pf[UU] < uuflr) { pf[UU] = uuflr; fl = 1; }
+  if (fl) fail_set(P, fr, NFAIL_FLOOR, 0, i, j, k);
+  *p_gamma = gamma;
+  if (check_gamma(gamma)) fail_set(P, fr, NFAIL_GAMMA_MAX, 2, i, j, k);
+  return true;
+}
 
+HD void store_zone(const Params& P, const Ptrs& X, int stage, long long c, const double* pf, double pgam,
+                   const FailRec& fr) {
+  const long long NT = P.nt;
   double* dst = (stage == 0) ? X.ph : X.p;
+#pragma unroll
   for (int v = 0; v < NP; v++) dst[v*NT + c] = pf[v];
   X.gamma[c] = pgam;
   if (fr.pflag) X.pflag[c] = fr.pflag;
+}
+
+// Full recover_primitives() tree for a zone whose U^{n+1} is known.
+template <bool MIXED>
+HD void recover_zone(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k, const double* U,
+                     FailRec& fr) {
+  const long long NT = P.nt;
+  const long long c = cidx(P, i, j, k);
+  const double Dt = (stage == 0) ? 0.5*dt : dt;
+  const Geom* g = geom_at(P, X, i, j, k, CENT);
+  double pf[NP], p_raw[NP];
+  zone_guess(P, X, stage, i, c, pf);
+#pragma unroll
+  for (int v = 0; v < NP; v++) p_raw[v] = X.p[v*NT + c];
+  double pgam = 1.;
+  recover_primitives<MIXED>(P, X, i, j, k, Dt, pf, U, p_raw, g, floor_rho(P, X, i, c), floor_uu(P, X, i, c),
+                            &pgam, fr);
+  store_zone(P, X, stage, c, pf, pgam, fr);
+}
+
+// Whole zone in one pass (the host build's sequence; the device runs the two
+// passes).  Returns false and leaves U filled when the full tree is needed.
+template <bool MIXED>
+HD bool update_zone_fast(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
+                         FailRec& fr, double* U) {
+  const long long c = cidx(P, i, j, k);
+  const double Dt = (stage == 0) ? 0.5*dt : dt;
+  const Geom* g = geom_at(P, X, i, j, k, CENT);
+  zone_conserved<MIXED>(P, X, stage, Dt, i, j, k, g, fr, U);
+  phase_fence();
+  double pf[NP];
+  zone_guess(P, X, stage, i, c, pf);
+  double pgam = 1.;
+  if (!recover_fast(P, X, i, j, k, pf, U, g, floor_rho(P, X, i, c), floor_uu(P, X, i, c), &pgam, fr))
+    return false;
+  store_zone(P, X, stage, c, pf, pgam, fr);
+  return true;
+}
+
+template <bool MIXED>
+HD void update_zone(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
+                    FailRec& fr) {
+  double U[NP];
+  if (!update_zone_fast<MIXED>(P, X, stage, dt, i, j, k, fr, U))
+    recover_zone<MIXED>(P, X, stage, dt, i, j, k, U, fr);
 }
 
 // --------------------------------------------------------------------------

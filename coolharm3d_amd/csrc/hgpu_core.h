@@ -221,6 +221,9 @@ struct Api {
     rc |= BE::alloc(c, (void**)&X.rhoflr, sizeof(double) * T.rhoflr.size());
     rc |= BE::alloc(c, (void**)&X.uuflr, sizeof(double) * T.uuflr.size());
     rc |= BE::alloc(c, (void**)&X.gamma, sizeof(double) * NT);
+    const long long nint = (long long)P.n1 * P.n2 * P.n3;
+    rc |= BE::alloc(c, (void**)&X.Uslow, sizeof(double) * NP * NT);
+    rc |= BE::alloc(c, (void**)&X.slow_idx, sizeof(unsigned) * nint);
     rc |= BE::alloc(c, (void**)&X.pflag, sizeof(int) * NT);
     rc |= BE::alloc(c, (void**)&X.nfail, sizeof(int) * (P.track_nfail ? N_NFAIL * NT : 1));
     rc |= BE::alloc(c, (void**)&X.status, sizeof(unsigned long long) * ST_WORDS);
@@ -262,7 +265,7 @@ struct Api {
     Ptrs& X = c->X;
     void* ptrs[] = {X.p, X.ph, X.F, X.emf, (void*)X.geom, (void*)X.conn, (void*)X.omega, (void*)X.tr,
                     (void*)X.rscale, (void*)X.irscale, (void*)X.rhoflr, (void*)X.uuflr, X.gamma,
-                    X.pflag, X.nfail, X.status, X.scal, X.Ff, X.emff, (void*)X.geomf};
+                    X.pflag, X.nfail, X.status, X.scal, X.Ff, X.emff, (void*)X.geomf, X.Uslow, X.slow_idx};
     for (void* p : ptrs) if (p) BE::release(c, p);
     BE::fini(c);
     delete c;
@@ -334,6 +337,7 @@ struct Api {
     for (int d = 0; d < 3; d++) rc |= BE::flux(c, src, d);
     rc |= BE::emf(c);
     rc |= BE::update(c, stage);
+    rc |= BE::recover_queued(c, stage);
     rc |= bounds(c, dst, 0);
     rc |= BE::fixup_interp(c, dst, PFLAG_INTERP_V);
     rc |= BE::fixup_interp(c, dst, PFLAG_INTERP_PRIM);

@@ -61,7 +61,7 @@ inline void x_of_xp(const CoordParams& C, const double* xp, double* x, double J[
 }
 
 // trigvalues(), src/metric.c:29-40 (COORDSINGFIX)
-inline void trigvalues(double th, double* cth, double* sth) {
+HD void trigvalues(double th, double* cth, double* sth) {
   *sth = sin(th);
   *cth = cos(th);
   if (fabs(*sth) < 1.E-16) {
@@ -72,33 +72,35 @@ inline void trigvalues(double th, double* cth, double* sth) {
 
 // Kerr-Schild metric in (t, r, th, phi): ks_gcov_func(), ks_gcon_func(),
 // bl_gdet_func() of src/metric.c:1168-1252, :523-539
-inline void ks_metric(const CoordParams& C, double r, double th, double gcov[4][4], double gcon[4][4], double* gdet) {
+HD void ks_metric_rth(double a, double M, double r, double th, double gcov[4][4], double gcon[4][4],
+                      double* gdet) {
+  const double asq = a*a;
   double t4, t12;
   trigvalues(th, &t4, &t12);
   {
-    double t2 = r*r, t3 = C.asq, t5 = t4*t4, t7 = t2 + t3*t5;
-    double t9 = C.M*r/t7, t10 = 2.0*t9, t13 = t12*t12;
-    for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) gcov[a][b] = 0.;
+    double t2 = r*r, t3 = asq, t5 = t4*t4, t7 = t2 + t3*t5;
+    double t9 = M*r/t7, t10 = 2.0*t9, t13 = t12*t12;
+    for (int p = 0; p < 4; p++) for (int q = 0; q < 4; q++) gcov[p][q] = 0.;
     gcov[0][0] = t10 - 1.0;
     gcov[0][1] = t10;
-    gcov[0][3] = -2.0*t9*C.a*t13;
+    gcov[0][3] = -2.0*t9*a*t13;
     gcov[1][0] = t10;
     gcov[1][1] = 1.0 + t10;
-    gcov[1][3] = -C.a*gcov[1][1]*t13;
+    gcov[1][3] = -a*gcov[1][1]*t13;
     gcov[2][2] = t7;
     gcov[3][0] = gcov[0][3];
     gcov[3][1] = gcov[1][3];
     gcov[3][3] = t13*(t2 + t3*(1.0 + 2.0*t9*t13));
   }
   {
-    double t1 = C.M*r, t2 = r*r, t3 = C.asq, t5 = t4*t4;
+    double t1 = M*r, t2 = r*r, t3 = asq, t5 = t4*t4;
     double t8 = 1/(t2 + t3*t5), t10 = 2.0*t1*t8, t14 = t12*t12;
-    for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) gcon[a][b] = 0.;
+    for (int p = 0; p < 4; p++) for (int q = 0; q < 4; q++) gcon[p][q] = 0.;
     gcon[0][0] = -1.0 - t10;
     gcon[0][1] = t10;
     gcon[1][0] = t10;
     gcon[1][1] = (t2 - 2.0*t1 + t3)*t8;
-    gcon[1][3] = C.a*t8;
+    gcon[1][3] = a*t8;
     gcon[2][2] = t8;
     gcon[3][1] = gcon[1][3];
     gcon[3][3] = t8/t14;
@@ -107,9 +109,27 @@ inline void ks_metric(const CoordParams& C, double r, double th, double gcov[4][
     double cth, sth;
     trigvalues(th, &cth, &sth);
     double cos_2th = (cth - sth)*(cth + sth);
-    double a2 = C.asq, r2 = r*r;
+    double a2 = asq, r2 = r*r;
     *gdet = r2*fabs(sth)*(1. + 0.5*(a2/r2)*(1. + cos_2th));
   }
+}
+
+inline void ks_metric(const CoordParams& C, double r, double th, double gcov[4][4], double gcon[4][4], double* gdet) {
+  ks_metric_rth(C.a, C.M, r, th, gcov, gcon, gdet);
+}
+
+// Lapse, shift and normal observer of a record whose gcov, gcon and g are set
+// (calc_all_geom(), src/metric.c:4235-4250).
+HD void geom_finish(Geom* G) {
+  G->g_inv = 1./G->g;
+  G->ncon[0] = sqrt(-G->gcon[0][0]);
+  G->alpha = 1./G->ncon[0];
+  G->ncon[1] = -G->alpha * G->gcon[0][1];
+  G->ncon[2] = -G->alpha * G->gcon[0][2];
+  G->ncon[3] = -G->alpha * G->gcon[0][3];
+  G->beta[0] = -G->gcon[0][1]/G->gcon[0][0];
+  G->beta[1] = -G->gcon[0][2]/G->gcon[0][0];
+  G->beta[2] = -G->gcon[0][3]/G->gcon[0][0];
 }
 
 // Geometry record at (i,j,k,pos): calc_all_geom(), src/metric.c:4111-4233,

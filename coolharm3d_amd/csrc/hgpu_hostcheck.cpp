@@ -83,6 +83,7 @@ struct HostBE {
     }
     return 0;
   }
+  static int recover_queued(hgpu_ctx*, int) { return 0; }   // update() runs the whole tree
   static int update(hgpu_ctx* c, int stage) {
     Box b = interior_box(c->P);
     const double dt = c->X.scal[SC_DT];

@@ -7,6 +7,7 @@
 // Failure counters are summed across the wavefront before one atomic per wave.
 #include "hgpu_core.h"
 #include <hip/hip_runtime.h>
+#include <algorithm>
 
 namespace hgpu {
 
@@ -169,18 +170,84 @@ __global__ __launch_bounds__(BLOCK) void k_emf_mixed(Params P, Ptrs X, Box b, Ti
   emf_edge_mixed(P, X, i, j, k);
 }
 
+// Conserved update U^{n+1} of every interior zone into the U buffer
+// ([8][NT], indexed like the state).
 template <bool MIXED>
-__global__ __launch_bounds__(BLOCK) void k_update(Params P, Ptrs X, int stage, Box b, Tiling T) {
+__global__ __launch_bounds__(BLOCK) void k_conserved(Params P, Ptrs X, int stage, Box b, Tiling T) {
   int i, j, k;
   const bool active = tile_index(b, T, i, j, k);
   FailRec fr; fr.clear();
   long long c = 0;
   if (active) {
     const double dt = X.scal[SC_DT];
-    update_zone<MIXED>(P, X, stage, dt, i, j, k, fr);
+    const double Dt = (stage == 0) ? 0.5*dt : dt;
     c = cidx(P, i, j, k);
+    double U[NP];
+    zone_conserved<MIXED>(P, X, stage, Dt, i, j, k, geom_at(P, X, i, j, k, CENT), fr, U);
+#pragma unroll
+    for (int l = 0; l < NP; l++) X.Uslow[l*P.nt + c] = U[l];
+  }
+  commit_fail(P, X, fr, active, c, MASK_GAMMA);
+}
+
+// The common con-to-prim branch; zones that need the full recover_primitives()
+// tree are queued (wave-aggregated slot allocation) for k_recover.
+__global__ __launch_bounds__(BLOCK) void k_invert(Params P, Ptrs X, int stage, Box b, Tiling T) {
+  int i, j, k;
+  const bool active = tile_index(b, T, i, j, k);
+  FailRec fr; fr.clear();
+  long long c = 0;
+  bool queue = false;
+  if (active) {
+    c = cidx(P, i, j, k);
+    double U[NP], pf[NP];
+#pragma unroll
+    for (int l = 0; l < NP; l++) U[l] = X.Uslow[l*P.nt + c];
+    zone_guess(P, X, stage, i, c, pf);
+    double pgam = 1.;
+    if (recover_fast(P, X, i, j, k, pf, U, geom_at(P, X, i, j, k, CENT), floor_rho(P, X, i, c),
+                     floor_uu(P, X, i, c), &pgam, fr))
+      store_zone(P, X, stage, c, pf, pgam, fr);
+    else
+      queue = true;
+  }
+  const unsigned long long ballot = __ballot(queue);
+  if (ballot) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)ballot) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = (unsigned)atomicAdd(&X.status[ST_SLOW], (unsigned long long)__popcll(ballot));
+    base = __shfl(base, leader, 64);
+    if (queue) X.slow_idx[base + (unsigned)__popcll(ballot & ((1ull << lane) - 1ull))] = (unsigned)c;
   }
   commit_fail(P, X, fr, active, c, MASK_ALL);
+}
+
+// Full con-to-prim tree for the queued zones: a fixed grid strides over the
+// queue, so every wave works on queued zones only.
+template <bool MIXED>
+__global__ __launch_bounds__(BLOCK) void k_recover(Params P, Ptrs X, int stage) {
+  const unsigned n = (unsigned)X.status[ST_SLOW];
+  const unsigned stride = gridDim.x * BLOCK;
+  const unsigned rounds = (n + stride - 1) / stride;
+  const double dt = X.scal[SC_DT];
+  for (unsigned r = 0; r < rounds; r++) {
+    const unsigned slot = r * stride + blockIdx.x * BLOCK + threadIdx.x;
+    const bool active = slot < n;
+    FailRec fr; fr.clear();
+    long long c = 0;
+    if (active) {
+      c = X.slow_idx[slot];
+      const int k = (int)(c % P.n3t);
+      const int j = (int)((c / P.n3t) % P.n2t);
+      const int i = (int)(c / ((long long)P.n3t * P.n2t));
+      double U[NP];
+#pragma unroll
+      for (int l = 0; l < NP; l++) U[l] = X.Uslow[l*P.nt + c];
+      recover_zone<MIXED>(P, X, stage, dt, i, j, k, U, fr);
+    }
+    commit_fail(P, X, fr, active, c, MASK_ALL);
+  }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_bounds(Params P, Ptrs X, double* S, int dir, int skip_mpi, Box b) {
@@ -431,9 +498,20 @@ struct GpuBE {
     Box b = interior_box(c->P);
     Scope s(c, "update");
     const Tiling T = make_tiling(b);
-    if (c->P.mixed) hipLaunchKernelGGL(k_update<true>, dim3(T.nblocks), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage, b, T);
-    else hipLaunchKernelGGL(k_update<false>, dim3(T.nblocks), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage, b, T);
+    hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_SLOW, 1);
+    hipStream_t st = priv(c)->stream;
+    if (c->P.mixed) hipLaunchKernelGGL(k_conserved<true>, dim3(T.nblocks), dim3(BLOCK), 0, st, c->P, c->X, stage, b, T);
+    else hipLaunchKernelGGL(k_conserved<false>, dim3(T.nblocks), dim3(BLOCK), 0, st, c->P, c->X, stage, b, T);
+    hipLaunchKernelGGL(k_invert, dim3(T.nblocks), dim3(BLOCK), 0, st, c->P, c->X, stage, b, T);
     return hip_check(hipGetLastError(), "k_update");
+  }
+  static int recover_queued(hgpu_ctx* c, int stage) {
+    Scope s(c, "recover");
+    const long long nint = (long long)c->P.n1 * c->P.n2 * c->P.n3;
+    const unsigned nb = (unsigned)std::min<long long>((nint + BLOCK - 1) / BLOCK, 1024);
+    if (c->P.mixed) hipLaunchKernelGGL(k_recover<true>, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage);
+    else hipLaunchKernelGGL(k_recover<false>, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage);
+    return hip_check(hipGetLastError(), "k_recover");
   }
   static int bounds_dir(hgpu_ctx* c, double* S, int dir, int skip_mpi) {
     Box b = ghost_box(c->P, dir);

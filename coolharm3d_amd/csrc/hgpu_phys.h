@@ -59,6 +59,15 @@ HD void atomic_or_u64(unsigned long long* a, unsigned long long v) {
 
 HD bool is_finite(double x) { return isfinite(x); }
 
+// Compiler-level memory fence between phases of a long kernel body: values
+// loaded before it are not kept in registers across it (they are loaded
+// again, from cache), which shortens live ranges.  No effect on results.
+HD void phase_fence() {
+#ifdef __HIP_DEVICE_COMPILE__
+  asm volatile("" ::: "memory");
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // Metric contractions, src/metric.c:338-383
 HD void lower(const double* ucon, const Geom* g, double* ucov) {
@@ -269,46 +278,69 @@ HD void u2d_func_vsq(const U2d& s, const double* x, double* dx, double* f, doubl
   *f = -0.5 * (*df);
 }
 
-// general_newton_raphson(), src/utoprim_2d_fast.c:489-592; *iters = steps taken
-HD int u2d_newton(const U2d& s, double* x, int* iters) {
-  double f = 1., df = 1., dx[2], x_old[2];
-  double errx = 1.;
-  int n_iter = 0, i_extra = 0, doing_extra = 0;
+// general_newton_raphson(), src/utoprim_2d_fast.c:489-592, as a resumable
+// iteration: u2d_newton_step() performs one pass of the reference's loop body
+// and returns whether to keep iterating, so a persistent kernel can interleave
+// cells; u2d_newton() runs it to completion.
+struct NewtonIter {
+  double x[2];
+  double f, df, errx;
+  int n_iter, i_extra, doing_extra;
+};
+
+HD void u2d_newton_init(NewtonIter& it, const double* x) {
+  it.x[0] = x[0]; it.x[1] = x[1];
+  it.f = 1.; it.df = 1.; it.errx = 1.;
+  it.n_iter = 0; it.i_extra = 0; it.doing_extra = 0;
+}
+
+HD bool u2d_newton_step(const U2d& s, NewtonIter& it) {
   const double dv = (1. - 1.e-15);
-  x_old[0] = x[0]; x_old[1] = x[1];
-  int keep_iterating = 1;
-  while (keep_iterating) {
-    u2d_func_vsq(s, x, dx, &f, &df);
-    errx = 0.;
-    x_old[0] = x[0]; x_old[1] = x[1];
-    x[0] += dx[0];
-    x[1] += dx[1];
-    errx = (x[0] == 0.) ? fabs(dx[0]) : fabs(dx[0]/x[0]);
-    if (x[0] < 0.) { x[0] = fabs(x[0]); }
-    else { if (x[0] > W_TOO_BIG) { x[0] = x_old[0]; } }
-    if (x[1] < 0.) { x[1] = 0.; }
-    else { if (x[1] > 1.) { x[1] = dv; } }
-    if ((fabs(errx) <= NEWT_TOL) && (doing_extra == 0) && (EXTRA_NEWT_ITER > 0)) doing_extra = 1;
-    if (doing_extra == 1) i_extra++;
-    if (((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) ||
-        (n_iter >= (MAX_NEWT_ITER - 1))) keep_iterating = 0;
-    n_iter++;
-  }
-  if (iters) *iters = n_iter;
-  if ((!is_finite(f)) || (!is_finite(df))) return 2;
-  if (fabs(errx) <= NEWT_TOL) return 0;
-  else if ((fabs(errx) <= MIN_NEWT_TOL) && (fabs(errx) > NEWT_TOL)) return 0;
+  double dx[2], x_old[2];
+  u2d_func_vsq(s, it.x, dx, &it.f, &it.df);
+  it.errx = 0.;
+  x_old[0] = it.x[0]; x_old[1] = it.x[1];
+  it.x[0] += dx[0];
+  it.x[1] += dx[1];
+  it.errx = (it.x[0] == 0.) ? fabs(dx[0]) : fabs(dx[0]/it.x[0]);
+  if (it.x[0] < 0.) { it.x[0] = fabs(it.x[0]); }
+  else { if (it.x[0] > W_TOO_BIG) { it.x[0] = x_old[0]; } }
+  if (it.x[1] < 0.) { it.x[1] = 0.; }
+  else { if (it.x[1] > 1.) { it.x[1] = dv; } }
+  if ((fabs(it.errx) <= NEWT_TOL) && (it.doing_extra == 0) && (EXTRA_NEWT_ITER > 0)) it.doing_extra = 1;
+  if (it.doing_extra == 1) it.i_extra++;
+  bool keep = !(((fabs(it.errx) <= NEWT_TOL) && (it.doing_extra == 0)) || (it.i_extra > EXTRA_NEWT_ITER) ||
+                (it.n_iter >= (MAX_NEWT_ITER - 1)));
+  it.n_iter++;
+  return keep;
+}
+
+HD int u2d_newton_result(const NewtonIter& it) {
+  if ((!is_finite(it.f)) || (!is_finite(it.df))) return 2;
+  if (fabs(it.errx) <= NEWT_TOL) return 0;
+  else if ((fabs(it.errx) <= MIN_NEWT_TOL) && (fabs(it.errx) > NEWT_TOL)) return 0;
   return 1;
 }
 
-// Utoprim_2d_fast() + Utoprim_new_body(), src/utoprim_2d_fast.c:135-408.
-// prim: guess on input; rho,u,u~ replaced only on success, B always set from U
-// (as the reference does before the solve).  gam: adiabatic index,
-// gmog = (gam-1)/gam (gam_m1_o_gam, src/main.c:299).  w_guess > 0 replaces the
-// W estimate built from the guess (learned initial guess of the batched solver).
-HD int utoprim_2d_fast(const double* U, const Geom* geom, double gam, double gmog, double* prim,
-                       double* gamma_out, double* bsq_out, int* iters, double w_guess = -1.) {
-  if (iters) *iters = 0;
+HD int u2d_newton(const U2d& s, double* x, int* iters) {
+  NewtonIter it;
+  u2d_newton_init(it, x);
+  while (u2d_newton_step(s, it)) {}
+  x[0] = it.x[0]; x[1] = it.x[1];
+  if (iters) *iters = it.n_iter;
+  return u2d_newton_result(it);
+}
+
+// Quantities of Utoprim_new_body() that the final step needs besides the
+// Newton solution (src/utoprim_2d_fast.c:395-406).
+struct U2dAux {
+  double QdotB, Qcon[4], Bcon[4], ncon[4];
+};
+
+// Utoprim_2d_fast() + Utoprim_new_body() up to the initial guess
+// (src/utoprim_2d_fast.c:155-300): the invariants of U the Newton solve works
+// with.  Sets prim's B components.  Returns -100 when U[RHO] <= 0.
+HD int u2d_prepare(const double* U, const Geom* geom, double gmog, double* prim, U2d& s, U2dAux& aux) {
   if (U[0] <= 0.) return -100;
   double inv_gdet = geom->g_inv;
   double alpha = geom->alpha;
@@ -326,9 +358,9 @@ HD int utoprim_2d_fast(const double* U, const Geom* geom, double gam, double gmo
   Ut[B2] = geomfactor * U[B2];
   Ut[B3] = geomfactor * U[B3];
 
-  U2d s;
   s.gmog = gmog;
-  double Bcon[4] = {0., Ut[B1], Ut[B2], Ut[B3]};
+  double* Bcon = aux.Bcon;
+  Bcon[0] = 0.; Bcon[1] = Ut[B1]; Bcon[2] = Ut[B2]; Bcon[3] = Ut[B3];
   s.Bsq = geom->gcov[1][1]*Bcon[1]*Bcon[1]
         + geom->gcov[2][2]*Bcon[2]*Bcon[2]
         + geom->gcov[3][3]*Bcon[3]*Bcon[3]
@@ -336,16 +368,36 @@ HD int utoprim_2d_fast(const double* U, const Geom* geom, double gam, double gmo
            + geom->gcov[1][3]*Bcon[1]*Bcon[3]
            + geom->gcov[2][3]*Bcon[2]*Bcon[3]);
   double Qcov[4] = {Ut[UU], Ut[U1], Ut[U2], Ut[U3]};
-  double Qcon[4];
-  raise_v(Qcov, geom, Qcon);
-  double QdotB = Qcov[1]*Bcon[1] + Qcov[2]*Bcon[2] + Qcov[3]*Bcon[3];
-  s.QdotBsq = QdotB*QdotB;
-  s.Qdotn = -alpha*Qcon[0];
-  double Qsq = dot4(Qcov, Qcon);
+  raise_v(Qcov, geom, aux.Qcon);
+  aux.QdotB = Qcov[1]*Bcon[1] + Qcov[2]*Bcon[2] + Qcov[3]*Bcon[3];
+  s.QdotBsq = aux.QdotB*aux.QdotB;
+  s.Qdotn = -alpha*aux.Qcon[0];
+  double Qsq = dot4(Qcov, aux.Qcon);
   s.Qtsq = Qsq + s.Qdotn*s.Qdotn;
   s.D = Ut[RHO];
   s.half_Bsq = 0.5*s.Bsq;
+  for (int l = 0; l < 4; l++) aux.ncon[l] = geom->ncon[l];
+  return 0;
+}
 
+// Initial (W, v^2) from an estimate W_last (src/utoprim_2d_fast.c:313-330):
+// W is raised tenfold until it lies above the physical root's lower bound.
+// vsq_guess >= 0 replaces v^2(W) when no increase was needed.
+HD void u2d_guess_w(const U2d& s, double W_last, double* x2d, double vsq_guess = -1.) {
+  int i_increase = 0;
+  while (((W_last*W_last*W_last * (W_last + 2.*s.Bsq)
+           - s.QdotBsq*(2.*W_last + s.Bsq)) <= W_last*W_last*(s.Qtsq - s.Bsq*s.Bsq))
+         && (i_increase < 10)) {
+    W_last *= 10.;
+    i_increase++;
+  }
+  x2d[0] = fabs(W_last);
+  x2d[1] = (vsq_guess >= 0. && i_increase == 0) ? vsq_guess : u2d_x1_of_x0(s, W_last);
+}
+
+// The reference's initial guess from the incoming primitives' u and u~
+// (src/utoprim_2d_fast.c:302-330).  Returns 2 when their u~^2 is unusable.
+HD int u2d_guess(const U2d& s, const Geom* geom, double gam, const double* prim, double* x2d) {
   double utsq = geom->gcov[1][1]*prim[U1]*prim[U1]
               + geom->gcov[2][2]*prim[U2]*prim[U2]
               + geom->gcov[3][3]*prim[U3]*prim[U3]
@@ -360,40 +412,57 @@ HD int utoprim_2d_fast(const double* U, const Geom* geom, double gam, double gmo
   double u = prim[UU];
   double p = (gam - 1.)*u;
   double w = rho0 + u + p;
-  double W_last = w*gammasq;
-  if (w_guess > 0.) W_last = w_guess;
-  int i_increase = 0;
-  while (((W_last*W_last*W_last * (W_last + 2.*s.Bsq)
-           - s.QdotBsq*(2.*W_last + s.Bsq)) <= W_last*W_last*(s.Qtsq - s.Bsq*s.Bsq))
-         && (i_increase < 10)) {
-    W_last *= 10.;
-    i_increase++;
-  }
-  double x2d[2];
-  x2d[0] = fabs(W_last);
-  x2d[1] = u2d_x1_of_x0(s, W_last);
-  int retval = u2d_newton(s, x2d, iters);
+  u2d_guess_w(s, w*gammasq, x2d);
+  return 0;
+}
+
+HD int u2d_setup(const double* U, const Geom* geom, double gam, double gmog, double* prim, U2d& s,
+                 U2dAux& aux, double* x2d) {
+  int rc = u2d_prepare(U, geom, gmog, prim, s, aux);
+  if (rc) return rc;
+  return u2d_guess(s, geom, gam, prim, x2d);
+}
+
+// Utoprim_new_body() after the Newton solve (src/utoprim_2d_fast.c:345-408).
+HD int u2d_finish(const U2d& s, const U2dAux& aux, double gam, const double* x2d, int retval, double* prim,
+                  double* gamma_out, double* bsq_out) {
   double W = x2d[0];
   double vsq = x2d[1];
   if ((retval != 0) || (W == FAIL_VAL)) return retval*100 + 1;
   if (W <= 0. || W > W_TOO_BIG) return 3;
   if (vsq >= 1.) return 4;
   double gtmp = sqrt(1. - vsq);
-  gamma = 1./gtmp;
-  rho0 = s.D * gtmp;
+  double gamma = 1./gtmp;
+  double rho0 = s.D * gtmp;
   *gamma_out = gamma;
-  w = W * (1. - vsq);
-  p = (gam - 1.)*(w - rho0)/gam;        // pressure_rho0_w, src/u2p_util.c:387-390
-  u = w - (rho0 + p);
+  double w = W * (1. - vsq);
+  double p = (gam - 1.)*(w - rho0)/gam;        // pressure_rho0_w, src/u2p_util.c:387-390
+  double u = w - (rho0 + p);
   prim[RHO] = rho0;
   prim[UU] = u;
   double g_o_WBsq = gamma/(W + s.Bsq);
-  double QdB_o_W = QdotB / W;
+  double QdB_o_W = aux.QdotB / W;
   *bsq_out = s.Bsq * (1. - vsq) + QdB_o_W*QdB_o_W;
-  prim[U1] = g_o_WBsq * (Qcon[1] + geom->ncon[1] * s.Qdotn + QdB_o_W*Bcon[1]);
-  prim[U2] = g_o_WBsq * (Qcon[2] + geom->ncon[2] * s.Qdotn + QdB_o_W*Bcon[2]);
-  prim[U3] = g_o_WBsq * (Qcon[3] + geom->ncon[3] * s.Qdotn + QdB_o_W*Bcon[3]);
+  prim[U1] = g_o_WBsq * (aux.Qcon[1] + aux.ncon[1] * s.Qdotn + QdB_o_W*aux.Bcon[1]);
+  prim[U2] = g_o_WBsq * (aux.Qcon[2] + aux.ncon[2] * s.Qdotn + QdB_o_W*aux.Bcon[2]);
+  prim[U3] = g_o_WBsq * (aux.Qcon[3] + aux.ncon[3] * s.Qdotn + QdB_o_W*aux.Bcon[3]);
   return 0;
+}
+
+// Utoprim_2d_fast() + Utoprim_new_body(), src/utoprim_2d_fast.c:135-408.
+// prim: guess on input; rho,u,u~ replaced only on success, B always set from U
+// (as the reference does before the solve).  gam: adiabatic index,
+// gmog = (gam-1)/gam (gam_m1_o_gam, src/main.c:299).
+HD int utoprim_2d_fast(const double* U, const Geom* geom, double gam, double gmog, double* prim,
+                       double* gamma_out, double* bsq_out, int* iters) {
+  if (iters) *iters = 0;
+  U2d s;
+  U2dAux aux;
+  double x2d[2];
+  int rc = u2d_setup(U, geom, gam, gmog, prim, s, aux, x2d);
+  if (rc) return rc;
+  int retval = u2d_newton(s, x2d, iters);
+  return u2d_finish(s, aux, gam, x2d, retval, prim, gamma_out, bsq_out);
 }
 
 // ---------------------------------------------------------------------------

@@ -73,6 +73,7 @@ enum {
   ST_BOUNDARY_PHYS = 3,      // boundary_phys_pflag
   ST_VMAX = 4,               // 4..7: max_char_speed[0..3] as ordered bits
   ST_NFAILTOT = 8,           // 8..22: nfailtot[15]
+  ST_SLOW = 23,              // zones queued for the full con-to-prim tree this stage
   ST_WORDS = 32
 };
 
@@ -121,6 +122,8 @@ struct Ptrs {
   const double* rhoflr;    // floor tables (floor_dim 1: [N1T], 3: [NT])
   const double* uuflr;
   double* gamma;           // p_gamma [NT]
+  double* Uslow;           // U^{n+1} of the stage [8][NT]
+  unsigned* slow_idx;      // zones queued for the full con-to-prim tree [n_interior]
   int* pflag;              // [NT]
   int* nfail;              // [15][NT]
   unsigned long long* status;  // [ST_WORDS]

@@ -128,6 +128,75 @@ int hgpu_profile_read(hgpu_ctx *ctx, const char *name, double *total_ms, long lo
 int hgpu_con2prim_batch(hgpu_ctx *ctx, long long n, const double *U, const double *geom,
                         double *prim, double *gamma, int *code, int *iters);
 
+/* ------------------------------------------------------------------------
+ * Batched con-to-prim of independent zones on Kerr-Schild records, with a
+ * learned initial guess.  No context: every call takes the stream to enqueue
+ * on (a hipStream_t, NULL = the null stream) and works only in memory the
+ * caller passes (device addresses; host addresses in the host build).  Arrays
+ * are structure-of-arrays: U[8][n], rth[2][n] (r, theta), prim[8][n].
+ * Spin a and adiabatic index gam are arguments.  n < 2^31.
+ * No counterpart in the reference: the reference inverts zone by zone inside
+ * recover_primitives() (src/step_ch.c:832) with Utoprim_2d_fast()
+ * (src/utoprim_2d_fast.c:135), which is the per-zone algorithm used here.
+ * ------------------------------------------------------------------------ */
+
+/* Samples of the inversion workload from seed: r log-uniform in [1.3, 100],
+ * theta uniform in [0.05, pi - 0.05], Lorentz factor log-uniform in [1, 50],
+ * b^2/rho log-uniform in [1e-4, 1e2], u/rho log-uniform in [1e-4, 10], rho = 1,
+ * velocity and field directions isotropic in the normal observer's frame.
+ * Writes U, rth, the true primitives prim, and (if guess != NULL) the
+ * perturbed-truth guess guess[8][n] (rho, u, u~ times 1 +- 10 %). */
+int hgpu_c2p_samples(void *stream, long long n, unsigned long long seed, double a, double gam,
+                     double *U, double *rth, double *prim, double *guess);
+
+/* Workspace hgpu_c2p_solve() needs (bytes; 0 when compact == 0). */
+long long hgpu_c2p_workspace_bytes(long long n, int compact);
+
+/* Utoprim_2d_fast() on every zone.  Initial guess: mlp == NULL -> the plain
+ * guess; else the network's guess, restarting once from the plain guess if that
+ * attempt fails.  plain_guess: 0 = the incoming prim (the reference's guess),
+ * 1 = cold (rho = D, u = 1e-3 D, u~ = 0).  compact = 1 runs a persistent first
+ * pass of budget_first Newton iterations per zone, then passes of budget_next
+ * over a compacted queue of the zones still iterating (0 = defaults); compact =
+ * 0 solves each zone to the end in one pass.  Both give identical results.
+ * Out: prim (B from U; rho, u, u~ the solution, or the plain guess where no
+ * attempt succeeded), code (the return code of Utoprim_2d_fast()), iters
+ * (Newton iterations over both attempts). */
+int hgpu_c2p_solve(void *stream, long long n, double a, double gam, const double *U, const double *rth,
+                   const float *mlp, int plain_guess, int compact, int budget_first, int budget_next,
+                   void *work, long long work_bytes, double *prim, int *code, int *iters);
+
+/* The network (FP32): 8 inputs -> 64 GELU -> 64 GELU -> 2 outputs (log(W/D),
+ * log gamma).  out[0..5] = parameter count, trainable count, offset of the
+ * trainable part, inputs, hidden width, outputs. */
+int hgpu_mlp_sizes(int *out);
+
+/* Features feat[8][n] of each zone and, if prim != NULL (solved primitives),
+ * the training targets target[2][n]. */
+int hgpu_mlp_features(void *stream, long long n, double a, double gam, const double *U, const double *rth,
+                      const double *prim, float *feat, float *target);
+
+/* Network outputs y[2][n]. */
+int hgpu_mlp_forward(void *stream, long long n, const float *w, const float *feat, float *y);
+
+/* Workspace of hgpu_mlp_grad() for n samples / hgpu_mlp_train() for a batch. */
+long long hgpu_mlp_workspace_bytes(long long batch);
+
+/* Mean loss 0.5 |o - t_n|^2 over the n samples and its gradient with respect
+ * to the trainable parameters, grad[trainable count]. */
+int hgpu_mlp_grad(void *stream, long long n, const float *w, const float *feat, const float *target,
+                  void *work, long long work_bytes, float *grad, float *loss);
+
+/* Input/output normalisation from the data and seeded initial weights. */
+int hgpu_mlp_init(void *stream, unsigned long long seed, long long n, const float *feat, const float *target,
+                  float *w);
+
+/* steps Adam steps (cosine-decayed learning rate lr) on batches of `batch`
+ * samples drawn with replacement; loss_hist[steps] gets each batch's loss. */
+int hgpu_mlp_train(void *stream, long long n, const float *feat, const float *target, unsigned long long seed,
+                   int steps, int batch, float lr, void *work, long long work_bytes, float *w,
+                   float *loss_hist);
+
 #ifdef __cplusplus
 }
 #endif

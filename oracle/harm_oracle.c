@@ -1943,6 +1943,27 @@ void orc_vchar(const double* g42, double gam, const double* prim, int dir, doubl
 int orc_utoprim_2d(const double* g42, double gam, const double* U, double* prim, double* gamma, double* bsq, int* iters) {
   return utoprim_2d_fast(U, (const geom_t*)g42, gam, prim, gamma, bsq, iters);
 }
+/* Kerr-Schild record at (r, th) in Kerr-Schild coordinates (identity
+   coordinate map): ks_gcov_func(), ks_gcon_func(), bl_gdet_func() and the lapse
+   and shift of calc_all_geom(), src/metric.c:1168-1252, :523-539, :4235-4250. */
+void orc_ks_geom_rth(double a, double r, double th, double* g42) {
+  orc_t tmp;
+  geom_t* G = (geom_t*)g42;
+  memset(&tmp, 0, sizeof(tmp));
+  tmp.c.a = a;
+  ks_gcov(&tmp, r, th, G->gcov);
+  ks_gcon(&tmp, r, th, G->gcon);
+  G->g = ks_gdet(&tmp, r, th);
+  G->g_inv = 1. / G->g;
+  G->ncon[0] = sqrt(-G->gcon[0][0]);
+  G->alpha = 1. / G->ncon[0];
+  G->ncon[1] = -G->alpha * G->gcon[0][1];
+  G->ncon[2] = -G->alpha * G->gcon[0][2];
+  G->ncon[3] = -G->alpha * G->gcon[0][3];
+  G->beta[0] = -G->gcon[0][1] / G->gcon[0][0];
+  G->beta[1] = -G->gcon[0][2] / G->gcon[0][0];
+  G->beta[2] = -G->gcon[0][3] / G->gcon[0][0];
+}
 int orc_utoprim_1d(const double* g42, double gam, const double* U, double* prim, double* gamma, double* bsq) {
   return utoprim_1d(U, (const geom_t*)g42, gam, prim, gamma, bsq);
 }

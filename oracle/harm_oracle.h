@@ -66,6 +66,8 @@ void orc_prim_to_flux(const double* geom42, double gam, const double* prim, int 
 void orc_vchar(const double* geom42, double gam, const double* prim, int dir, double* cmax, double* cmin);
 int orc_utoprim_2d(const double* geom42, double gam, const double* U, double* prim,
                    double* gamma, double* bsq, int* iters);
+/* Kerr-Schild record (struct of_geom layout) at (r, th), Kerr-Schild coordinates */
+void orc_ks_geom_rth(double a, double r, double th, double* geom42);
 int orc_utoprim_1d(const double* geom42, double gam, const double* U, double* prim,
                    double* gamma, double* bsq);
 /* face flux (HLLE when flux == 1, LF when 0) of left/right primitive states */

@@ -84,6 +84,7 @@ def lib():
         L.orc_utoprim_1d.argtypes = [dp, ctypes.c_double, dp, dp, dp, dp]
         L.orc_face_flux.argtypes = [dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp]
         L.orc_recon_line.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp]
+        L.orc_ks_geom_rth.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
         _LIB = L
     return _LIB
 
@@ -246,6 +247,13 @@ def utoprim_2d(geom42, gam, U, guess):
     ret = lib().orc_utoprim_2d(_dp(np.ascontiguousarray(geom42, np.float64)), gam,
                                _dp(np.ascontiguousarray(U, np.float64)), _dp(prim), _dp(gamma), _dp(bsq), _ip(it))
     return ret, prim, gamma[0], bsq[0], int(it[0])
+
+
+def ks_geom_rth(a, r, th):
+    """Kerr-Schild record (struct of_geom layout, 42 doubles) at (r, th)."""
+    out = np.zeros(42)
+    lib().orc_ks_geom_rth(a, r, th, _dp(out))
+    return out
 
 
 def utoprim_1d(geom42, gam, U, guess):

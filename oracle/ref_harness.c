@@ -226,3 +226,43 @@ void ref_get_diag(double *out)
 
 /* Corner divergence of B, src/diag.c:503 (divb_calc); returns max |div B|. */
 extern double divb_calc(int i, int j, int k);
+
+/* Batched inversion of independent Kerr-Schild zones with the reference's own
+   Utoprim_2d_fast() (src/utoprim_2d_fast.c:135) on records built by its metric
+   functions ks_gcov_func(), ks_gcon_func() (src/metric.c:1168-1252) and
+   bl_gdet_func() (:523-539) in Kerr-Schild coordinates, with the lapse and
+   shift of calc_all_geom() (:4221-4232).  Arrays are structure-of-arrays:
+   U[8][n], rth[2][n], prim[8][n] (guess in, result out).  Needs no ref_setup(). */
+extern int Utoprim_2d_fast(double *U, struct of_geom *geom, double *prim, double *gamma, double *bsq);
+extern void bl_gdet_func(double *x, double *gdet);
+void ref_c2p_batch(long n, double a_spin, double gam_in, const double *U, const double *rth, double *prim,
+                   int *code)
+{
+  long z;
+  int l;
+  M = 1.;
+  a = a_spin;
+  asq = a * a;
+  gam = gam_in;
+  gam_m1_o_gam = (gam - 1.) / gam;
+  for (z = 0; z < n; z++) {
+    struct of_geom geom;
+    double x[NDIM], Uz[NP], pz[NP], gamma = 1., bsq = 0.;
+    x[0] = 0.; x[1] = rth[z]; x[2] = rth[n + z]; x[3] = 0.;
+    ks_gcov_func(x, geom.gcov);
+    ks_gcon_func(x, geom.gcon);
+    bl_gdet_func(x, &(geom.g));
+    geom.g_inv = 1. / geom.g;
+    geom.ncon[0] = sqrt(-geom.gcon[0][0]);
+    geom.alpha = 1. / geom.ncon[0];
+    geom.ncon[1] = -geom.alpha * geom.gcon[0][1];
+    geom.ncon[2] = -geom.alpha * geom.gcon[0][2];
+    geom.ncon[3] = -geom.alpha * geom.gcon[0][3];
+    geom.beta[0] = -geom.gcon[0][1] / geom.gcon[0][0];
+    geom.beta[1] = -geom.gcon[0][2] / geom.gcon[0][0];
+    geom.beta[2] = -geom.gcon[0][3] / geom.gcon[0][0];
+    for (l = 0; l < NP; l++) { Uz[l] = U[l * n + z]; pz[l] = prim[l * n + z]; }
+    code[z] = Utoprim_2d_fast(Uz, &geom, pz, &gamma, &bsq);
+    for (l = 0; l < NP; l++) prim[l * n + z] = pz[l];
+  }
+}

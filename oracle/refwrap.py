@@ -29,6 +29,26 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+_C2P_LIB = {}
+
+
+def c2p_batch(a, gam, U, rth, prim, cfg="c1"):
+    """The reference's Utoprim_2d_fast() over independent Kerr-Schild zones
+    (ref_c2p_batch): U[8][n], rth[2][n], prim[8][n] guess in / result out.
+    Returns the codes.  Uses the Kerr-Schild build (c1)."""
+    if cfg not in _C2P_LIB:
+        L = ctypes.CDLL(ref_lib_path(cfg), mode=os.RTLD_LOCAL | os.RTLD_NOW)
+        L.ref_c2p_batch.argtypes = [ctypes.c_long, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _C2P_LIB[cfg] = L
+    n = U.shape[1]
+    for arr in (U, rth, prim):
+        assert arr.dtype == np.float64 and arr.flags["C_CONTIGUOUS"]
+    code = np.zeros(n, np.int32)
+    _C2P_LIB[cfg].ref_c2p_batch(n, a, gam, _p(U), _p(rth), _p(prim), _p(code))
+    return code
+
+
 class RefRun:
     """One reference process state (load the .so under a private name so two
     configurations can coexist in one interpreter)."""

@@ -1,0 +1,314 @@
+"""Batched con-to-prim and the learned initial guess.
+
+CPU tests pin the oracle (Philox known-answer vectors, the network's gradient
+by finite differences, the reference's own Utoprim_2d_fast() through
+oracle/_ref) and check the host build of the kernel bodies against it; GPU
+tests check the HIP library against the host build and the oracle, and show
+what the trained network gains over the plain guess.
+
+Per-zone algorithm: Utoprim_2d_fast(), src/utoprim_2d_fast.c:135-408, as
+called by recover_primitives(), src/step_ch.c:832.  The sampler and the
+network have no counterpart in the reference.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import c2p_oracle as co
+import pyoracle
+import refwrap
+
+A_SPIN, GAM = 0.9, 5.0 / 3.0
+
+
+@pytest.fixture(scope="module")
+def host():
+    from coolharm3d_amd.c2p import Con2PrimBatch
+    return Con2PrimBatch(hostcheck=True)
+
+
+@pytest.fixture(scope="module")
+def host_samples(host):
+    return host.samples(3000, 2024, A_SPIN, GAM, guess=True)
+
+
+# ---------------------------------------------------------------------------
+# oracle pins
+
+def test_philox_known_answers():
+    """Random123's published philox4x32-10 vectors (kat_vectors)."""
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for ctr, key, expect in kat:
+        got = co.philox4x32_10(np.array([ctr], np.uint64), np.array(key, np.uint64))[0]
+        assert [int(x) for x in got] == list(expect)
+
+
+def test_ks_record_closed_forms():
+    """Kerr-Schild record: g_ab g^bc = delta, sqrt(-det g) = g, Schwarzschild limit."""
+    for a in (0.0, 0.5, 0.9, -0.7):
+        for r, th in ((1.3, 0.05), (2.0, 1.0), (10.0, 2.5), (100.0, np.pi / 2)):
+            g = co.ks_geom(a, r, th)
+            gcov, gcon = co.gcov_of(g), co.gcon_of(g)
+            assert np.allclose(gcov @ gcon, np.eye(4), atol=1e-12 * max(1.0, r * r))
+            assert np.isclose(np.sqrt(-np.linalg.det(gcov)), g[32], rtol=1e-12)
+            assert np.isclose(g[34], 1.0 / np.sqrt(-gcon[0, 0]), rtol=1e-15)
+    g = co.gcov_of(co.ks_geom(0.0, 5.0, 0.7))
+    expect = np.zeros((4, 4))
+    expect[0, 0], expect[0, 1], expect[1, 0], expect[1, 1] = -(1 - 2 / 5.0), 2 / 5.0, 2 / 5.0, 1 + 2 / 5.0
+    expect[2, 2], expect[3, 3] = 25.0, 25.0 * np.sin(0.7) ** 2
+    assert np.allclose(g, expect, rtol=1e-14, atol=1e-14)
+
+
+def test_mlp_gradient_finite_differences():
+    """The numpy backward pass against central differences of its own loss."""
+    rng = np.random.default_rng(5)
+    w = rng.normal(0, 0.3, co.N_PARAM)
+    w[co.OFF_ISD:co.OFF_ISD + co.N_IN] = rng.uniform(0.5, 2, co.N_IN)
+    w[co.OFF_YSD:co.OFF_YSD + co.N_OUT] = rng.uniform(0.5, 2, co.N_OUT)
+    feat = rng.normal(0, 1.5, (co.N_IN, 40))
+    target = rng.normal(0, 1, (co.N_OUT, 40))
+    loss, g = co.mlp_loss_grad(w, feat, target)
+    assert np.isclose(loss, co.mlp_loss(w, feat, target), rtol=1e-14)
+    idx = np.concatenate([rng.choice(co.N_TRAIN, 60, replace=False), [0, co.N_TRAIN - 1, co.N_TRAIN - 2]])
+    h = 1e-6
+    for i in idx:
+        wp, wm = w.copy(), w.copy()
+        wp[co.TRAIN_OFF + i] += h
+        wm[co.TRAIN_OFF + i] -= h
+        fd = (co.mlp_loss(wp, feat, target) - co.mlp_loss(wm, feat, target)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-7 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+@pytest.mark.skipif(not refwrap.have_ref("c1"), reason="reference build oracle/_ref/c1 not built")
+def test_oracle_utoprim_matches_reference_build(host_samples):
+    """Oracle Utoprim_2d_fast() == the reference's own, bit for bit, on the workload."""
+    d = host_samples
+    U, rth, guess = (np.ascontiguousarray(d[k]) for k in ("U", "rth", "guess"))
+    pr = guess.copy()
+    code_ref = refwrap.c2p_batch(A_SPIN, GAM, U, rth, pr)
+    P, code, _ = co.utoprim_zones(A_SPIN, GAM, U, rth, guess)
+    assert np.array_equal(code, code_ref)
+    assert np.array_equal(P, pr)
+
+
+# ---------------------------------------------------------------------------
+# host build of the kernel bodies vs the oracle
+
+def test_sampler_matches_oracle(host, host_samples):
+    d = host_samples
+    m = 400
+    U, rth, P, G = co.samples(2024, np.arange(m), A_SPIN, GAM)
+    assert np.allclose(d["rth"][:, :m], rth, rtol=1e-15, atol=0)
+    assert np.allclose(d["prim"][:, :m], P, rtol=1e-10, atol=0)
+    assert np.allclose(d["guess"][:, :m], G, rtol=1e-10, atol=0)
+    scale = np.abs(U).max(axis=0)
+    assert np.all(np.abs(d["U"][:, :m] - U) <= 1e-10 * scale)
+
+
+def test_sample_distribution(host_samples):
+    """Ranges and marginals of the workload; b^2/rho from the oracle's own state."""
+    d = host_samples
+    r, th = d["rth"]
+    assert r.min() >= 1.3 and r.max() <= 100.0
+    assert th.min() >= 0.05 and th.max() <= np.pi - 0.05
+    P = d["prim"]
+    assert np.all(P[0] == 1.0)
+    assert P[1].min() >= 1e-4 and P[1].max() <= 10.0
+    m = 600
+    sig, lor = np.empty(m), np.empty(m)
+    for z in range(m):
+        bsq, gamma, _ = co.bsq_of(co.ks_geom(A_SPIN, r[z], th[z]), P[:, z], GAM)
+        sig[z], lor[z] = bsq, gamma
+    assert sig.min() >= 1e-4 * (1 - 1e-9) and sig.max() <= 1e2 * (1 + 1e-9)
+    assert lor.min() >= 1.0 and lor.max() <= 50.0 * (1 + 1e-9)
+
+    def ks_uniform(x):
+        x = np.sort(x)
+        return np.max(np.abs(x - (np.arange(x.size) + 0.5) / x.size))
+    bound = 1.63 / np.sqrt(m) + 0.5 / m        # Kolmogorov 1 % critical value
+    assert ks_uniform(np.log(lor) / np.log(50.0)) < bound
+    assert ks_uniform((np.log10(sig) + 4) / 6) < bound
+    assert ks_uniform((np.log(r) - np.log(1.3)) / (np.log(100.0) - np.log(1.3))) < bound
+
+
+def test_host_solve_matches_oracle(host, host_samples):
+    """The host build's batched solve == the oracle per zone, bit for bit."""
+    d = host_samples
+    m = 1500
+    U, rth, guess = (np.ascontiguousarray(d[k][:, :m]) for k in ("U", "rth", "guess"))
+    P, code, iters = co.utoprim_zones(A_SPIN, GAM, U, rth, guess)
+    p, c, it = host.solve(U, rth, A_SPIN, GAM, plain_guess="prim", prim_in=guess, compact=True)
+    assert np.array_equal(c, code)
+    assert np.array_equal(it, iters)
+    assert np.array_equal(p, P)
+
+
+@pytest.mark.parametrize("budgets", [(1, 1), (3, 2), (8, 4), (0, 0)])
+@pytest.mark.parametrize("plain", ["cold", "prim"])
+def test_host_compaction_is_exact(host, host_samples, budgets, plain):
+    """Parking zones in the compaction queue keeps every decision: identical results."""
+    d = host_samples
+    kw = dict(plain_guess=plain, prim_in=d["guess"] if plain == "prim" else None)
+    p0, c0, i0 = host.solve(d["U"], d["rth"], A_SPIN, GAM, compact=False, **kw)
+    p1, c1, i1 = host.solve(d["U"], d["rth"], A_SPIN, GAM, compact=True, budget_first=budgets[0],
+                            budget_next=budgets[1], **kw)
+    assert np.array_equal(p0, p1) and np.array_equal(c0, c1) and np.array_equal(i0, i1)
+
+
+def test_host_round_trip(host, host_samples):
+    """prim -> U -> prim recovers the sampled state (cold guess)."""
+    d = host_samples
+    p, c, it = host.solve(d["U"], d["rth"], A_SPIN, GAM, plain_guess="cold")
+    ok = c == 0
+    assert ok.mean() > 0.999
+    P = d["prim"]
+    for v in (0, 2, 3, 4):
+        assert np.all(np.abs(p[v, ok] - P[v, ok]) <= 1e-8 * np.maximum(np.abs(P[v, ok]), 1.0))
+    assert np.all(np.abs(p[1, ok] - P[1, ok]) <= 1e-9 * (P[0, ok] + P[1, ok]))
+    # B = U[B]/g exactly; U[B] = b^i u^t - b^t u^i carries cancellation ~ gamma
+    assert np.all(np.abs(p[5:] - P[5:]) <= 1e-9 * np.abs(P[5:]).max(axis=0))
+
+
+def test_host_mlp_matches_numpy(host, host_samples):
+    d = host_samples
+    feat, target = host.features(d["U"], d["rth"], A_SPIN, GAM, prim=d["prim"])
+    w = host.init(3, feat, target)
+    y = host.forward(w, feat)
+    assert np.allclose(y, co.mlp_forward(w, feat), rtol=2e-5, atol=2e-5)
+    m = 700
+    f, t = np.ascontiguousarray(feat[:, :m]), np.ascontiguousarray(target[:, :m])
+    g, loss = host.grad(w, f, t)
+    lo, gref = co.mlp_loss_grad(w, f, t)
+    assert abs(loss[0] - lo) <= 1e-5 * abs(lo)
+    assert np.max(np.abs(g - gref)) <= 2e-5 * np.max(np.abs(gref))
+    # targets are log(h gamma), log(gamma) of the true state
+    assert np.all(np.isfinite(target)) and np.all(target[1] >= -1e-6) and np.all(target[0] >= target[1] - 1e-6)
+
+
+def test_host_training_and_fallback(host):
+    """A short training lowers the loss; solutions from the learned guess equal
+    the plain guess's to the Newton tolerance, never fail where it succeeds."""
+    w, hist = host.train_guess(3000, 91, A_SPIN, GAM, steps=120, batch=256, lr=3e-3)
+    assert hist[-10:].mean() < 0.5 * hist[:5].mean()
+    d = host.samples(3000, 17, A_SPIN, GAM)
+    p0, c0, _ = host.solve(d["U"], d["rth"], A_SPIN, GAM, plain_guess="cold")
+    p1, c1, _ = host.solve(d["U"], d["rth"], A_SPIN, GAM, plain_guess="cold", mlp=w)
+    assert np.all(c1[c0 == 0] == 0)
+    ok = (c0 == 0) & (c1 == 0)
+    assert np.all(np.abs(p1[0, ok] - p0[0, ok]) <= 1e-8 * p0[0, ok])
+
+
+def test_entry_point_errors(host):
+    d = host.samples(10, 1, A_SPIN, GAM)
+    rc = host.lib.hgpu_c2p_solve(None, 10, A_SPIN, GAM, host.ptr(d["U"]), host.ptr(d["rth"]), None, 2, 0, 0, 0,
+                                 None, 0, host.ptr(d["prim"]), host.ptr(np.zeros(10, np.int32)),
+                                 host.ptr(np.zeros(10, np.int32)))
+    assert rc < 0 and b"plain_guess" in host.lib.hgpu_last_error()
+    rc = host.lib.hgpu_c2p_solve(None, 10, A_SPIN, GAM, host.ptr(d["U"]), host.ptr(d["rth"]), None, 1, 1, 0, 0,
+                                 None, 0, host.ptr(d["prim"]), host.ptr(np.zeros(10, np.int32)),
+                                 host.ptr(np.zeros(10, np.int32)))
+    assert rc < 0 and b"workspace" in host.lib.hgpu_last_error()
+    rc = host.lib.hgpu_c2p_samples(None, 10, 1, 1.0, GAM, host.ptr(d["U"]), host.ptr(d["rth"]), host.ptr(d["prim"]),
+                                   None)
+    assert rc < 0 and b"spin" in host.lib.hgpu_last_error()
+
+
+# ---------------------------------------------------------------------------
+# GPU
+
+@pytest.fixture(scope="module")
+def gpu():
+    from coolharm3d_amd.c2p import Con2PrimBatch
+    return Con2PrimBatch()
+
+
+@pytest.mark.gpu
+def test_gpu_samples_match_host(gpu, host):
+    n = 1 << 15
+    g = gpu.samples(n, 2024, A_SPIN, GAM, guess=True)
+    h = host.samples(n, 2024, A_SPIN, GAM, guess=True)
+    for k in ("rth", "prim", "guess"):
+        assert np.allclose(gpu.to_numpy(g[k]), h[k], rtol=1e-12, atol=0), k
+    U = gpu.to_numpy(g["U"])
+    assert np.all(np.abs(U - h["U"]) <= 1e-11 * np.abs(h["U"]).max(axis=0))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("plain", ["cold", "prim"])
+def test_gpu_solve_matches_host_and_oracle(gpu, host, plain):
+    """GPU batched solve vs the host build (same inputs) and vs the oracle."""
+    n = 1 << 15
+    g = gpu.samples(n, 99, A_SPIN, GAM, guess=True)
+    U, rth, guess = (gpu.to_numpy(g[k]) for k in ("U", "rth", "guess"))
+    kw = dict(plain_guess=plain)
+    p, c, it = gpu.solve(g["U"], g["rth"], A_SPIN, GAM, prim_in=g["guess"] if plain == "prim" else None,
+                         compact=True, **kw)
+    p, c, it = gpu.to_numpy(p), gpu.to_numpy(c), gpu.to_numpy(it)
+    ph, ch, ih = host.solve(U, rth, A_SPIN, GAM, prim_in=guess if plain == "prim" else None, compact=False, **kw)
+    # the Kerr-Schild record uses the device's sin/cos; a last-ulp difference may
+    # move a zone across the Newton tolerance
+    assert np.mean(c == ch) >= 0.9999
+    assert np.mean(it == ih) >= 0.999
+    both = (c == 0) & (ch == 0)
+    for v in range(5):
+        assert np.all(np.abs(p[v, both] - ph[v, both]) <= 1e-9 * (np.abs(ph[v, both]) + ph[0, both]))
+    m = 300
+    gin = guess[:, :m] if plain == "prim" else None
+    if plain == "cold":
+        gin = np.zeros((8, m))
+        for z in range(m):
+            gm = co.ks_geom(A_SPIN, rth[0, z], rth[1, z])
+            D = U[0, z] * gm[34] / gm[32]
+            gin[:2, z] = (D, 1e-3 * D)
+    P, code, _ = co.utoprim_zones(A_SPIN, GAM, U[:, :m], rth[:, :m], gin)
+    assert np.mean(code == c[:m]) >= 0.99
+    ok = (code == 0) & (c[:m] == 0)
+    assert np.all(np.abs(P[0, ok] - p[0, :m][ok]) <= 1e-9 * P[0, ok])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("budgets", [(1, 1), (3, 2), (8, 4)])
+def test_gpu_compaction_is_exact(gpu, budgets):
+    n = 1 << 18
+    g = gpu.samples(n, 5, A_SPIN, GAM)
+    p0, c0, i0 = (gpu.to_numpy(x) for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, compact=False))
+    p1, c1, i1 = (gpu.to_numpy(x) for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, compact=True,
+                                                     budget_first=budgets[0], budget_next=budgets[1]))
+    assert np.array_equal(c0, c1) and np.array_equal(i0, i1) and np.array_equal(p0, p1)
+
+
+@pytest.mark.gpu
+def test_gpu_mlp_gradient_matches_cpu(gpu):
+    """Project forward/backward kernels vs the float64 numpy implementation."""
+    g = gpu.samples(5000, 12, A_SPIN, GAM)
+    feat, target = gpu.features(g["U"], g["rth"], A_SPIN, GAM, prim=g["prim"])
+    w = gpu.init(4, feat, target)
+    f, t, wn = gpu.to_numpy(feat), gpu.to_numpy(target), gpu.to_numpy(w)
+    y = gpu.to_numpy(gpu.forward(w, feat))
+    assert np.allclose(y, co.mlp_forward(wn, f), rtol=2e-5, atol=2e-5)
+    gr, loss = gpu.grad(w, feat, target)
+    lo, gref = co.mlp_loss_grad(wn, f, t)
+    assert abs(gpu.to_numpy(loss)[0] - lo) <= 1e-5 * abs(lo)
+    assert np.max(np.abs(gpu.to_numpy(gr) - gref)) <= 2e-5 * np.max(np.abs(gref))
+
+
+@pytest.mark.gpu
+def test_gpu_learned_guess_gain(gpu):
+    """Trained on zones it solved itself, the network's guess lowers the mean
+    Newton iterations of the cold-guess solve without adding failures, and
+    solutions agree to the Newton tolerance."""
+    w, hist = gpu.train_guess(1 << 17, 777, A_SPIN, GAM, steps=1500, batch=4096, lr=3e-3)
+    h = gpu.to_numpy(hist)
+    assert h[-50:].mean() < 0.05 * h[:5].mean()
+    g = gpu.samples(1 << 18, 31337, A_SPIN, GAM)
+    p0, c0, i0 = (gpu.to_numpy(x) for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, plain_guess="cold"))
+    p1, c1, i1 = (gpu.to_numpy(x) for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, plain_guess="cold", mlp=w))
+    assert i1.mean() < 0.8 * i0.mean(), (i1.mean(), i0.mean())
+    assert (c1 != 0).mean() <= (c0 != 0).mean()
+    ok = (c0 == 0) & (c1 == 0)
+    for v in range(5):
+        assert np.all(np.abs(p1[v, ok] - p0[v, ok]) <= 1e-8 * (np.abs(p0[v, ok]) + p0[0, ok]))
