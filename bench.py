@@ -22,7 +22,18 @@ Also reported:
                        FP64 step on the 256x256x192 torus after 1, 10, 100 steps;
   cpu_baseline      -- the reference C (oracle/_ref/c1, built from the reference
                        sources) on the host cores: independent processes on x3
-                       slabs of the same grid, timed around step_ch() only.
+                       slabs of the same grid, timed around step_ch() only;
+  con2prim          -- batched Utoprim_2d_fast() over 2^24 independent Kerr-Schild
+                       zones (r log-uniform in [1.3, 100], theta uniform in
+                       [0.05, pi - 0.05], a = 0.9, Lorentz factor log-uniform in
+                       [1, 50], b^2/rho in [1e-4, 1e2], u/rho in [1e-4, 10],
+                       isotropic directions, Gamma = 5/3, FP64): solves/s, mean
+                       Newton iterations and failure rate from the cold guess and
+                       from the learned guess (8-64-64-2 GELU network, FP32,
+                       trained on the GPU on zones the library solved itself,
+                       training time reported apart), one-pass and compacted
+                       solves, with the reference's own Utoprim_2d_fast() timed
+                       on the host cores beside it.
 """
 from __future__ import annotations
 
@@ -133,6 +144,110 @@ def cpu_baseline(args):
                        f"an upper bound on an MPI run)")}
 
 
+C2P_BYTES = 152.0   # per solve: U (64 B) + (r, theta) (16 B) read, prims (64 B) + code, iterations (8 B) written
+
+
+def c2p_cpu_worker(args):
+    """One CPU process: the reference's Utoprim_2d_fast() (oracle/_ref/c1) on its
+    share of the workload's zones from the cold guess; prints solves and seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import refwrap
+    from coolharm3d_amd.c2p import Con2PrimBatch
+    H = Con2PrimBatch(hostcheck=True)          # host build: inputs only
+    n = args.c2p_cpu_zones
+    d = H.samples(n, 1 + args.worker, 0.9, 5.0 / 3.0)
+    U, rth = d["U"].numpy(), d["rth"].numpy()
+    prim = np.zeros((8, n))
+    a = 0.9
+    # cold guess rho = D, u = 1e-3 D: D = alpha U[RHO] / sqrt(-g), Kerr-Schild lapse and determinant
+    r, th = rth
+    c, s = np.cos(th), np.sin(th)
+    sig = r * r + a * a * c * c
+    alpha = 1.0 / np.sqrt(1.0 + 2.0 * r / sig)
+    gdet = r * r * np.abs(s) * (1.0 + 0.5 * (a * a / (r * r)) * (1.0 + (c - s) * (c + s)))
+    D = U[0] * alpha / gdet
+    prim[0], prim[1] = D, 1e-3 * D
+    prim = np.ascontiguousarray(prim)
+    t0 = time.perf_counter()
+    code = refwrap.c2p_batch(0.9, 5.0 / 3.0, U, rth, prim)
+    el = time.perf_counter() - t0
+    print(json.dumps({"n": n, "s": el, "fail": int((code != 0).sum())}), flush=True)
+
+
+def c2p_cpu_baseline(args):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from refwrap import have_ref
+    if not have_ref("c1"):
+        return {"value": None, "unit": "solves/s", "cores": 0, "kind": "reference",
+                "sample": "oracle/_ref/c1 is not built on this machine"}
+    ncores = min(16, os.cpu_count() or 1)
+    procs = []
+    for w in range(ncores):
+        cmd = [sys.executable, os.path.abspath(__file__), "--_c2p-cpu-worker", "--worker", str(w),
+               "--c2p-cpu-zones", str(args.c2p_cpu_zones)]
+        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                                      env=dict(os.environ, OMP_NUM_THREADS="1")))
+    rate, n_ok = 0.0, 0
+    for p in procs:
+        out, _ = p.communicate()
+        for line in out.splitlines():
+            if line.startswith("{"):
+                r = json.loads(line)
+                rate += r["n"] / r["s"]
+                n_ok += 1
+    return {"value": rate if n_ok else None, "unit": "solves/s", "cores": n_ok, "kind": "reference",
+            "sample": (f"reference Utoprim_2d_fast() (gcc -O2, oracle/_ref/c1) from the cold guess, {n_ok} "
+                       f"independent processes, each {args.c2p_cpu_zones} zones of the same distribution "
+                       f"(different seeds); rates summed")}
+
+
+def con2prim_report(args):
+    """Batched con-to-prim of 2^k independent zones: cold vs learned guess,
+    one-pass vs compacted solves."""
+    import numpy as np
+    from coolharm3d_amd.c2p import Con2PrimBatch, stats
+    a, gam = 0.9, 5.0 / 3.0
+    B = Con2PrimBatch()
+    n = 1 << args.c2p_log2
+    t0 = time.perf_counter()
+    w, hist = B.train_guess(1 << 18, 777, a, gam, steps=args.c2p_train_steps, batch=4096, lr=3e-3)
+    B.sync()
+    train_s = time.perf_counter() - t0
+    d = B.samples(n, 1, a, gam)
+    work = B.workspace(n, True)
+    out = (B.empty((8, n), np.float64), B.empty((n,), np.int32), B.empty((n,), np.int32))
+    T = B.timer()
+    runs = {}
+    for name, kw in (("cold_onepass", dict(compact=False)), ("cold_compact", dict(compact=True)),
+                     ("learned_onepass", dict(compact=False, mlp=w)), ("learned_compact", dict(compact=True, mlp=w))):
+        ms = []
+        for rep in range(1 + args.c2p_reps):
+            T.start()
+            B.solve(d["U"], d["rth"], a, gam, plain_guess="cold", work=work, out=out, **kw)
+            el = T.stop()
+            if rep > 0:
+                ms.append(el)
+        st = stats(out[1], out[2])
+        t = float(np.median(ms))
+        runs[name] = {"ms": t, "solves_per_s": n / t * 1e3, "mean_iters": st["mean_iters"],
+                      "max_iters": st["max_iters"], "failure_rate": st["failure_rate"],
+                      "achieved_GBs": C2P_BYTES * n / (t * 1e-3) / 1e9}
+    best = max(runs, key=lambda k: runs[k]["solves_per_s"])
+    rep = {"zones": n, "value": runs[best]["solves_per_s"], "unit": "solves/s", "best": best, "runs": runs,
+           "plain_guess": "cold (rho = D, u = 1e-3 D, u~ = 0)",
+           "learned_guess_gain": {
+               "mean_iters": runs["cold_compact"]["mean_iters"] / runs["learned_compact"]["mean_iters"],
+               "solves_per_s": runs["learned_compact"]["solves_per_s"] / runs["cold_compact"]["solves_per_s"]},
+           "training": {"zones": 1 << 18, "steps": args.c2p_train_steps, "batch": 4096, "seconds": train_s,
+                        "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-10:].mean())},
+           "roofline": {"bound": "hbm", "algo_bytes_per_solve": C2P_BYTES, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "achieved": runs[best]["achieved_GBs"], "frac": runs[best]["achieved_GBs"] / HBM_PEAK_GBS}}
+    if args.cpu_baseline == "auto":
+        rep["cpu_baseline"] = c2p_cpu_baseline(args)
+    return rep
+
+
 def timed_run(args, g, p0, precision):
     """Warm-up + timed step_ch() of one grid; returns timings and per-kernel figures."""
     from coolharm3d_amd import Harm3D
@@ -209,6 +324,13 @@ def main():
     ap.add_argument("--slab", type=int, default=4)
     ap.add_argument("--slab-steps", type=int, default=4)
     ap.add_argument("--cpu-kind", choices=["reference", "port"], default="reference")
+    ap.add_argument("--c2p", choices=["auto", "off"], default="auto",
+                    help="batched con-to-prim report (2^c2p-log2 zones)")
+    ap.add_argument("--c2p-log2", type=int, default=24)
+    ap.add_argument("--c2p-reps", type=int, default=3)
+    ap.add_argument("--c2p-train-steps", type=int, default=1500)
+    ap.add_argument("--c2p-cpu-zones", type=int, default=1 << 19)
+    ap.add_argument("--_c2p-cpu-worker", action="store_true", dest="c2p_cpu_worker")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one plain device copy of this many MiB before the run "
                          "(calibrates the HBM byte counters of a profiler run)")
@@ -216,6 +338,9 @@ def main():
 
     if args.cpu_worker:
         cpu_worker(args)
+        return
+    if args.c2p_cpu_worker:
+        c2p_cpu_worker(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -227,13 +352,14 @@ def main():
 
     n1, n2, n3 = args.n1, args.n2, args.n3
     if args.copy_probe_mib > 0:
-        import torch
-        nel = args.copy_probe_mib * (1 << 20) // 8
-        a = torch.ones(nel, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
+        import numpy as np
+        from coolharm3d_amd.c2p import Con2PrimBatch
+        B = Con2PrimBatch()
+        nb = args.copy_probe_mib * (1 << 20)
+        src, dst = B.empty((nb,), np.uint8), B.empty((nb,), np.uint8)
+        B._chk(B.lib.hgpu_dev_copy_kernel(dst.ptr, src.ptr, nb, None), "hgpu_dev_copy_kernel")
+        B.sync()
+        del src, dst
     g = torus_grid(n1, n2, n3)
     p0 = torus(g)
     el, kern, finite = timed_run(args, g, p0, args.precision)
@@ -274,6 +400,8 @@ def main():
     if rank == 0 and args.cpu_baseline == "auto":
         cpu = cpu_baseline(args)
 
+    c2p = con2prim_report(args) if args.c2p == "auto" else None
+
     line = {
         "metric": METRIC, "value": value, "unit": "zone-cycles/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
@@ -287,6 +415,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
         "precision_variant": prec_rep,
+        "con2prim": c2p,
     }
     if rank == 0:
         print(json.dumps(line), flush=True)

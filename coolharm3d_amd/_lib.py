@@ -52,6 +52,10 @@ SIGNATURES = {
     "hgpu_fixup_bounds": (ctypes.c_int, [ctypes.c_void_p]),
     "hgpu_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "hgpu_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "hgpu_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "hgpu_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_double_p]),
+    "hgpu_set_launch_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "hgpu_get_launch_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p, c_int_p, c_int_p]),
     "hgpu_download_pflag": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
     "hgpu_download_gamma": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     "hgpu_download_nfail": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
@@ -65,6 +69,17 @@ SIGNATURES = {
     "hgpu_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c_double_p, c_ll_p]),
     "hgpu_con2prim_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    # device plumbing
+    "hgpu_dev_count": (ctypes.c_int, [c_int_p]),
+    "hgpu_dev_alloc": (ctypes.c_int, [_ll, ctypes.POINTER(ctypes.c_void_p)]),
+    "hgpu_dev_free": (ctypes.c_int, [_vp]),
+    "hgpu_dev_copy": (ctypes.c_int, [_vp, _vp, _ll, ctypes.c_int, _vp]),
+    "hgpu_dev_copy_kernel": (ctypes.c_int, [_vp, _vp, _ll, _vp]),
+    "hgpu_stream_sync": (ctypes.c_int, [_vp]),
+    "hgpu_event_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "hgpu_event_destroy": (ctypes.c_int, [_vp]),
+    "hgpu_event_record": (ctypes.c_int, [_vp, _vp]),
+    "hgpu_event_elapsed": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
     # batched con-to-prim and its network (pointers are addresses: ints)
     "hgpu_c2p_samples": (ctypes.c_int, [_vp, _ll, ctypes.c_ulonglong, _d, _d, _vp, _vp, _vp, _vp]),
     "hgpu_c2p_workspace_bytes": (_ll, [_ll, ctypes.c_int]),

@@ -59,6 +59,13 @@ __global__ __launch_bounds__(CB) void k_c2p_samples(long long n, unsigned long l
     for (int l = 0; l < NP; l++) guess[l*n + t] = g[l];
 }
 
+// Plain streaming copy (16 B per lane per iteration): the reference kernel of
+// known traffic that calibrates the profiler's HBM byte counters.
+__global__ __launch_bounds__(CB) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, long long n16) {
+  const long long stride = (long long)gridDim.x * CB;
+  for (long long t = (long long)blockIdx.x * CB + threadIdx.x; t < n16; t += stride) dst[t] = src[t];
+}
+
 __global__ void k_fill_u32(unsigned* p, int m) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < m) p[t] = 0u;
@@ -320,6 +327,59 @@ int grad_launch(hipStream_t stream, long long n, const float* w, const float* fe
 using namespace hgpu;
 
 extern "C" {
+
+int hgpu_dev_count(int* out) {
+  if (!out) return set_error(-1, "null argument");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *out = n;
+  return 0;
+}
+
+int hgpu_dev_alloc(long long bytes, void** out) {
+  if (!out || bytes < 0) return set_error(-1, "bad argument");
+  *out = nullptr;
+  return hipc(hipMalloc(out, (size_t)(bytes > 0 ? bytes : 8)), "hipMalloc");
+}
+
+int hgpu_dev_free(void* p) { return p ? hipc(hipFree(p), "hipFree") : 0; }
+
+int hgpu_dev_copy(void* dst, const void* src, long long bytes, int kind, void* stream) {
+  if (bytes < 0 || (bytes > 0 && (!dst || !src))) return set_error(-1, "bad argument");
+  if (bytes == 0) return 0;
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                        : kind == 2 ? hipMemcpyDeviceToDevice : hipMemcpyHostToHost;
+  if (kind < 0 || kind > 2) return set_error(-2, "kind must be 0 (to device), 1 (to host) or 2 (device to device)");
+  if (int rc = hipc(hipMemcpyAsync(dst, src, (size_t)bytes, k, st(stream)), "hipMemcpyAsync")) return rc;
+  return kind == 1 ? hipc(hipStreamSynchronize(st(stream)), "hipStreamSynchronize") : 0;
+}
+
+int hgpu_dev_copy_kernel(void* dst, const void* src, long long bytes, void* stream) {
+  if (bytes < 0 || bytes % 16 || (bytes > 0 && (!dst || !src))) return set_error(-1, "bytes must be a multiple of 16");
+  if (reinterpret_cast<uintptr_t>(dst) % 16 || reinterpret_cast<uintptr_t>(src) % 16)
+    return set_error(-2, "buffers must be 16-byte aligned");
+  if (bytes == 0) return 0;
+  const long long n16 = bytes / 16;
+  hipLaunchKernelGGL(k_copy16, dim3(persistent_blocks(n16)), dim3(CB), 0, st(stream), static_cast<uint4*>(dst),
+                     static_cast<const uint4*>(src), n16);
+  return hipc(hipGetLastError(), "k_copy16");
+}
+
+int hgpu_stream_sync(void* stream) { return hipc(hipStreamSynchronize(st(stream)), "hipStreamSynchronize"); }
+
+int hgpu_event_create(void** ev) {
+  if (!ev) return set_error(-1, "null argument");
+  return hipc(hipEventCreate(reinterpret_cast<hipEvent_t*>(ev)), "hipEventCreate");
+}
+int hgpu_event_destroy(void* ev) { return ev ? hipc(hipEventDestroy(static_cast<hipEvent_t>(ev)), "hipEventDestroy") : 0; }
+int hgpu_event_record(void* ev, void* stream) {
+  return hipc(hipEventRecord(static_cast<hipEvent_t>(ev), st(stream)), "hipEventRecord");
+}
+int hgpu_event_elapsed(void* ev0, void* ev1, float* ms) {
+  if (!ms) return set_error(-1, "null argument");
+  if (int rc = hipc(hipEventSynchronize(static_cast<hipEvent_t>(ev1)), "hipEventSynchronize")) return rc;
+  return hipc(hipEventElapsedTime(ms, static_cast<hipEvent_t>(ev0), static_cast<hipEvent_t>(ev1)), "hipEventElapsedTime");
+}
 
 int hgpu_c2p_samples(void* stream, long long n, unsigned long long seed, double a, double gam, double* U,
                      double* rth, double* prim, double* guess) {

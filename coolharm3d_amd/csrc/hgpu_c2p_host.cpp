@@ -4,6 +4,9 @@
 // check library, which the CPU tests use to exercise the kernels' arithmetic
 // and the compaction queue without a GPU.
 #include "hgpu_c2p_api.h"
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 using namespace hgpu;
@@ -36,6 +39,47 @@ void grad_host(long long n, const float* w, const float* feat, const float* targ
 }  // namespace
 
 extern "C" {
+
+// device plumbing of the host build: "device" memory is host memory
+int hgpu_dev_count(int* out) {
+  if (!out) return set_error(-1, "null argument");
+  *out = 0;
+  return 0;
+}
+int hgpu_dev_alloc(long long bytes, void** out) {
+  if (!out || bytes < 0) return set_error(-1, "bad argument");
+  *out = std::malloc((size_t)(bytes > 0 ? bytes : 8));
+  return *out ? 0 : set_error(-4, "out of host memory");
+}
+int hgpu_dev_free(void* p) { std::free(p); return 0; }
+int hgpu_dev_copy(void* dst, const void* src, long long bytes, int kind, void*) {
+  if (bytes < 0 || (bytes > 0 && (!dst || !src))) return set_error(-1, "bad argument");
+  if (kind < 0 || kind > 2) return set_error(-2, "kind must be 0 (to device), 1 (to host) or 2 (device to device)");
+  if (bytes) std::memmove(dst, src, (size_t)bytes);
+  return 0;
+}
+int hgpu_dev_copy_kernel(void* dst, const void* src, long long bytes, void*) {
+  if (bytes < 0 || bytes % 16 || (bytes > 0 && (!dst || !src))) return set_error(-1, "bytes must be a multiple of 16");
+  if (bytes) std::memmove(dst, src, (size_t)bytes);
+  return 0;
+}
+int hgpu_stream_sync(void*) { return 0; }
+int hgpu_event_create(void** ev) {
+  if (!ev) return set_error(-1, "null argument");
+  *ev = new double(0.);
+  return 0;
+}
+int hgpu_event_destroy(void* ev) { delete static_cast<double*>(ev); return 0; }
+int hgpu_event_record(void* ev, void*) {
+  *static_cast<double*>(ev) = std::chrono::duration<double, std::milli>(
+      std::chrono::steady_clock::now().time_since_epoch()).count();
+  return 0;
+}
+int hgpu_event_elapsed(void* ev0, void* ev1, float* ms) {
+  if (!ms) return set_error(-1, "null argument");
+  *ms = (float)(*static_cast<double*>(ev1) - *static_cast<double*>(ev0));
+  return 0;
+}
 
 int hgpu_c2p_samples(void*, long long n, unsigned long long seed, double a, double gam, double* U, double* rth,
                      double* prim, double* guess) {

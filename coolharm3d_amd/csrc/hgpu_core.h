@@ -346,16 +346,25 @@ struct Api {
     return rc;
   }
 
+  // One step_ch(): predictor, corrector, new time step (src/step_ch.c:50-234).
+  static int one_step(hgpu_ctx* c) {
+    int rc = 0;
+    rc |= BE::clear_failures(c);      // src/step_ch.c:65-69
+    rc |= substep(c, 0);
+    rc |= BE::clear_failures(c);      // src/step_ch.c:117-121
+    rc |= substep(c, 1);
+    rc |= BE::timestep(c);            // src/step_ch.c:218-223
+    return rc;
+  }
+
   static int step(hgpu_ctx* c, int nsteps) {
     if (!c) return set_error(-1, "null context");
+    if (nsteps < 0) return set_error(-2, "nsteps must be >= 0");
+    if (BE::graph_enabled(c)) return BE::graph_steps(c, nsteps, [c]() { return one_step(c); });
     int rc = 0;
     for (int s = 0; s < nsteps && !rc; s++) {
       rc |= BE::mark(c, "step", 0);
-      rc |= BE::clear_failures(c);      // src/step_ch.c:65-69
-      rc |= substep(c, 0);
-      rc |= BE::clear_failures(c);      // src/step_ch.c:117-121
-      rc |= substep(c, 1);
-      rc |= BE::timestep(c);            // src/step_ch.c:218-223
+      rc |= one_step(c);
       rc |= BE::mark(c, "step", 1);
     }
     return rc;

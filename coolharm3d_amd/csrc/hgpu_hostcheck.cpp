@@ -36,6 +36,26 @@ struct HostBE {
   static int zero(hgpu_ctx*, void* p, size_t bytes) { memset(p, 0, bytes); return 0; }
   static int sync(hgpu_ctx*) { return 0; }
   static int mark(hgpu_ctx*, const char*, int) { return 0; }
+  static int set_graph(hgpu_ctx*, int) { return 0; }   // the host build runs steps directly
+  static bool graph_enabled(hgpu_ctx*) { return false; }
+  static int invalidate_graph(hgpu_ctx*) { return 0; }
+  // launch shapes do not apply to the host build (zones run one by one)
+  static int set_shape(hgpu_ctx*, int fam, int, int, int) {
+    return (fam < 0 || fam >= 6) ? set_error(-2, "kernel family out of range") : 0;
+  }
+  static int get_shape(hgpu_ctx*, int fam, int* bs, int* kc, int* remap) {
+    if (fam < 0 || fam >= 6) return set_error(-2, "kernel family out of range");
+    if (bs) *bs = 256;
+    if (kc) *kc = 0;
+    if (remap) *remap = 1;
+    return 0;
+  }
+  static int autotune(hgpu_ctx*, int, double* report) {
+    if (report) for (int f = 0; f < 5; f++) { report[f*4] = 256; report[f*4 + 1] = 0; report[f*4 + 2] = 1; report[f*4 + 3] = 0; }
+    return 0;
+  }
+  template <class F>
+  static int graph_steps(hgpu_ctx*, int, F&&) { return 0; }
   static int profile_enable(hgpu_ctx*, int) { return 0; }
   static int profile_read(hgpu_ctx*, const char*, double* ms, long long* n) {
     if (ms) *ms = 0.;

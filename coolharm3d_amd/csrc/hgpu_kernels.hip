@@ -13,11 +13,22 @@ namespace hgpu {
 
 constexpr int BLOCK = 256;
 
+// Launch shape of a stencil-kernel family (tile decomposition below).
+struct LaunchShape {
+  unsigned bs = 256;     // threads per workgroup: 64, 128 or 256
+  unsigned kc = 0;       // k-extent of a 2D tile, dividing bs; 0 = flat tiles
+  unsigned remap = 1;    // XCD-contiguous tile order
+};
+enum { FAM_FLUX1 = 0, FAM_FLUX2, FAM_FLUX3, FAM_EMF, FAM_CONS, FAM_INV, N_FAM };
+
 struct ProfEvent { std::string name; hipEvent_t a, b; };
 
 struct GpuPriv {
   hipStream_t stream = nullptr;
   bool prof = false;
+  bool graph = false;                  // replay captured steps
+  hipGraphExec_t step_exec = nullptr;  // one step_ch(), captured on first use
+  LaunchShape shape[N_FAM];            // per kernel family
   std::vector<ProfEvent> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, long long>> acc;
@@ -92,39 +103,68 @@ __device__ __forceinline__ void commit_fail(const Params& P, const Ptrs& X, cons
 }
 
 // Work decomposition of the stencil kernels.  The index box is cut into
-// x1-planes of (j,k) and each plane into tiles of BLOCK consecutive (j,k)
-// (k fastest).  Hardware dispatches workgroup p to XCD p % 8, and each XCD has
-// its own L2; the logical tile of workgroup p is chosen so that every XCD gets
-// one contiguous range of tiles (whole x1-slabs), processed in order, so the
-// +-3-zone stencils of neighbouring tiles hit the same L2.
+// x1-planes of (j,k) and each plane into tiles of one workgroup: either bs
+// consecutive (j,k) points (k fastest; kc = 0), or kc k-points x bs/kc j-rows
+// (kc > 0).  Hardware dispatches workgroup p to XCD p % 8, and each XCD has its
+// own L2; with remap set, the logical tile of workgroup p is chosen so that
+// every XCD gets one contiguous range of tiles (whole x1-slabs), processed in
+// order, so the +-3-zone stencils of neighbouring tiles hit the same L2.  The
+// shape of each kernel family is picked by the autotuner (or set by the caller).
 struct Tiling {
-  unsigned X;         // tiles per plane
-  unsigned nplane;    // (j,k) points per plane
-  unsigned nk;        // k extent
-  unsigned ni;        // number of planes
-  unsigned nblocks;   // X * ni rounded up to a multiple of 8
+  unsigned ni, nj, nk;  // box extents
+  unsigned bs, kc, jr;  // tile: kc x jr (kc > 0) or bs consecutive points (kc = 0)
+  unsigned nkt;         // tiles along k (2D tiles)
+  unsigned X;           // tiles per plane
+  unsigned nblocks;     // X * ni, rounded up to a multiple of 8 when remapped
+  unsigned remap;
 };
 
-static inline Tiling make_tiling(const Box& b) {
+static inline Tiling make_tiling(const Box& b, const LaunchShape& L = LaunchShape()) {
   Tiling T;
   T.nk = (unsigned)(b.hi[2] - b.lo[2] + 1);
-  T.nplane = (unsigned)(b.hi[1] - b.lo[1] + 1) * T.nk;
+  T.nj = (unsigned)(b.hi[1] - b.lo[1] + 1);
   T.ni = (unsigned)(b.hi[0] - b.lo[0] + 1);
-  T.X = (T.nplane + BLOCK - 1) / BLOCK;
-  T.nblocks = ((T.X * T.ni + 7u) / 8u) * 8u;
+  T.bs = L.bs;
+  T.remap = L.remap;
+  if (L.kc > 0 && L.kc < T.nk && L.bs % L.kc == 0) {
+    T.kc = L.kc;
+    T.jr = L.bs / L.kc;
+    T.nkt = (T.nk + T.kc - 1) / T.kc;
+    T.X = T.nkt * ((T.nj + T.jr - 1) / T.jr);
+  } else {
+    T.kc = 0;
+    T.jr = 0;
+    T.nkt = 0;
+    T.X = (T.nj * T.nk + T.bs - 1) / T.bs;
+  }
+  T.nblocks = T.X * T.ni;
+  if (T.remap) T.nblocks = ((T.nblocks + 7u) / 8u) * 8u;
   return T;
 }
 
 __device__ __forceinline__ bool tile_index(const Box& b, const Tiling& T, int& i, int& j, int& k) {
   const unsigned p = blockIdx.x;
-  const unsigned L = (p & 7u) * (T.nblocks >> 3) + (p >> 3);
+  const unsigned L = T.remap ? (p & 7u) * (T.nblocks >> 3) + (p >> 3) : p;
   const unsigned ii = L / T.X;
-  const unsigned t = (L - ii * T.X) * BLOCK + threadIdx.x;
-  if (ii >= T.ni || t >= T.nplane) return false;
-  const unsigned jj = t / T.nk;
+  if (ii >= T.ni) return false;
+  const unsigned t = L - ii * T.X;
+  unsigned jj, kk;
+  if (T.kc == 0) {
+    const unsigned q = t * T.bs + threadIdx.x;
+    jj = q / T.nk;
+    kk = q - jj * T.nk;
+    if (jj >= T.nj) return false;
+  } else {
+    const unsigned tj = t / T.nkt;
+    const unsigned tk = t - tj * T.nkt;
+    const unsigned r = threadIdx.x / T.kc;
+    kk = tk * T.kc + (threadIdx.x - r * T.kc);
+    jj = tj * T.jr + r;
+    if (kk >= T.nk || jj >= T.nj) return false;
+  }
   i = b.lo[0] + (int)ii;
   j = b.lo[1] + (int)jj;
-  k = b.lo[2] + (int)(t - jj * T.nk);
+  k = b.lo[2] + (int)kk;
   return true;
 }
 
@@ -366,7 +406,9 @@ struct GpuBE {
   static void fini(hgpu_ctx* c) {
     GpuPriv* g = priv(c);
     if (!g) return;
-    if (g->stream) { (void)hipStreamSynchronize(g->stream); (void)hipStreamDestroy(g->stream); }
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->step_exec) (void)hipGraphExecDestroy(g->step_exec);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
     for (auto& pe : g->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
     for (auto ev : g->pool) (void)hipEventDestroy(ev);
     for (auto& kv : g->open) (void)hipEventDestroy(kv.second);
@@ -448,6 +490,160 @@ struct GpuBE {
     }
     g->pending.clear();
   }
+  // Step replay from a captured graph.  Every launch of a step is enqueued on
+  // the library stream with arguments fixed at context creation (state lives
+  // in device memory), and the step path records no memset or copy nodes, so
+  // one captured step_ch() replays any number of steps.  Kernel scopes are not
+  // recorded inside the capture; "step" times whole replays.
+  static int set_graph(hgpu_ctx* c, int on) {
+    GpuPriv* g = priv(c);
+    int rc = sync(c);
+    g->graph = on != 0;
+    return rc;
+  }
+  static bool graph_enabled(hgpu_ctx* c) { return priv(c)->graph; }
+  static int invalidate_graph(hgpu_ctx* c) {
+    GpuPriv* g = priv(c);
+    int rc = 0;
+    if (g->step_exec) {
+      rc = sync(c);
+      rc |= hip_check(hipGraphExecDestroy(g->step_exec), "hipGraphExecDestroy");
+      g->step_exec = nullptr;
+    }
+    return rc;
+  }
+  template <class F>
+  static int graph_steps(hgpu_ctx* c, int nsteps, F&& one_step) {
+    GpuPriv* g = priv(c);
+    if (!g->step_exec) {
+      const bool prof = g->prof;
+      g->prof = false;
+      int rc = hip_check(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      if (rc) { g->prof = prof; return rc; }
+      rc = one_step();
+      hipGraph_t graph = nullptr;
+      const int rc_end = hip_check(hipStreamEndCapture(g->stream, &graph), "hipStreamEndCapture");
+      g->prof = prof;
+      if (rc || rc_end) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc ? rc : rc_end;
+      }
+      rc = hip_check(hipGraphInstantiate(&g->step_exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+      (void)hipGraphDestroy(graph);
+      if (rc) { g->step_exec = nullptr; return rc; }
+    }
+    int rc = 0;
+    for (int s = 0; s < nsteps && !rc; s++) {
+      rc |= mark(c, "step", 0);
+      rc |= hip_check(hipGraphLaunch(g->step_exec, g->stream), "hipGraphLaunch");
+      rc |= mark(c, "step", 1);
+    }
+    return rc;
+  }
+
+  // Launch-shape autotuner.  For each stencil family (flux in x1, x2, x3, EMF,
+  // conserved update) every candidate shape -- workgroup size 64/128/256, flat
+  // or kc x (bs/kc) tiles with kc in {16, 32, 64}, XCD remap on/off -- is timed
+  // with HIP events over `reps` launches on the current state, and the fastest
+  // kept.  These kernels write only face fluxes, EMFs and the conserved
+  // scratch, which every substep recomputes before use; the failure words and
+  // counters they also touch are saved before and restored after.  The
+  // inversion keeps its own shape (it rewrites the state) unless set.
+  static int check_shape(const LaunchShape& L) {
+    if (L.bs != 64 && L.bs != 128 && L.bs != 256) return set_error(-2, "workgroup size must be 64, 128 or 256");
+    if (L.kc != 0 && (L.kc > L.bs || L.bs % L.kc != 0)) return set_error(-2, "tile k-extent must divide the workgroup size");
+    if (L.remap > 1) return set_error(-2, "remap must be 0 or 1");
+    return 0;
+  }
+  static int set_shape(hgpu_ctx* c, int fam, int bs, int kc, int remap) {
+    if (fam < 0 || fam >= N_FAM) return set_error(-2, "kernel family out of range");
+    LaunchShape L;
+    L.bs = (unsigned)bs; L.kc = (unsigned)kc; L.remap = (unsigned)remap;
+    if (bs < 0 || kc < 0 || remap < 0) return set_error(-2, "negative launch-shape field");
+    if (int rc = check_shape(L)) return rc;
+    int rc = invalidate_graph(c);
+    priv(c)->shape[fam] = L;
+    return rc;
+  }
+  static int get_shape(hgpu_ctx* c, int fam, int* bs, int* kc, int* remap) {
+    if (fam < 0 || fam >= N_FAM) return set_error(-2, "kernel family out of range");
+    const LaunchShape& L = priv(c)->shape[fam];
+    if (bs) *bs = (int)L.bs;
+    if (kc) *kc = (int)L.kc;
+    if (remap) *remap = (int)L.remap;
+    return 0;
+  }
+  static int conserved_only(hgpu_ctx* c) {
+    Box b = interior_box(c->P);
+    const Tiling T = make_tiling(b, priv(c)->shape[FAM_CONS]);
+    hipStream_t st = priv(c)->stream;
+    if (c->P.mixed) hipLaunchKernelGGL(k_conserved<true>, dim3(T.nblocks), dim3(T.bs), 0, st, c->P, c->X, 0, b, T);
+    else hipLaunchKernelGGL(k_conserved<false>, dim3(T.nblocks), dim3(T.bs), 0, st, c->P, c->X, 0, b, T);
+    return hip_check(hipGetLastError(), "k_conserved");
+  }
+  static int launch_family(hgpu_ctx* c, int fam) {
+    if (fam <= FAM_FLUX3) return flux(c, c->X.p, fam - FAM_FLUX1);
+    if (fam == FAM_EMF) return emf(c);
+    return conserved_only(c);
+  }
+  static int autotune(hgpu_ctx* c, int reps, double* report) {
+    GpuPriv* g = priv(c);
+    if (reps < 1) reps = 1;
+    int rc = invalidate_graph(c);
+    const long long NT = c->P.nt;
+    const size_t st_bytes = sizeof(unsigned long long) * ST_WORDS;
+    const size_t nf_bytes = c->P.track_nfail ? sizeof(int) * N_NFAIL * NT : 0;
+    const size_t pf_bytes = sizeof(int) * NT;
+    void* save = nullptr;
+    rc |= alloc(c, &save, st_bytes + nf_bytes + pf_bytes);
+    if (rc) return rc;
+    char* sv = static_cast<char*>(save);
+    hipStream_t st = g->stream;
+    rc |= hip_check(hipMemcpyAsync(sv, c->X.status, st_bytes, hipMemcpyDeviceToDevice, st), "save");
+    if (nf_bytes) rc |= hip_check(hipMemcpyAsync(sv + st_bytes, c->X.nfail, nf_bytes, hipMemcpyDeviceToDevice, st), "save");
+    rc |= hip_check(hipMemcpyAsync(sv + st_bytes + nf_bytes, c->X.pflag, pf_bytes, hipMemcpyDeviceToDevice, st), "save");
+    const bool prof = g->prof;
+    g->prof = false;
+    hipEvent_t e0 = ev(g), e1 = ev(g);
+    // fluxes first, so the EMF and update candidates read current face values
+    for (int d = 0; d < 3 && !rc; d++) rc |= flux(c, c->X.p, d);
+    const unsigned bss[3] = {64, 128, 256}, kcs[4] = {0, 16, 32, 64};
+    for (int fam = FAM_FLUX1; fam <= FAM_CONS && !rc; fam++) {
+      LaunchShape best = g->shape[fam];
+      float best_ms = 1e30f;
+      for (unsigned bs : bss) for (unsigned kc : kcs) for (unsigned remap = 0; remap <= 1; remap++) {
+        LaunchShape L;
+        L.bs = bs; L.kc = kc; L.remap = remap;
+        if (check_shape(L)) { g_last_error.clear(); continue; }
+        g->shape[fam] = L;
+        rc |= launch_family(c, fam);   // warm-up
+        rc |= hip_check(hipEventRecord(e0, st), "hipEventRecord");
+        for (int r = 0; r < reps; r++) rc |= launch_family(c, fam);
+        rc |= hip_check(hipEventRecord(e1, st), "hipEventRecord");
+        rc |= hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.f;
+        rc |= hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (rc) break;
+        ms /= (float)reps;
+        if (ms < best_ms) { best_ms = ms; best = L; }
+      }
+      g->shape[fam] = best;
+      if (report) {
+        report[fam*4 + 0] = best.bs; report[fam*4 + 1] = best.kc;
+        report[fam*4 + 2] = best.remap; report[fam*4 + 3] = best_ms;
+      }
+    }
+    g->pool.push_back(e0);
+    g->pool.push_back(e1);
+    g->prof = prof;
+    rc |= hip_check(hipMemcpyAsync(c->X.status, sv, st_bytes, hipMemcpyDeviceToDevice, st), "restore");
+    if (nf_bytes) rc |= hip_check(hipMemcpyAsync(c->X.nfail, sv + st_bytes, nf_bytes, hipMemcpyDeviceToDevice, st), "restore");
+    rc |= hip_check(hipMemcpyAsync(c->X.pflag, sv + st_bytes + nf_bytes, pf_bytes, hipMemcpyDeviceToDevice, st), "restore");
+    rc |= sync(c);
+    release(c, save);
+    return rc;
+  }
+
   static int profile_read(hgpu_ctx* c, const char* name, double* ms, long long* cnt) {
     GpuPriv* g = priv(c);
     int rc = sync(c);
@@ -472,8 +668,8 @@ struct GpuBE {
   static int flux(hgpu_ctx* c, const double* src, int d) {
     Box b = flux_box(c->P, d);
     Scope s(c, "flux");
-    const Tiling T = make_tiling(b);
-    const dim3 grid(T.nblocks), blk(BLOCK);
+    const Tiling T = make_tiling(b, priv(c)->shape[FAM_FLUX1 + d]);
+    const dim3 grid(T.nblocks), blk(T.bs);
     hipStream_t st = priv(c)->stream;
     if (c->P.mixed) {
       if (d == 0) hipLaunchKernelGGL(k_flux_mixed<0>, grid, blk, 0, st, c->P, c->X, src, b, T);
@@ -489,20 +685,21 @@ struct GpuBE {
   static int emf(hgpu_ctx* c) {
     Box b = edge_box(c->P);
     Scope s(c, "emf");
-    const Tiling T = make_tiling(b);
-    if (c->P.mixed) hipLaunchKernelGGL(k_emf_mixed, dim3(T.nblocks), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, b, T);
-    else hipLaunchKernelGGL(k_emf, dim3(T.nblocks), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, b, T);
+    const Tiling T = make_tiling(b, priv(c)->shape[FAM_EMF]);
+    if (c->P.mixed) hipLaunchKernelGGL(k_emf_mixed, dim3(T.nblocks), dim3(T.bs), 0, priv(c)->stream, c->P, c->X, b, T);
+    else hipLaunchKernelGGL(k_emf, dim3(T.nblocks), dim3(T.bs), 0, priv(c)->stream, c->P, c->X, b, T);
     return hip_check(hipGetLastError(), "k_emf");
   }
   static int update(hgpu_ctx* c, int stage) {
     Box b = interior_box(c->P);
     Scope s(c, "update");
-    const Tiling T = make_tiling(b);
+    const Tiling T = make_tiling(b, priv(c)->shape[FAM_CONS]);
+    const Tiling T2 = make_tiling(b, priv(c)->shape[FAM_INV]);
     hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_SLOW, 1);
     hipStream_t st = priv(c)->stream;
-    if (c->P.mixed) hipLaunchKernelGGL(k_conserved<true>, dim3(T.nblocks), dim3(BLOCK), 0, st, c->P, c->X, stage, b, T);
-    else hipLaunchKernelGGL(k_conserved<false>, dim3(T.nblocks), dim3(BLOCK), 0, st, c->P, c->X, stage, b, T);
-    hipLaunchKernelGGL(k_invert, dim3(T.nblocks), dim3(BLOCK), 0, st, c->P, c->X, stage, b, T);
+    if (c->P.mixed) hipLaunchKernelGGL(k_conserved<true>, dim3(T.nblocks), dim3(T.bs), 0, st, c->P, c->X, stage, b, T);
+    else hipLaunchKernelGGL(k_conserved<false>, dim3(T.nblocks), dim3(T.bs), 0, st, c->P, c->X, stage, b, T);
+    hipLaunchKernelGGL(k_invert, dim3(T2.nblocks), dim3(T2.bs), 0, st, c->P, c->X, stage, b, T2);
     return hip_check(hipGetLastError(), "k_update");
   }
   static int recover_queued(hgpu_ctx* c, int stage) {

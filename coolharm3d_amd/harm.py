@@ -140,6 +140,32 @@ class Harm3D:
         self._chk(self.lib.hgpu_step(self.h, nsteps), "hgpu_step")
         self.nstep += nsteps
 
+    def set_graph(self, on=True):
+        """Replay steps from a captured HIP graph (hgpu_set_graph)."""
+        self._chk(self.lib.hgpu_set_graph(self.h, 1 if on else 0), "hgpu_set_graph")
+
+    FAMILIES = ("flux1", "flux2", "flux3", "emf", "conserved", "invert")
+
+    def autotune(self, reps=2):
+        """Times the launch-shape candidates of the stencil kernels on the
+        current state and keeps the fastest (hgpu_autotune).  Returns
+        {family: (bs, kc, remap, ms)}."""
+        rep = np.zeros(20)
+        self._chk(self.lib.hgpu_autotune(self.h, reps, _dp(rep)), "hgpu_autotune")
+        return {f: (int(rep[4 * i]), int(rep[4 * i + 1]), int(rep[4 * i + 2]), float(rep[4 * i + 3]))
+                for i, f in enumerate(self.FAMILIES[:5])}
+
+    def set_launch_shape(self, family, bs=256, kc=0, remap=1):
+        fam = self.FAMILIES.index(family) if isinstance(family, str) else int(family)
+        self._chk(self.lib.hgpu_set_launch_shape(self.h, fam, bs, kc, remap), "hgpu_set_launch_shape")
+
+    def launch_shape(self, family):
+        fam = self.FAMILIES.index(family) if isinstance(family, str) else int(family)
+        v = np.zeros(3, np.int32)
+        self._chk(self.lib.hgpu_get_launch_shape(self.h, fam, _ip(v[0:1]), _ip(v[1:2]), _ip(v[2:3])),
+                  "hgpu_get_launch_shape")
+        return tuple(int(x) for x in v)
+
     def advance(self, stage):
         self._chk(self.lib.hgpu_substep(self.h, stage), "hgpu_substep")
 

@@ -89,6 +89,24 @@ int hgpu_fixup_bounds(hgpu_ctx *ctx);
  * dt = timestep().  Enqueued on the library stream; returns without waiting. */
 int hgpu_step(hgpu_ctx *ctx, int nsteps);
 
+/* on != 0: hgpu_step() captures one step_ch() into a HIP graph on first use and
+ * replays it (one launch per step instead of ~60 kernel launches); per-kernel
+ * profile scopes are then not recorded, "step" still is.  No counterpart in
+ * the reference (launch mechanics only; results are identical). */
+int hgpu_set_graph(hgpu_ctx *ctx, int on);
+
+/* Launch shapes of the stencil kernel families (0-2: face fluxes in x1, x2,
+ * x3; 3: EMFs; 4: conserved update; 5: inversion): workgroup size bs (64, 128,
+ * 256), tile k-extent kc (0 = bs consecutive (j,k) points, else kc x bs/kc
+ * tiles; kc divides bs), XCD-contiguous tile order remap (0/1).
+ * hgpu_autotune() times every candidate of families 0-4 on the current state
+ * (reps launches each) and keeps the fastest; report (NULL or [5][4]) gets
+ * bs, kc, remap, ms per launch.  The state, flags and counters are unchanged;
+ * results do not depend on the shapes.  No counterpart in the reference. */
+int hgpu_autotune(hgpu_ctx *ctx, int reps, double *report);
+int hgpu_set_launch_shape(hgpu_ctx *ctx, int family, int bs, int kc, int remap);
+int hgpu_get_launch_shape(hgpu_ctx *ctx, int family, int *bs, int *kc, int *remap);
+
 /* One advance() stage (0: predictor into ph, 1: corrector into p) followed by
  * bounds and fixups, as step_ch() sequences them (src/step_ch.c:92-180). */
 int hgpu_substep(hgpu_ctx *ctx, int stage);
@@ -139,6 +157,23 @@ int hgpu_con2prim_batch(hgpu_ctx *ctx, long long n, const double *U, const doubl
  * recover_primitives() (src/step_ch.c:832) with Utoprim_2d_fast()
  * (src/utoprim_2d_fast.c:135), which is the per-zone algorithm used here.
  * ------------------------------------------------------------------------ */
+
+/* Device plumbing for callers without a GPU array library: device count,
+ * device buffers, copies (kind 0: host to device, 1: device to host -- waits,
+ * 2: device to device) on a stream, and events for timing work on a stream.
+ * No counterpart in the reference. */
+int hgpu_dev_count(int *out);
+int hgpu_dev_alloc(long long bytes, void **out);
+int hgpu_dev_free(void *p);
+int hgpu_dev_copy(void *dst, const void *src, long long bytes, int kind, void *stream);
+/* device-to-device copy by a plain streaming kernel (bytes % 16 == 0, 16-byte
+ * aligned): traffic of known size for calibrating HBM byte counters */
+int hgpu_dev_copy_kernel(void *dst, const void *src, long long bytes, void *stream);
+int hgpu_stream_sync(void *stream);
+int hgpu_event_create(void **ev);
+int hgpu_event_destroy(void *ev);
+int hgpu_event_record(void *ev, void *stream);
+int hgpu_event_elapsed(void *ev0, void *ev1, float *ms);
 
 /* Samples of the inversion workload from seed: r log-uniform in [1.3, 100],
  * theta uniform in [0.05, pi - 0.05], Lorentz factor log-uniform in [1, 50],

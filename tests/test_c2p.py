@@ -28,9 +28,20 @@ def host():
     return Con2PrimBatch(hostcheck=True)
 
 
+def _np(d):
+    return {k: v.numpy() for k, v in d.items()}
+
+
+def _solve_np(B, U, rth, prim_in=None, **kw):
+    """Solve on numpy inputs through the library's buffers; numpy outputs."""
+    Ud, rd = B.upload(U, np.float64), B.upload(rth, np.float64)
+    pin = B.upload(prim_in, np.float64) if prim_in is not None else None
+    return tuple(x.numpy() for x in B.solve(Ud, rd, A_SPIN, GAM, prim_in=pin, **kw))
+
+
 @pytest.fixture(scope="module")
 def host_samples(host):
-    return host.samples(3000, 2024, A_SPIN, GAM, guess=True)
+    return _np(host.samples(3000, 2024, A_SPIN, GAM, guess=True))
 
 
 # ---------------------------------------------------------------------------
@@ -141,7 +152,7 @@ def test_host_solve_matches_oracle(host, host_samples):
     m = 1500
     U, rth, guess = (np.ascontiguousarray(d[k][:, :m]) for k in ("U", "rth", "guess"))
     P, code, iters = co.utoprim_zones(A_SPIN, GAM, U, rth, guess)
-    p, c, it = host.solve(U, rth, A_SPIN, GAM, plain_guess="prim", prim_in=guess, compact=True)
+    p, c, it = _solve_np(host, U, rth, prim_in=guess, plain_guess="prim", compact=True)
     assert np.array_equal(c, code)
     assert np.array_equal(it, iters)
     assert np.array_equal(p, P)
@@ -153,16 +164,16 @@ def test_host_compaction_is_exact(host, host_samples, budgets, plain):
     """Parking zones in the compaction queue keeps every decision: identical results."""
     d = host_samples
     kw = dict(plain_guess=plain, prim_in=d["guess"] if plain == "prim" else None)
-    p0, c0, i0 = host.solve(d["U"], d["rth"], A_SPIN, GAM, compact=False, **kw)
-    p1, c1, i1 = host.solve(d["U"], d["rth"], A_SPIN, GAM, compact=True, budget_first=budgets[0],
-                            budget_next=budgets[1], **kw)
+    p0, c0, i0 = _solve_np(host, d["U"], d["rth"], compact=False, **kw)
+    p1, c1, i1 = _solve_np(host, d["U"], d["rth"], compact=True, budget_first=budgets[0],
+                           budget_next=budgets[1], **kw)
     assert np.array_equal(p0, p1) and np.array_equal(c0, c1) and np.array_equal(i0, i1)
 
 
 def test_host_round_trip(host, host_samples):
     """prim -> U -> prim recovers the sampled state (cold guess)."""
     d = host_samples
-    p, c, it = host.solve(d["U"], d["rth"], A_SPIN, GAM, plain_guess="cold")
+    p, c, it = _solve_np(host, d["U"], d["rth"], plain_guess="cold")
     ok = c == 0
     assert ok.mean() > 0.999
     P = d["prim"]
@@ -175,13 +186,15 @@ def test_host_round_trip(host, host_samples):
 
 def test_host_mlp_matches_numpy(host, host_samples):
     d = host_samples
-    feat, target = host.features(d["U"], d["rth"], A_SPIN, GAM, prim=d["prim"])
-    w = host.init(3, feat, target)
-    y = host.forward(w, feat)
+    U, rth, P = (host.upload(d[k]) for k in ("U", "rth", "prim"))
+    featd, targetd = host.features(U, rth, A_SPIN, GAM, prim=P)
+    wd = host.init(3, featd, targetd)
+    feat, target, w = featd.numpy(), targetd.numpy(), wd.numpy()
+    y = host.forward(wd, featd).numpy()
     assert np.allclose(y, co.mlp_forward(w, feat), rtol=2e-5, atol=2e-5)
     m = 700
     f, t = np.ascontiguousarray(feat[:, :m]), np.ascontiguousarray(target[:, :m])
-    g, loss = host.grad(w, f, t)
+    g, loss = (x.numpy() for x in host.grad(wd, host.upload(f), host.upload(t)))
     lo, gref = co.mlp_loss_grad(w, f, t)
     assert abs(loss[0] - lo) <= 1e-5 * abs(lo)
     assert np.max(np.abs(g - gref)) <= 2e-5 * np.max(np.abs(gref))
@@ -195,8 +208,8 @@ def test_host_training_and_fallback(host):
     w, hist = host.train_guess(3000, 91, A_SPIN, GAM, steps=120, batch=256, lr=3e-3)
     assert hist[-10:].mean() < 0.5 * hist[:5].mean()
     d = host.samples(3000, 17, A_SPIN, GAM)
-    p0, c0, _ = host.solve(d["U"], d["rth"], A_SPIN, GAM, plain_guess="cold")
-    p1, c1, _ = host.solve(d["U"], d["rth"], A_SPIN, GAM, plain_guess="cold", mlp=w)
+    p0, c0, _ = (x.numpy() for x in host.solve(d["U"], d["rth"], A_SPIN, GAM, plain_guess="cold"))
+    p1, c1, _ = (x.numpy() for x in host.solve(d["U"], d["rth"], A_SPIN, GAM, plain_guess="cold", mlp=w))
     assert np.all(c1[c0 == 0] == 0)
     ok = (c0 == 0) & (c1 == 0)
     assert np.all(np.abs(p1[0, ok] - p0[0, ok]) <= 1e-8 * p0[0, ok])
@@ -204,16 +217,14 @@ def test_host_training_and_fallback(host):
 
 def test_entry_point_errors(host):
     d = host.samples(10, 1, A_SPIN, GAM)
-    rc = host.lib.hgpu_c2p_solve(None, 10, A_SPIN, GAM, host.ptr(d["U"]), host.ptr(d["rth"]), None, 2, 0, 0, 0,
-                                 None, 0, host.ptr(d["prim"]), host.ptr(np.zeros(10, np.int32)),
-                                 host.ptr(np.zeros(10, np.int32)))
+    code, iters = host.empty((10,), np.int32), host.empty((10,), np.int32)
+    rc = host.lib.hgpu_c2p_solve(None, 10, A_SPIN, GAM, d["U"].ptr, d["rth"].ptr, None, 2, 0, 0, 0,
+                                 None, 0, d["prim"].ptr, code.ptr, iters.ptr)
     assert rc < 0 and b"plain_guess" in host.lib.hgpu_last_error()
-    rc = host.lib.hgpu_c2p_solve(None, 10, A_SPIN, GAM, host.ptr(d["U"]), host.ptr(d["rth"]), None, 1, 1, 0, 0,
-                                 None, 0, host.ptr(d["prim"]), host.ptr(np.zeros(10, np.int32)),
-                                 host.ptr(np.zeros(10, np.int32)))
+    rc = host.lib.hgpu_c2p_solve(None, 10, A_SPIN, GAM, d["U"].ptr, d["rth"].ptr, None, 1, 1, 0, 0,
+                                 None, 0, d["prim"].ptr, code.ptr, iters.ptr)
     assert rc < 0 and b"workspace" in host.lib.hgpu_last_error()
-    rc = host.lib.hgpu_c2p_samples(None, 10, 1, 1.0, GAM, host.ptr(d["U"]), host.ptr(d["rth"]), host.ptr(d["prim"]),
-                                   None)
+    rc = host.lib.hgpu_c2p_samples(None, 10, 1, 1.0, GAM, d["U"].ptr, d["rth"].ptr, d["prim"].ptr, None)
     assert rc < 0 and b"spin" in host.lib.hgpu_last_error()
 
 
@@ -229,12 +240,11 @@ def gpu():
 @pytest.mark.gpu
 def test_gpu_samples_match_host(gpu, host):
     n = 1 << 15
-    g = gpu.samples(n, 2024, A_SPIN, GAM, guess=True)
-    h = host.samples(n, 2024, A_SPIN, GAM, guess=True)
+    g = _np(gpu.samples(n, 2024, A_SPIN, GAM, guess=True))
+    h = _np(host.samples(n, 2024, A_SPIN, GAM, guess=True))
     for k in ("rth", "prim", "guess"):
-        assert np.allclose(gpu.to_numpy(g[k]), h[k], rtol=1e-12, atol=0), k
-    U = gpu.to_numpy(g["U"])
-    assert np.all(np.abs(U - h["U"]) <= 1e-11 * np.abs(h["U"]).max(axis=0))
+        assert np.allclose(g[k], h[k], rtol=1e-12, atol=0), k
+    assert np.all(np.abs(g["U"] - h["U"]) <= 1e-11 * np.abs(h["U"]).max(axis=0))
 
 
 @pytest.mark.gpu
@@ -242,29 +252,33 @@ def test_gpu_samples_match_host(gpu, host):
 def test_gpu_solve_matches_host_and_oracle(gpu, host, plain):
     """GPU batched solve vs the host build (same inputs) and vs the oracle."""
     n = 1 << 15
-    g = gpu.samples(n, 99, A_SPIN, GAM, guess=True)
-    U, rth, guess = (gpu.to_numpy(g[k]) for k in ("U", "rth", "guess"))
+    gd = gpu.samples(n, 99, A_SPIN, GAM, guess=True)
+    g = _np(gd)
     kw = dict(plain_guess=plain)
-    p, c, it = gpu.solve(g["U"], g["rth"], A_SPIN, GAM, prim_in=g["guess"] if plain == "prim" else None,
-                         compact=True, **kw)
-    p, c, it = gpu.to_numpy(p), gpu.to_numpy(c), gpu.to_numpy(it)
-    ph, ch, ih = host.solve(U, rth, A_SPIN, GAM, prim_in=guess if plain == "prim" else None, compact=False, **kw)
+    p, c, it = (x.numpy() for x in gpu.solve(gd["U"], gd["rth"], A_SPIN, GAM,
+                                             prim_in=gd["guess"] if plain == "prim" else None, compact=True, **kw))
+    ph, ch, ih = _solve_np(host, g["U"], g["rth"], prim_in=g["guess"] if plain == "prim" else None,
+                           compact=False, **kw)
     # the Kerr-Schild record uses the device's sin/cos; a last-ulp difference may
     # move a zone across the Newton tolerance
+    print(f"[c2p] {plain}: codes equal {np.mean(c == ch):.6f} iters equal {np.mean(it == ih):.6f} "
+          f"prims exact {np.mean(p == ph):.4f}", flush=True)
     assert np.mean(c == ch) >= 0.9999
     assert np.mean(it == ih) >= 0.999
     both = (c == 0) & (ch == 0)
     for v in range(5):
         assert np.all(np.abs(p[v, both] - ph[v, both]) <= 1e-9 * (np.abs(ph[v, both]) + ph[0, both]))
     m = 300
-    gin = guess[:, :m] if plain == "prim" else None
-    if plain == "cold":
+    U, rth = g["U"][:, :m], g["rth"][:, :m]
+    if plain == "prim":
+        gin = g["guess"][:, :m]
+    else:
         gin = np.zeros((8, m))
         for z in range(m):
             gm = co.ks_geom(A_SPIN, rth[0, z], rth[1, z])
             D = U[0, z] * gm[34] / gm[32]
             gin[:2, z] = (D, 1e-3 * D)
-    P, code, _ = co.utoprim_zones(A_SPIN, GAM, U[:, :m], rth[:, :m], gin)
+    P, code, _ = co.utoprim_zones(A_SPIN, GAM, U, rth, gin)
     assert np.mean(code == c[:m]) >= 0.99
     ok = (code == 0) & (c[:m] == 0)
     assert np.all(np.abs(P[0, ok] - p[0, :m][ok]) <= 1e-9 * P[0, ok])
@@ -275,9 +289,9 @@ def test_gpu_solve_matches_host_and_oracle(gpu, host, plain):
 def test_gpu_compaction_is_exact(gpu, budgets):
     n = 1 << 18
     g = gpu.samples(n, 5, A_SPIN, GAM)
-    p0, c0, i0 = (gpu.to_numpy(x) for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, compact=False))
-    p1, c1, i1 = (gpu.to_numpy(x) for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, compact=True,
-                                                     budget_first=budgets[0], budget_next=budgets[1]))
+    p0, c0, i0 = (x.numpy() for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, compact=False))
+    p1, c1, i1 = (x.numpy() for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, compact=True,
+                                               budget_first=budgets[0], budget_next=budgets[1]))
     assert np.array_equal(c0, c1) and np.array_equal(i0, i1) and np.array_equal(p0, p1)
 
 
@@ -287,13 +301,14 @@ def test_gpu_mlp_gradient_matches_cpu(gpu):
     g = gpu.samples(5000, 12, A_SPIN, GAM)
     feat, target = gpu.features(g["U"], g["rth"], A_SPIN, GAM, prim=g["prim"])
     w = gpu.init(4, feat, target)
-    f, t, wn = gpu.to_numpy(feat), gpu.to_numpy(target), gpu.to_numpy(w)
-    y = gpu.to_numpy(gpu.forward(w, feat))
+    f, t, wn = feat.numpy(), target.numpy(), w.numpy()
+    y = gpu.forward(w, feat).numpy()
     assert np.allclose(y, co.mlp_forward(wn, f), rtol=2e-5, atol=2e-5)
     gr, loss = gpu.grad(w, feat, target)
     lo, gref = co.mlp_loss_grad(wn, f, t)
-    assert abs(gpu.to_numpy(loss)[0] - lo) <= 1e-5 * abs(lo)
-    assert np.max(np.abs(gpu.to_numpy(gr) - gref)) <= 2e-5 * np.max(np.abs(gref))
+    print(f"[c2p] grad max rel err {np.max(np.abs(gr.numpy() - gref)) / np.max(np.abs(gref)):.3e}", flush=True)
+    assert abs(loss.numpy()[0] - lo) <= 1e-5 * abs(lo)
+    assert np.max(np.abs(gr.numpy() - gref)) <= 2e-5 * np.max(np.abs(gref))
 
 
 @pytest.mark.gpu
@@ -301,12 +316,14 @@ def test_gpu_learned_guess_gain(gpu):
     """Trained on zones it solved itself, the network's guess lowers the mean
     Newton iterations of the cold-guess solve without adding failures, and
     solutions agree to the Newton tolerance."""
-    w, hist = gpu.train_guess(1 << 17, 777, A_SPIN, GAM, steps=1500, batch=4096, lr=3e-3)
-    h = gpu.to_numpy(hist)
+    w, h = gpu.train_guess(1 << 17, 777, A_SPIN, GAM, steps=1500, batch=4096, lr=3e-3)
+    print(f"[c2p] training loss {h[:5].mean():.4g} -> {h[-50:].mean():.4g}", flush=True)
     assert h[-50:].mean() < 0.05 * h[:5].mean()
     g = gpu.samples(1 << 18, 31337, A_SPIN, GAM)
-    p0, c0, i0 = (gpu.to_numpy(x) for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, plain_guess="cold"))
-    p1, c1, i1 = (gpu.to_numpy(x) for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, plain_guess="cold", mlp=w))
+    p0, c0, i0 = (x.numpy() for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, plain_guess="cold"))
+    p1, c1, i1 = (x.numpy() for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, plain_guess="cold", mlp=w))
+    print(f"[c2p] mean iterations cold {i0.mean():.3f} learned {i1.mean():.3f}; failures "
+          f"{(c0 != 0).mean():.2e} / {(c1 != 0).mean():.2e}", flush=True)
     assert i1.mean() < 0.8 * i0.mean(), (i1.mean(), i0.mean())
     assert (c1 != 0).mean() <= (c0 != 0).mean()
     ok = (c0 == 0) & (c1 == 0)
