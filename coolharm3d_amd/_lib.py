@@ -36,6 +36,16 @@ class HgpuConfig(ctypes.Structure):
     ]
 
 
+class HgpuPnConfig(ctypes.Structure):
+    _fields_ = [
+        ("m1", ctypes.c_double), ("m2", ctypes.c_double), ("b0", ctypes.c_double), ("phi0", ctypes.c_double),
+        ("inspiral", ctypes.c_int), ("R0", ctypes.c_double), ("th_beg", ctypes.c_double),
+        ("th_length", ctypes.c_double), ("h_slope", ctypes.c_double),
+        ("n1", ctypes.c_int), ("n2", ctypes.c_int), ("n3", ctypes.c_int),
+        ("startx", ctypes.c_double * 4), ("dx", ctypes.c_double * 4),
+    ]
+
+
 # exported symbols and their signatures (restype, argtypes)
 SIGNATURES = {
     "hgpu_last_error": (ctypes.c_char_p, []),
@@ -80,6 +90,9 @@ SIGNATURES = {
     "hgpu_event_destroy": (ctypes.c_int, [_vp]),
     "hgpu_event_record": (ctypes.c_int, [_vp, _vp]),
     "hgpu_event_elapsed": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+    # binary-black-hole metric
+    "hgpu_pn_geometry": (ctypes.c_int, [_vp, ctypes.POINTER(HgpuPnConfig), _d, ctypes.c_int, _vp]),
+    "hgpu_pn_connection": (ctypes.c_int, [_vp, ctypes.POINTER(HgpuPnConfig), _d, _vp]),
     # batched con-to-prim and its network (pointers are addresses: ints)
     "hgpu_c2p_samples": (ctypes.c_int, [_vp, _ll, ctypes.c_ulonglong, _d, _d, _vp, _vp, _vp, _vp]),
     "hgpu_c2p_workspace_bytes": (_ll, [_ll, ctypes.c_int]),

@@ -2,7 +2,8 @@
 
 * ``coolharm3d_amd/libhgpu.so`` -- the product: HIP kernels for gfx950 behind the
   C ABI of ``include/hgpu.h`` (hipcc): the GRMHD advance (hgpu_kernels.hip) and
-  the batched con-to-prim with its learned guess (hgpu_c2p.hip).
+  the batched con-to-prim with its learned guess (hgpu_c2p.hip), and the
+  binary-black-hole metric (hgpu_pn.hip).
 * ``coolharm3d_amd/libhgpu_hostcheck.so`` -- the same kernel bodies compiled for
   the host (g++), used only by the CPU tests to compare the kernels' arithmetic
   with the reference where no GPU exists.
@@ -46,7 +47,7 @@ def build_gpu():
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
     tmp = GPU_LIB + ".tmp"
     _run([hipcc] + HIP_FLAGS + ["-o", tmp, os.path.join(CSRC, "hgpu_kernels.hip"),
-                                os.path.join(CSRC, "hgpu_c2p.hip")])
+                                os.path.join(CSRC, "hgpu_c2p.hip"), os.path.join(CSRC, "hgpu_pn.hip")])
     os.replace(tmp, GPU_LIB)
 
 
@@ -54,7 +55,7 @@ def build_hostcheck():
     cxx = shutil.which("g++") or "g++"
     tmp = HOSTCHECK_LIB + ".tmp"
     _run([cxx] + HOST_FLAGS + ["-o", tmp, os.path.join(CSRC, "hgpu_hostcheck.cpp"),
-                               os.path.join(CSRC, "hgpu_c2p_host.cpp")])
+                               os.path.join(CSRC, "hgpu_c2p_host.cpp"), os.path.join(CSRC, "hgpu_pn_host.cpp")])
     os.replace(tmp, HOSTCHECK_LIB)
 
 

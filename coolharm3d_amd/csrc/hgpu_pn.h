@@ -1,0 +1,264 @@
+// hgpu_pn.h -- time-dependent metric of an inspiralling black-hole binary for
+// the circumbinary-disk configuration (host/device bodies).
+//
+// The reference declares its post-Newtonian binary metrics
+// (dyn_35pn_nz_gcov_func, dyn_fullpn_nz_gcov_func, src/metric.h:104-171) but
+// ships no body for them, so this is written from the published method:
+//
+// * Metric: the 1PN near-zone metric of two point masses in harmonic
+//   coordinates (Blanchet, Faye & Ponsot 1998, Phys. Rev. D 58, 124002), G = c = 1:
+//     g00 = -1 + 2V - 2V^2,  g0i = -4 V_i,  gij = delta_ij (1 + 2V),
+//   with V the 1PN retarded potential of the two bodies,
+//     V = sum_a m_a/r_a [1 + 2 v_a^2 - (n_a.v_a)^2/2] - m_a m_b/(r_a r_ab)
+//         + m_a m_b (r_b^2 - r_a^2 - r_ab^2)/(4 r_a r_ab^3)     (b != a),
+//   and V_i = sum_a m_a v_a^i / r_a  (y_b at distance r_b from the field point).
+// * Orbit: quasi-circular, centre of mass at rest at the origin, Newtonian
+//   Kepler frequency omega = sqrt(M/b^3), separation shrinking by the leading
+//   (2.5PN) radiation reaction, db/dt = -(64/5) m1 m2 M / b^3 (Peters 1964):
+//     b(t)^4 = b0^4 - (256/5) m1 m2 M t,
+//     phi(t) = phi0 + (8 sqrt(M) / (5 k)) (b0^(5/2) - b(t)^(5/2)),  k = (256/5) m1 m2 M.
+// * Coordinates: spherical polar about the centre of mass, numerical
+//   coordinates X1 = log(r - R0), X2 in [0,1] with
+//   theta = th_length X2 + th_beg + h_slope sin(2 pi X2) (the modified
+//   Kerr-Schild map, src/coord.c:240-382), X3 = phi.  g in numerical
+//   coordinates is J^T g_harmonic J with J = d(t, x, y, z)/d(t, X1, X2, X3).
+// * Everything is templated on the scalar: double for values, Dual4 for values
+//   and the four partial derivatives d/d(t, X1, X2, X3) (forward-mode
+//   differentiation), from which the connection Gamma^a_bc, including the
+//   time derivatives of the moving metric, is formed exactly.
+#pragma once
+#include "hgpu_types.h"
+#include <math.h>
+
+namespace hgpu {
+
+// ---------------------------------------------------------------------------
+// forward-mode numbers in (t, X1, X2, X3); their functions are found by
+// argument-dependent lookup, so double arguments keep the libm functions
+namespace ad {
+struct Dual4 {
+  double v, d[4];
+};
+HD Dual4 dc(double v) { Dual4 r; r.v = v; r.d[0] = r.d[1] = r.d[2] = r.d[3] = 0.; return r; }
+HD Dual4 dvar(double v, int k) { Dual4 r = dc(v); r.d[k] = 1.; return r; }
+HD Dual4 operator+(const Dual4& a, const Dual4& b) {
+  Dual4 r; r.v = a.v + b.v; for (int k = 0; k < 4; k++) r.d[k] = a.d[k] + b.d[k]; return r;
+}
+HD Dual4 operator-(const Dual4& a, const Dual4& b) {
+  Dual4 r; r.v = a.v - b.v; for (int k = 0; k < 4; k++) r.d[k] = a.d[k] - b.d[k]; return r;
+}
+HD Dual4 operator-(const Dual4& a) {
+  Dual4 r; r.v = -a.v; for (int k = 0; k < 4; k++) r.d[k] = -a.d[k]; return r;
+}
+HD Dual4 operator*(const Dual4& a, const Dual4& b) {
+  Dual4 r; r.v = a.v * b.v; for (int k = 0; k < 4; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k]; return r;
+}
+HD Dual4 operator/(const Dual4& a, const Dual4& b) {
+  const double ib = 1. / b.v;
+  Dual4 r; r.v = a.v * ib;
+  for (int k = 0; k < 4; k++) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
+  return r;
+}
+HD Dual4 operator+(const Dual4& a, double s) { Dual4 r = a; r.v += s; return r; }
+HD Dual4 operator+(double s, const Dual4& a) { return a + s; }
+HD Dual4 operator-(const Dual4& a, double s) { Dual4 r = a; r.v -= s; return r; }
+HD Dual4 operator-(double s, const Dual4& a) { return (-a) + s; }
+HD Dual4 operator*(const Dual4& a, double s) {
+  Dual4 r; r.v = a.v * s; for (int k = 0; k < 4; k++) r.d[k] = a.d[k] * s; return r;
+}
+HD Dual4 operator*(double s, const Dual4& a) { return a * s; }
+HD Dual4 operator/(const Dual4& a, double s) { return a * (1. / s); }
+HD Dual4 operator/(double s, const Dual4& a) { return dc(s) / a; }
+HD Dual4 chain(const Dual4& a, double f, double df) {
+  Dual4 r; r.v = f; for (int k = 0; k < 4; k++) r.d[k] = df * a.d[k]; return r;
+}
+HD Dual4 sqrt(const Dual4& a) { const double s = ::sqrt(a.v); return chain(a, s, 0.5 / s); }
+HD Dual4 sin(const Dual4& a) { return chain(a, ::sin(a.v), ::cos(a.v)); }
+HD Dual4 cos(const Dual4& a) { return chain(a, ::cos(a.v), -::sin(a.v)); }
+HD Dual4 exp(const Dual4& a) { const double e = ::exp(a.v); return chain(a, e, e); }
+HD Dual4 pow(const Dual4& a, double p) { const double f = ::pow(a.v, p); return chain(a, f, p * f / a.v); }
+}  // namespace ad
+using ad::Dual4;
+using ad::dc;
+using ad::dvar;
+
+// ---------------------------------------------------------------------------
+struct PnBinary {
+  double m1, m2;          // masses (m1 + m2 = M)
+  double b0, phi0;        // separation and phase at t = 0
+  int inspiral;           // 1: radiation-reaction decay of b (when both masses are > 0);
+                          // 0: circular orbit of fixed b0
+  // numerical coordinates
+  double R0, th_beg, th_length, h_slope;
+};
+
+// Positions and velocities of the two holes at time t.
+template <class T>
+HD void pn_orbit(const PnBinary& B, const T& t, T y1[3], T y2[3], T v1[3], T v2[3]) {
+  const double M = B.m1 + B.m2;
+  const double k = (256. / 5.) * B.m1 * B.m2 * M;
+  T b, bdot, phi, omega;
+  if (B.inspiral && k > 0.) {
+    const T b4 = B.b0 * B.b0 * B.b0 * B.b0 - k * t;
+    b = pow(b4, 0.25);
+    bdot = -(k / 4.) / (b * b * b);
+    phi = B.phi0 + (8. * ::sqrt(M) / (5. * k)) * (::pow(B.b0, 2.5) - pow(b4, 0.625));
+    omega = ::sqrt(M) / pow(b4, 0.375);
+  } else {
+    b = t * 0. + B.b0;
+    bdot = t * 0.;
+    omega = t * 0. + ::sqrt(M / (B.b0 * B.b0 * B.b0));
+    phi = B.phi0 + omega * t;
+  }
+  const T c = cos(phi), s = sin(phi);
+  // relative separation vector y1 - y2 = b (c, s, 0); centre of mass at rest
+  const double f1 = B.m2 / M, f2 = -B.m1 / M;
+  const T rx = b * c, ry = b * s;
+  const T vx = bdot * c - b * omega * s, vy = bdot * s + b * omega * c;
+  y1[0] = f1 * rx; y1[1] = f1 * ry; y1[2] = t * 0.;
+  y2[0] = f2 * rx; y2[1] = f2 * ry; y2[2] = t * 0.;
+  v1[0] = f1 * vx; v1[1] = f1 * vy; v1[2] = t * 0.;
+  v2[0] = f2 * vx; v2[1] = f2 * vy; v2[2] = t * 0.;
+}
+
+// 1PN near-zone metric in harmonic coordinates at Cartesian point x, time t.
+template <class T>
+HD void pn_gcov_harmonic(const PnBinary& B, const T& t, const T x[3], T g[4][4]) {
+  T y[2][3], v[2][3];
+  pn_orbit(B, t, y[0], y[1], v[0], v[1]);
+  const double m[2] = {B.m1, B.m2};
+  T r[2], nv[2], vsq[2];
+  for (int a = 0; a < 2; a++) {
+    const T dx = x[0] - y[a][0], dy = x[1] - y[a][1], dz = x[2] - y[a][2];
+    r[a] = sqrt(dx * dx + dy * dy + dz * dz);
+    nv[a] = (dx * v[a][0] + dy * v[a][1] + dz * v[a][2]) / r[a];
+    vsq[a] = v[a][0] * v[a][0] + v[a][1] * v[a][1] + v[a][2] * v[a][2];
+  }
+  const T ex = y[0][0] - y[1][0], ey = y[0][1] - y[1][1], ez = y[0][2] - y[1][2];
+  const T r12 = sqrt(ex * ex + ey * ey + ez * ez);
+  const T r12_3 = r12 * r12 * r12;
+  T U = t * 0., V = t * 0.;
+  for (int a = 0; a < 2; a++) {
+    const int b = 1 - a;
+    const T mr = m[a] / r[a];
+    U = U + mr;
+    V = V + mr * (1. + 2. * vsq[a] - 0.5 * nv[a] * nv[a]) - m[a] * m[b] / (r[a] * r12)
+          + (m[a] * m[b] / 4.) * (r[b] * r[b] - r[a] * r[a] - r12 * r12) / (r[a] * r12_3);
+  }
+  g[0][0] = -1. + 2. * V - 2. * U * U;
+  for (int i = 0; i < 3; i++) {
+    T Vi = t * 0.;
+    for (int a = 0; a < 2; a++) Vi = Vi + m[a] * v[a][i] / r[a];
+    g[0][i + 1] = g[i + 1][0] = -4. * Vi;
+  }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) g[i + 1][j + 1] = (i == j) ? 1. + 2. * U : t * 0.;
+}
+
+// Metric in the numerical coordinates (t, X1, X2, X3) of the spherical grid.
+template <class T>
+HD void pn_gcov_num(const PnBinary& B, const T& t, const T& X1, const T& X2, const T& X3, T g[4][4]) {
+  const T er = exp(X1);
+  const T r = er + B.R0;
+  const T th = B.th_length * X2 + B.th_beg + B.h_slope * sin(2. * M_PI * X2);
+  const T dr = er;
+  const T dth = B.th_length + 2. * M_PI * B.h_slope * cos(2. * M_PI * X2);
+  const T st = sin(th), ct = cos(th), sp = sin(X3), cp = cos(X3);
+  T x[3] = {r * st * cp, r * st * sp, r * ct};
+  T gh[4][4];
+  pn_gcov_harmonic(B, t, x, gh);
+  // J[mu][nu] = d x^mu / d X^nu (t row and column trivial)
+  T J[4][4];
+  for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) J[a][b] = t * 0.;
+  J[0][0] = t * 0. + 1.;
+  J[1][1] = st * cp * dr; J[1][2] = r * ct * cp * dth; J[1][3] = -r * st * sp;
+  J[2][1] = st * sp * dr; J[2][2] = r * ct * sp * dth; J[2][3] = r * st * cp;
+  J[3][1] = ct * dr;      J[3][2] = -r * st * dth;     J[3][3] = t * 0.;
+  T tmp[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int n = 0; n < 4; n++) {
+      T s = t * 0.;
+      for (int b = 0; b < 4; b++) s = s + gh[a][b] * J[b][n];
+      tmp[a][n] = s;
+    }
+  for (int m2 = 0; m2 < 4; m2++)
+    for (int n = m2; n < 4; n++) {
+      T s = t * 0.;
+      for (int a = 0; a < 4; a++) s = s + J[a][m2] * tmp[a][n];
+      g[m2][n] = s;
+      g[n][m2] = s;
+    }
+}
+
+// Inverse and determinant of a 4x4 matrix by the Laplace expansion in 2x2
+// minors (closed form: no pivoting, no indexed scratch).
+HD double invert4(const double a[4][4], double b[4][4]) {
+  const double s0 = a[0][0]*a[1][1] - a[1][0]*a[0][1], s1 = a[0][0]*a[1][2] - a[1][0]*a[0][2];
+  const double s2 = a[0][0]*a[1][3] - a[1][0]*a[0][3], s3 = a[0][1]*a[1][2] - a[1][1]*a[0][2];
+  const double s4 = a[0][1]*a[1][3] - a[1][1]*a[0][3], s5 = a[0][2]*a[1][3] - a[1][2]*a[0][3];
+  const double c5 = a[2][2]*a[3][3] - a[3][2]*a[2][3], c4 = a[2][1]*a[3][3] - a[3][1]*a[2][3];
+  const double c3 = a[2][1]*a[3][2] - a[3][1]*a[2][2], c2 = a[2][0]*a[3][3] - a[3][0]*a[2][3];
+  const double c1 = a[2][0]*a[3][2] - a[3][0]*a[2][2], c0 = a[2][0]*a[3][1] - a[3][0]*a[2][1];
+  const double det = s0*c5 - s1*c4 + s2*c3 + s3*c2 - s4*c1 + s5*c0;
+  const double id = 1. / det;
+  b[0][0] = ( a[1][1]*c5 - a[1][2]*c4 + a[1][3]*c3) * id;
+  b[0][1] = (-a[0][1]*c5 + a[0][2]*c4 - a[0][3]*c3) * id;
+  b[0][2] = ( a[3][1]*s5 - a[3][2]*s4 + a[3][3]*s3) * id;
+  b[0][3] = (-a[2][1]*s5 + a[2][2]*s4 - a[2][3]*s3) * id;
+  b[1][0] = (-a[1][0]*c5 + a[1][2]*c2 - a[1][3]*c1) * id;
+  b[1][1] = ( a[0][0]*c5 - a[0][2]*c2 + a[0][3]*c1) * id;
+  b[1][2] = (-a[3][0]*s5 + a[3][2]*s2 - a[3][3]*s1) * id;
+  b[1][3] = ( a[2][0]*s5 - a[2][2]*s2 + a[2][3]*s1) * id;
+  b[2][0] = ( a[1][0]*c4 - a[1][1]*c2 + a[1][3]*c0) * id;
+  b[2][1] = (-a[0][0]*c4 + a[0][1]*c2 - a[0][3]*c0) * id;
+  b[2][2] = ( a[3][0]*s4 - a[3][1]*s2 + a[3][3]*s0) * id;
+  b[2][3] = (-a[2][0]*s4 + a[2][1]*s2 - a[2][3]*s0) * id;
+  b[3][0] = (-a[1][0]*c3 + a[1][1]*c1 - a[1][2]*c0) * id;
+  b[3][1] = ( a[0][0]*c3 - a[0][1]*c1 + a[0][2]*c0) * id;
+  b[3][2] = (-a[3][0]*s3 + a[3][1]*s1 - a[3][2]*s0) * id;
+  b[3][3] = ( a[2][0]*s3 - a[2][1]*s1 + a[2][2]*s0) * id;
+  return det;
+}
+
+// Geometry record (struct of_geom layout) at (t, X): metric, inverse, sqrt(-g),
+// lapse, shift and normal observer, as calc_all_geom() forms them
+// (src/metric.c:4221-4232).
+HD void pn_geom(const PnBinary& B, double t, double X1, double X2, double X3, Geom* G) {
+  double g[4][4];
+  pn_gcov_num(B, t, X1, X2, X3, g);
+  for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) G->gcov[a][b] = g[a][b];
+  const double det = invert4(g, G->gcon);
+  G->g = ::sqrt(-det);
+  G->g_inv = 1. / G->g;
+  G->ncon[0] = ::sqrt(-G->gcon[0][0]);
+  G->alpha = 1. / G->ncon[0];
+  G->ncon[1] = -G->alpha * G->gcon[0][1];
+  G->ncon[2] = -G->alpha * G->gcon[0][2];
+  G->ncon[3] = -G->alpha * G->gcon[0][3];
+  G->beta[0] = -G->gcon[0][1] / G->gcon[0][0];
+  G->beta[1] = -G->gcon[0][2] / G->gcon[0][0];
+  G->beta[2] = -G->gcon[0][3] / G->gcon[0][0];
+}
+
+// Connection Gamma^d_{ab} = g^{dc} (d_a g_cb + d_b g_ca - d_c g_ab) / 2 at
+// (t, X), time derivatives included; conn[(d*4 + a)*4 + b] (conn[l][d][a][b]).
+HD void pn_conn(const PnBinary& B, double t, double X1, double X2, double X3, double* conn) {
+  Dual4 g[4][4];
+  pn_gcov_num(B, dvar(t, 0), dvar(X1, 1), dvar(X2, 2), dvar(X3, 3), g);
+  double gv[4][4], gi[4][4];
+  for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) gv[a][b] = g[a][b].v;
+  invert4(gv, gi);
+  double low[4][4][4];   // Gamma_{c a b}
+  for (int c = 0; c < 4; c++)
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) low[c][a][b] = 0.5 * (g[c][b].d[a] + g[c][a].d[b] - g[a][b].d[c]);
+  for (int d = 0; d < 4; d++)
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) {
+        double s = 0.;
+        for (int c = 0; c < 4; c++) s += gi[d][c] * low[c][a][b];
+        conn[(d * 4 + a) * 4 + b] = s;
+      }
+}
+
+}  // namespace hgpu

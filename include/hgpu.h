@@ -232,6 +232,34 @@ int hgpu_mlp_train(void *stream, long long n, const float *feat, const float *ta
                    int steps, int batch, float lr, void *work, long long work_bytes, float *w,
                    float *loss_hist);
 
+/* ------------------------------------------------------------------------
+ * Time-dependent metric of an inspiralling black-hole binary (the circumbinary
+ * disk configuration).  The reference declares these metrics
+ * (dyn_35pn_nz_gcov_func, dyn_fullpn_nz_gcov_func, src/metric.h:104-171) without
+ * bodies; this one is the 1PN near-zone metric of two point masses in harmonic
+ * coordinates (Blanchet, Faye & Ponsot 1998) with a quasi-circular orbit
+ * shrinking by leading-order radiation reaction (Peters 1964), on spherical
+ * numerical coordinates X1 = log(r - R0), theta(X2) as the modified
+ * Kerr-Schild map, X3 = phi, about the centre of mass.
+ * ------------------------------------------------------------------------ */
+typedef struct hgpu_pn_config {
+  double m1, m2;             /* hole masses                                          */
+  double b0, phi0;           /* separation and orbital phase at t = 0               */
+  int inspiral;              /* 1: radiation-reaction inspiral, 0: fixed circular orbit */
+  double R0, th_beg, th_length, h_slope;  /* numerical-coordinate map              */
+  int n1, n2, n3;            /* zones of the block (ghosts included if wanted)     */
+  double startx[4], dx[4];   /* X of the block's first zone corner, zone widths    */
+} hgpu_pn_config;
+
+/* Geometry records (struct of_geom, 42 doubles) of every zone of the block at
+ * position pos (0 FACE1, 1 FACE2, 2 FACE3, 3 CENT, 4 CORN as coord(),
+ * src/coord.c:182-228) and time t: geom[(i*n2 + j)*n3 + k][42]. */
+int hgpu_pn_geometry(void *stream, const hgpu_pn_config *cfg, double t, int pos, double *geom);
+
+/* Connection Gamma^d_{ab} (time derivatives included) at zone centres and
+ * time t: conn[(i*n2 + j)*n3 + k][d][a][b]. */
+int hgpu_pn_connection(void *stream, const hgpu_pn_config *cfg, double t, double *conn);
+
 #ifdef __cplusplus
 }
 #endif

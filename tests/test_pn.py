@@ -1,0 +1,150 @@
+"""Binary-black-hole metric (the circumbinary-disk configuration).
+
+The reference has no body for its post-Newtonian binary metrics
+(src/metric.h:104-171 declares them only), so this row is parity-unpinned:
+the metric is held to what the mathematics fixes -- the inverse and
+determinant, the flat far field, the single-hole limit (harmonic Schwarzschild
+to 1PN order), the pi-rotation symmetry of an equal-mass binary, the helical
+symmetry of a circular orbit, and the connection against finite differences
+of the metric (time derivatives included).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from coolharm3d_amd import _lib
+from coolharm3d_amd.pn import PnMetric, bbh_block, CENT, FACE1
+
+
+def one_zone(X1, X2, X3, **kw):
+    c = bbh_block(1, 1, 1, ghosts=0, **kw)
+    for l, x in ((1, X1), (2, X2), (3, X3)):
+        c.startx[l] = x - 0.5 * c.dx[l]
+    return c
+
+
+def geom(X, t=0.0, hostcheck=True, **kw):
+    return PnMetric(one_zone(*X, **kw), hostcheck=hostcheck).geometry(t, CENT).reshape(42)
+
+
+def gcov(g42):
+    return g42[16:32].reshape(4, 4)
+
+
+def gcon(g42):
+    return g42[0:16].reshape(4, 4)
+
+
+POINTS = [(math.log(30.0), 0.5, 0.1), (math.log(45.0), 0.27, 2.2), (math.log(200.0), 0.81, 4.0),
+          (math.log(259.0), 0.05, 6.0)]
+
+
+def test_inverse_determinant_lapse():
+    for X in POINTS:
+        for t in (0.0, 250.0):
+            g = geom(X, t)
+            a, b = gcov(g), gcon(g)
+            assert np.allclose(a @ b, np.eye(4), atol=1e-13 * np.abs(a).max() * np.abs(b).max())
+            assert np.isclose(g[32], math.sqrt(-np.linalg.det(a)), rtol=1e-12)
+            assert np.isclose(g[34], 1.0 / math.sqrt(-b[0, 0]), rtol=1e-15)
+            assert np.allclose(g[38:42], [1.0 / g[34], -g[34] * b[0, 1], -g[34] * b[0, 2], -g[34] * b[0, 3]],
+                               rtol=1e-14, atol=1e-300)
+
+
+def test_single_hole_is_harmonic_schwarzschild():
+    """m2 = 0: the 1PN truncation of Schwarzschild in harmonic coordinates,
+    g00 = -1 + 2m/r - 2m^2/r^2, g_ij = (1 + 2m/r) delta_ij, no shift."""
+    for X in POINTS:
+        r, th = math.exp(X[0]), math.pi * X[1] + 0.3 * math.sin(2 * math.pi * X[1])
+        dth = math.pi + 2 * math.pi * 0.3 * math.cos(2 * math.pi * X[1])
+        g = gcov(geom(X, 0.0, m1=1.0, m2=0.0))
+        f = 1 + 2 / r
+        expect = np.diag([-1 + 2 / r - 2 / r ** 2, f * r * r, f * r * r * dth * dth, f * (r * math.sin(th)) ** 2])
+        assert np.allclose(g, expect, rtol=1e-13, atol=1e-13 * r * r)
+
+
+def test_flat_far_field():
+    X = (math.log(1e7), 0.4, 1.0)
+    r = 1e7
+    g = gcov(geom(X))
+    flat = gcov(geom(X, m1=0.0, m2=1e-300))
+    scale = np.sqrt(np.abs(np.outer(np.diag(flat), np.diag(flat))))
+    assert np.all(np.abs(g - flat) <= 3.0 / r * scale)
+    assert np.isclose(flat[0, 0], -1.0, rtol=1e-12)
+
+
+def test_equal_mass_pi_rotation():
+    """Rotating an equal-mass binary by pi about its axis maps it onto itself."""
+    for X in POINTS:
+        for t in (0.0, 123.0):
+            a = gcov(geom(X, t))
+            b = gcov(geom((X[0], X[1], X[2] + math.pi), t))
+            assert np.allclose(a, b, rtol=1e-12, atol=1e-13 * np.abs(a).max())
+
+
+def test_circular_orbit_helical_symmetry():
+    """Fixed circular orbit: g(t + s, phi + omega s) = g(t, phi)."""
+    omega = math.sqrt(1.0 / 20.0 ** 3)
+    for X in POINTS[:2]:
+        a = gcov(geom(X, 10.0, inspiral=False))
+        s = 37.0
+        b = gcov(geom((X[0], X[1], X[2] + omega * s), 10.0 + s, inspiral=False))
+        assert np.allclose(a, b, rtol=1e-11, atol=1e-12 * np.abs(a).max())
+
+
+@pytest.mark.parametrize("inspiral", [True, False])
+def test_connection_against_finite_differences(inspiral):
+    X, t = (math.log(40.0), 0.37, 1.3), 50.0
+    conn = PnMetric(one_zone(*X, inspiral=inspiral), hostcheck=True).connection(t).reshape(4, 4, 4)
+    h = [1e-2, 1e-5, 1e-5, 1e-5]
+
+    def g_at(dt=0.0, dX=(0.0, 0.0, 0.0)):
+        return gcov(geom((X[0] + dX[0], X[1] + dX[1], X[2] + dX[2]), t + dt, inspiral=inspiral))
+
+    dg = np.empty((4, 4, 4))   # dg[c][a][b] = d_c g_ab
+    dg[0] = (g_at(h[0]) - g_at(-h[0])) / (2 * h[0])
+    for c in range(1, 4):
+        e = [0.0, 0.0, 0.0]
+        e[c - 1] = h[c]
+        m = [-x for x in e]
+        dg[c] = (g_at(0.0, e) - g_at(0.0, m)) / (2 * h[c])
+    gi = np.linalg.inv(g_at())
+    # low[c][a][b] = Gamma_{cab} = (d_a g_cb + d_b g_ca - d_c g_ab) / 2
+    low = np.empty((4, 4, 4))
+    for c in range(4):
+        for a in range(4):
+            for b in range(4):
+                low[c, a, b] = 0.5 * (dg[a, c, b] + dg[b, c, a] - dg[c, a, b])
+    fd = np.einsum("dc,cab->dab", gi, low)
+    scale = np.abs(fd).max()
+    assert np.allclose(conn, fd, rtol=1e-5, atol=1e-7 * scale)
+    assert np.allclose(conn, conn.transpose(0, 2, 1), rtol=0, atol=1e-15 * scale)
+    # the time derivative enters: Gamma^t_tt differs between a moving and a frozen metric
+    if inspiral:
+        assert abs(conn[0, 0, 0]) > 0
+
+
+def test_entry_point_errors():
+    lib = _lib.load(hostcheck=True)
+    c = one_zone(*POINTS[0])
+    c.b0 = 0.0
+    out = np.zeros(42)
+    rc = lib.hgpu_pn_geometry(None, c, 0.0, CENT, out.ctypes.data)
+    assert rc < 0 and b"separation" in lib.hgpu_last_error()
+    c = one_zone(*POINTS[0])
+    rc = lib.hgpu_pn_geometry(None, c, 0.0, 7, out.ctypes.data)
+    assert rc < 0 and b"pos" in lib.hgpu_last_error()
+
+
+@pytest.mark.gpu
+def test_gpu_metric_matches_host():
+    c = bbh_block(30, 16, 40)
+    G, H = PnMetric(c), PnMetric(c, hostcheck=True)
+    for pos in (FACE1, CENT):
+        a, b = G.geometry(75.0, pos), H.geometry(75.0, pos)
+        assert np.allclose(a, b, rtol=1e-13, atol=1e-13 * np.abs(b).max(axis=-1, keepdims=True))
+    a, b = G.connection(75.0), H.connection(75.0)
+    assert np.allclose(a, b, rtol=1e-12, atol=1e-13 * np.abs(b).max())
