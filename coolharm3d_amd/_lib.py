@@ -33,6 +33,8 @@ class HgpuConfig(ctypes.Structure):
         ("rhomin", ctypes.c_double), ("uumin", ctypes.c_double),
         ("rhopower", ctypes.c_double), ("uupower", ctypes.c_double),
         ("precision", ctypes.c_int),
+        ("bbh_m1", ctypes.c_double), ("bbh_m2", ctypes.c_double), ("bbh_sep", ctypes.c_double),
+        ("bbh_phi0", ctypes.c_double), ("bbh_inspiral", ctypes.c_int),
     ]
 
 

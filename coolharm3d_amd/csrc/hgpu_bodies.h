@@ -404,20 +404,22 @@ HD double floor_uu(const Params& P, const Ptrs& X, int i, long long c) {
 // recover_primitives() tree in a second, compacted pass (recover_zone()).
 
 // U^{n+1} of the zone (src/step_ch.c:330-375)
+// gn: geometry of U^n (time t); g: geometry of the stage (source terms).  With a
+// static metric both are the zone's one record.
 template <bool MIXED>
 HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int i, int j, int k,
-                       const Geom* g, FailRec& fr, double* U) {
+                       const Geom* gn, const Geom* g, FailRec& fr, double* U) {
   const long long NT = P.nt;
   const long long c = cidx(P, i, j, k);
   {
-    // U^n = primtoflux(p_i) (src/step_ch.c:339-341)
+    // U^n = primtoflux(p_i) with geom_i (src/step_ch.c:319, :339-341)
     double pi[NP];
 #pragma unroll
     for (int v = 0; v < NP; v++) pi[v] = rt_val(P, X, v, i, X.p[v*NT + c]);
     State q;
-    int gf = get_state(pi, g, P.gam, &q);
+    int gf = get_state(pi, gn, P.gam, &q);
     if (stage == 0 && gf) fr.nf[NFAIL_GAMMA_CALC]++;
-    primtoflux(pi, &q, 0, g, U);
+    primtoflux(pi, &q, 0, gn, U);
   }
   phase_fence();
   double dU[NP];
@@ -525,13 +527,13 @@ HD bool update_zone_fast(const Params& P, const Ptrs& X, int stage, double dt, i
                          FailRec& fr, double* U) {
   const long long c = cidx(P, i, j, k);
   const double Dt = (stage == 0) ? 0.5*dt : dt;
-  const Geom* g = geom_at(P, X, i, j, k, CENT);
-  zone_conserved<MIXED>(P, X, stage, Dt, i, j, k, g, fr, U);
+  const int gid = geom_id(P, i, j, k, CENT);
+  zone_conserved<MIXED>(P, X, stage, Dt, i, j, k, X.geom_n + gid, X.geom + gid, fr, U);
   phase_fence();
   double pf[NP];
   zone_guess(P, X, stage, i, c, pf);
   double pgam = 1.;
-  if (!recover_fast(P, X, i, j, k, pf, U, g, floor_rho(P, X, i, c), floor_uu(P, X, i, c), &pgam, fr))
+  if (!recover_fast(P, X, i, j, k, pf, U, X.geom_new + gid, floor_rho(P, X, i, c), floor_uu(P, X, i, c), &pgam, fr))
     return false;
   store_zone(P, X, stage, c, pf, pgam, fr);
   return true;
@@ -541,8 +543,11 @@ template <bool MIXED>
 HD void update_zone(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
                     FailRec& fr) {
   double U[NP];
-  if (!update_zone_fast<MIXED>(P, X, stage, dt, i, j, k, fr, U))
-    recover_zone<MIXED>(P, X, stage, dt, i, j, k, U, fr);
+  if (!update_zone_fast<MIXED>(P, X, stage, dt, i, j, k, fr, U)) {
+    Ptrs Xn = X;
+    Xn.geom = X.geom_new;   // the full tree works at the result's time level
+    recover_zone<MIXED>(P, Xn, stage, dt, i, j, k, U, fr);
+  }
 }
 
 // --------------------------------------------------------------------------

@@ -319,7 +319,9 @@ HD int c2p_begin(const double* U, const Geom* geom, double gam, double gmog, con
     mlp_forward(mlp, f, y);
     const double lg = fmax((double)y[1], 0.);
     const double vsq = -expm1(-2.*lg);     // 1 - 1/gamma^2
-    u2d_guess_w(c.s, c.s.D*exp((double)y[0]), x0, vsq);
+    // a learned W close to the bracket's lower bound is nudged up by 2 % steps
+    // rather than tenfold, so a guess within a few per cent stays that close
+    u2d_guess_w(c.s, c.s.D*exp((double)y[0]), x0, vsq, 1.02, 200);
     c.has_fallback = (rc_plain == 0);
   } else {
     if (rc_plain) return rc_plain;

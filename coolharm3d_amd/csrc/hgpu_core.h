@@ -5,6 +5,7 @@
 #include "../../include/hgpu.h"
 #include "hgpu_bodies.h"
 #include "hgpu_geom.h"
+#include "hgpu_pn.h"
 #include <string>
 #include <vector>
 #include <map>
@@ -22,10 +23,39 @@ inline int set_error(int code, const std::string& msg) {
 
 }  // namespace hgpu
 
+namespace hgpu {
+// Time-dependent metric: the binary and the numerical coordinates of zone
+// (0,0,0)'s lower corner (startx).
+struct PnGrid {
+  PnBinary B;
+  double x0[4];
+};
+// Numerical coordinates of zone (i,j,k) at position pos (coord(), src/coord.c:182-228).
+HD void zone_X(const Params& P, const PnGrid& G, int i, int j, int k, int pos, double* X) {
+  const double h1 = (pos == FACE1 || pos == CORN) ? 0. : 0.5;
+  const double h2 = (pos == FACE2 || pos == CORN) ? 0. : 0.5;
+  const double h3 = (pos == FACE3 || pos == CORN) ? 0. : 0.5;
+  X[1] = G.x0[1] + (i + h1) * P.dx[1];
+  X[2] = G.x0[2] + (j + h2) * P.dx[2];
+  X[3] = G.x0[3] + (k + h3) * P.dx[3];
+}
+// Time of level tsel of the step: 0 t, 1 t + dt/2, 2 t + dt.
+HD double level_time(const Ptrs& X, int tsel) {
+  const double t = X.scal[SC_T], dt = X.scal[SC_DT];
+  return (tsel == 0) ? t : ((tsel == 1) ? t + 0.5*dt : t + dt);
+}
+}  // namespace hgpu
+
 struct hgpu_ctx {
   hgpu_config cfg;
   hgpu::Params P;
   hgpu::Ptrs X;
+  // time-dependent metric: geometry levels at t, t + dt/2, t + dt, their FP32
+  // copies (t, t + dt/2) and connections (t, t + dt/2); level 0 is X.geom's buffer
+  hgpu::Geom* lev[3] = {nullptr, nullptr, nullptr};
+  hgpu::GeomF* levf[2] = {nullptr, nullptr};
+  double* connlev[2] = {nullptr, nullptr};
+  hgpu::PnGrid pn;
   long long n_geom = 0, n_conn = 0;
   int n1_conn = 0;
   std::vector<double> stage;       // host staging buffer
@@ -40,22 +70,27 @@ namespace hgpu {
 // src/decomp.c:20-130, and global_init(), src/main.c:283-300).
 inline int make_params(const hgpu_config& c, Params& P) {
   if (c.n1 < 1 || c.n2 < 1 || c.n3 < 1) return set_error(-2, "grid sizes must be >= 1");
-  if (c.metric != 0 && c.metric != 2) return set_error(-2, "metric must be 0 (Minkowski Cartesian) or 2 (Kerr-Schild)");
+  if (c.metric != 0 && c.metric != 2 && c.metric != 3)
+    return set_error(-2, "metric must be 0 (Minkowski Cartesian), 2 (Kerr-Schild) or 3 (binary black hole)");
+  if (c.metric == 3 && c.coords != 1) return set_error(-2, "the binary-black-hole metric needs coords = 1 (spherical)");
+  if (c.metric == 3 && !(c.bbh_m1 >= 0. && c.bbh_m2 >= 0. && c.bbh_m1 + c.bbh_m2 > 0. && c.bbh_sep > 0.))
+    return set_error(-2, "binary: masses >= 0 with a positive sum and a positive separation required");
   if (c.coords != 0 && c.coords != 1) return set_error(-2, "coords must be 0 (identity) or 1 (modified Kerr-Schild)");
   if (c.recon != RECON_MC && c.recon != RECON_PPM_MC) return set_error(-2, "recon must be 2 (MC) or 5 (PPM+MC)");
   if (c.flux != FLUX_LF && c.flux != FLUX_HLLE) return set_error(-2, "flux must be 0 (LF) or 1 (HLLE)");
   if (c.bc != BC_CARTESIAN_OUTFLOW && c.bc != BC_SPHERICAL_OUTFLOW3) return set_error(-2, "bc must be 0 or 10");
   if (c.ct_para && c.recon != RECON_PPM_MC)
     return set_error(-2, "PPM EMFs (ct_para) use the PPM limiter of the reconstruction: recon must be 5");
-  if (c.cooling && c.metric != 2) return set_error(-2, "the h/r cooling function needs the Kerr-Schild metric");
+  if (c.cooling && c.metric == 0) return set_error(-2, "the h/r cooling function needs a black-hole metric");
   if (c.bc == BC_SPHERICAL_OUTFLOW3 && (c.n2 < NG || c.n3 < NG))
     return set_error(-2, "reflecting/periodic boundaries need at least NG interior zones in x2 and x3");
   std::memset(&P, 0, sizeof(P));
   P.n1 = c.n1; P.n2 = c.n2; P.n3 = c.n3;
   P.n1t = c.n1 + 2*NG; P.n2t = c.n2 + 2*NG; P.n3t = c.n3 + 2*NG;
   P.nt = (long long)P.n1t * P.n2t * P.n3t;
-  P.metric_dim = (c.metric == 2) ? 2 : 0;
+  P.metric_dim = (c.metric == 3) ? 3 : ((c.metric == 2) ? 2 : 0);
   P.flat = (c.metric == 0);
+  P.dyn = (c.metric == 3);
   P.recon = c.recon;
   P.flux = c.flux;
   P.ct_para = c.ct_para;
@@ -78,12 +113,16 @@ inline int make_params(const hgpu_config& c, Params& P) {
   return 0;
 }
 
+// Coordinate parameters of the host tables.  The binary metric uses the
+// spherical map with a non-spinning total mass for its per-x1 tables (cooling
+// targets, rescaling, floors); its geometry itself is filled on the device.
 inline CoordParams coord_params(const hgpu_config& c) {
   CoordParams C;
-  C.metric = c.metric;
+  C.metric = (c.metric == 3) ? 2 : c.metric;
   C.coords = c.coords;
   for (int l = 0; l < 4; l++) { C.startx[l] = c.startx[l]; C.dx[l] = c.dx[l]; }
-  C.a = c.a; C.asq = c.a*c.a; C.M = 1.; C.R0 = c.R0;
+  C.a = (c.metric == 3) ? 0. : c.a;
+  C.asq = C.a*C.a; C.M = 1.; C.R0 = c.R0;
   C.h_slope = c.h_slope; C.th_beg = c.th_beg; C.th_length = c.th_length;
   return C;
 }
@@ -95,8 +134,10 @@ struct HostTables {
 
 inline void build_tables(const hgpu_config& c, const Params& P, long long n_geom, long long n_conn, HostTables& T) {
   CoordParams C = coord_params(c);
-  T.geom.assign(n_geom * GEOM_DOUBLES, 0.);
-  if (P.metric_dim == 2) {
+  T.geom.assign(P.dyn ? 0 : n_geom * GEOM_DOUBLES, 0.);
+  if (P.dyn) {
+    // filled on the device at every step
+  } else if (P.metric_dim == 2) {
     for (int pos = 0; pos < NPOS; pos++)
       for (int i = 0; i < P.n1t; i++)
         for (int j = 0; j < P.n2t; j++)
@@ -104,7 +145,7 @@ inline void build_tables(const hgpu_config& c, const Params& P, long long n_geom
   } else {
     geom_record(C, NG, NG, NG, CENT, reinterpret_cast<Geom*>(&T.geom[0]));
   }
-  T.conn.assign(n_conn * 64, 0.);
+  T.conn.assign(P.dyn ? 0 : n_conn * 64, 0.);
   if (P.metric_dim == 2 && !P.flat) {
     for (int i = NG; i < NG + P.n1; i++)
       for (int j = NG; j < NG + P.n2; j++)
@@ -195,8 +236,15 @@ struct Api {
     int rc = make_params(*cfg, c->P);
     if (rc) { delete c; return rc; }
     const Params& P = c->P;
-    c->n_geom = (P.metric_dim == 2) ? (long long)NPOS * P.n1t * P.n2t : 1;
-    c->n_conn = (P.metric_dim == 2) ? (long long)P.n1 * P.n2 : 1;
+    c->n_geom = (P.metric_dim == 3) ? (long long)NPOS * P.nt : ((P.metric_dim == 2) ? (long long)NPOS * P.n1t * P.n2t : 1);
+    c->n_conn = (P.metric_dim == 3) ? (long long)P.n1 * P.n2 * P.n3 : ((P.metric_dim == 2) ? (long long)P.n1 * P.n2 : 1);
+    if (P.dyn) {
+      PnBinary& B = c->pn.B;
+      B.m1 = cfg->bbh_m1; B.m2 = cfg->bbh_m2; B.b0 = cfg->bbh_sep; B.phi0 = cfg->bbh_phi0;
+      B.inspiral = cfg->bbh_inspiral;
+      B.R0 = cfg->R0; B.th_beg = cfg->th_beg; B.th_length = cfg->th_length; B.h_slope = cfg->h_slope;
+      for (int l = 0; l < 4; l++) c->pn.x0[l] = cfg->startx[l];
+    }
     rc = BE::init(c);
     if (rc) { delete c; return rc; }
     HostTables T;
@@ -211,9 +259,18 @@ struct Api {
     rc |= BE::alloc(c, (void**)&X.emf, sizeof(double) * nE);
     rc |= BE::alloc(c, (void**)&X.Ff, sizeof(float) * (P.mixed ? 3 * NFLUX * NT : 2));
     rc |= BE::alloc(c, (void**)&X.emff, sizeof(float) * (P.mixed ? 3 * NT : 2));
-    rc |= BE::alloc(c, (void**)&X.geomf, sizeof(GeomF) * (P.mixed ? c->n_geom : 1));
-    rc |= BE::alloc(c, (void**)&X.geom, sizeof(double) * GEOM_DOUBLES * c->n_geom);
-    rc |= BE::alloc(c, (void**)&X.conn, sizeof(double) * 64 * c->n_conn);
+    rc |= BE::alloc(c, (void**)&c->levf[0], sizeof(GeomF) * (P.mixed ? c->n_geom : 1));
+    rc |= BE::alloc(c, (void**)&c->lev[0], sizeof(double) * GEOM_DOUBLES * c->n_geom);
+    rc |= BE::alloc(c, (void**)&c->connlev[0], sizeof(double) * 64 * c->n_conn);
+    X.geom = c->lev[0];
+    X.geomf = c->levf[0];
+    X.conn = c->connlev[0];
+    if (P.dyn) {
+      for (int l = 1; l < 3; l++) rc |= BE::alloc(c, (void**)&c->lev[l], sizeof(double) * GEOM_DOUBLES * c->n_geom);
+      rc |= BE::alloc(c, (void**)&c->levf[1], sizeof(GeomF) * (P.mixed ? c->n_geom : 1));
+      rc |= BE::alloc(c, (void**)&c->connlev[1], sizeof(double) * 64 * c->n_conn);
+    }
+    X.geom_n = X.geom_new = X.geom;
     rc |= BE::alloc(c, (void**)&X.omega, sizeof(double) * P.n1t);
     rc |= BE::alloc(c, (void**)&X.tr, sizeof(double) * P.n1t);
     rc |= BE::alloc(c, (void**)&X.rscale, sizeof(double) * 6 * P.n1t);
@@ -244,7 +301,7 @@ struct Api {
     if (P.mixed) {
       rc |= BE::zero(c, X.Ff, sizeof(float) * 3 * NFLUX * NT);
       rc |= BE::zero(c, X.emff, sizeof(float) * 3 * NT);
-      rc |= upload_geomf(c, T.geom.data());
+      if (!P.dyn) rc |= upload_geomf(c, T.geom.data());
     }
     rc |= BE::zero(c, X.gamma, sizeof(double) * NT);
     rc |= BE::zero(c, X.pflag, sizeof(int) * NT);
@@ -254,6 +311,7 @@ struct Api {
     sc[SC_T] = cfg->startx[0];
     sc[SC_DT] = cfg->dx[0];
     rc |= BE::h2d(c, X.scal, sc, sizeof(sc));
+    if (P.dyn) rc |= set_levels_at_state(c);
     rc |= BE::sync(c);
     if (rc) { destroy(c); return set_error(-3, "device initialisation failed: " + g_last_error); }
     *out = c;
@@ -263,9 +321,12 @@ struct Api {
   static int destroy(hgpu_ctx* c) {
     if (!c) return 0;
     Ptrs& X = c->X;
-    void* ptrs[] = {X.p, X.ph, X.F, X.emf, (void*)X.geom, (void*)X.conn, (void*)X.omega, (void*)X.tr,
+    // X.geom, X.geomf, X.conn point into the level buffers (level 0 is the
+    // buffer allocated for them), so the levels are what is released
+    void* ptrs[] = {X.p, X.ph, X.F, X.emf, (void*)X.omega, (void*)X.tr,
                     (void*)X.rscale, (void*)X.irscale, (void*)X.rhoflr, (void*)X.uuflr, X.gamma,
-                    X.pflag, X.nfail, X.status, X.scal, X.Ff, X.emff, (void*)X.geomf, X.Uslow, X.slow_idx};
+                    X.pflag, X.nfail, X.status, X.scal, X.Ff, X.emff, X.Uslow, X.slow_idx,
+                    c->lev[0], c->lev[1], c->lev[2], c->levf[0], c->levf[1], c->connlev[0], c->connlev[1]};
     for (void* p : ptrs) if (p) BE::release(c, p);
     BE::fini(c);
     delete c;
@@ -328,15 +389,36 @@ struct Api {
 
   // One advance() stage plus its boundaries and fixups (src/step_ch.c:92-121,
   // :159-180).
+  // Time-dependent metric: the state p (time t of the coming step) is read with
+  // level 0; fixup_bounds() and the diagnostics use it.
+  static int set_levels_at_state(hgpu_ctx* c) {
+    Ptrs& X = c->X;
+    int rc = BE::fill_geometry(c, c->lev[0], c->levf[0], 0, 0);
+    X.geom = X.geom_n = X.geom_new = c->lev[0];
+    X.geomf = c->levf[0];
+    X.conn = c->connlev[0];
+    return rc;
+  }
+
   static int substep(hgpu_ctx* c, int stage) {
     Ptrs& X = c->X;
     double* src = (stage == 0) ? X.p : X.ph;
     double* dst = (stage == 0) ? X.ph : X.p;
     int rc = 0;
+    if (c->P.dyn) {
+      // fluxes and sources at the stage's time, U^n at t, the result at the
+      // next level (ncurr = n_beg / n_mid / n_end, src/step_ch.c:74-162, :319-331)
+      X.geom = c->lev[stage];
+      X.geomf = c->levf[stage];
+      X.conn = c->connlev[stage];
+      X.geom_n = c->lev[0];
+      X.geom_new = c->lev[stage + 1];
+    }
     rc |= BE::reset_vmax(c);
     for (int d = 0; d < 3; d++) rc |= BE::flux(c, src, d);
     rc |= BE::emf(c);
     rc |= BE::update(c, stage);
+    if (c->P.dyn) X.geom = X.geom_new;
     rc |= BE::recover_queued(c, stage);
     rc |= bounds(c, dst, 0);
     rc |= BE::fixup_interp(c, dst, PFLAG_INTERP_V);
@@ -346,14 +428,42 @@ struct Api {
     return rc;
   }
 
+  // Geometry and connection of a step's three time levels for the
+  // time-dependent metric (set_general_geometry()/set_general_conn(),
+  // src/metric.c:3390-3647): levels t and t + dt/2 at all positions with their
+  // connections, t + dt at zone centres.
+  static int fill_levels(hgpu_ctx* c) {
+    int rc = 0;
+    rc |= BE::fill_geometry(c, c->lev[0], c->levf[0], 0, 0);
+    rc |= BE::fill_conn(c, c->connlev[0], 0);
+    rc |= BE::fill_geometry(c, c->lev[1], c->levf[1], 1, 0);
+    rc |= BE::fill_conn(c, c->connlev[1], 1);
+    rc |= BE::fill_geometry(c, c->lev[2], nullptr, 2, 1);
+    return rc;
+  }
+
+  // One advance() stage through the public entry point.
+  static int substep_api(hgpu_ctx* c, int stage) {
+    int rc = 0;
+    if (c->P.dyn && stage == 0) rc |= fill_levels(c);
+    rc |= substep(c, stage);
+    if (c->P.dyn && stage == 1) c->X.geom = c->X.geom_n = c->X.geom_new = c->lev[2];
+    return rc;
+  }
+
   // One step_ch(): predictor, corrector, new time step (src/step_ch.c:50-234).
   static int one_step(hgpu_ctx* c) {
     int rc = 0;
+    if (c->P.dyn) rc |= fill_levels(c);
     rc |= BE::clear_failures(c);      // src/step_ch.c:65-69
     rc |= substep(c, 0);
     rc |= BE::clear_failures(c);      // src/step_ch.c:117-121
     rc |= substep(c, 1);
     rc |= BE::timestep(c);            // src/step_ch.c:218-223
+    if (c->P.dyn) {
+      // the state is now at t + dt: level 2 (centres) describes it
+      c->X.geom = c->X.geom_n = c->X.geom_new = c->lev[2];
+    }
     return rc;
   }
 
@@ -371,7 +481,9 @@ struct Api {
   }
 
   static int fixup_bounds(hgpu_ctx* c) {
-    int rc = BE::fixup_init(c);
+    int rc = 0;
+    if (c->P.dyn) rc |= set_levels_at_state(c);
+    rc |= BE::fixup_init(c);
     rc |= bounds(c, c->X.p, 0);
     rc |= BE::sync(c);
     return rc;

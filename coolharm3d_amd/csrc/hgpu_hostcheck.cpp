@@ -179,6 +179,33 @@ struct HostBE {
     *out = m;
     return 0;
   }
+  static int fill_geometry(hgpu_ctx* c, Geom* dst, GeomF* dstf, int tsel, int cent_only) {
+    const Params& P = c->P;
+    const double t = level_time(c->X, tsel);
+    for (int pos = cent_only ? CENT : 0; pos < (cent_only ? CENT + 1 : NPOS); pos++)
+      for (int i = 0; i < P.n1t; i++)
+        for (int j = 0; j < P.n2t; j++)
+          for (int k = 0; k < P.n3t; k++) {
+            double Xc[4];
+            zone_X(P, c->pn, i, j, k, pos, Xc);
+            const int id = geom_id(P, i, j, k, pos);
+            pn_geom(c->pn.B, t, Xc[1], Xc[2], Xc[3], &dst[id]);
+            if (P.mixed && dstf) geom_f32(dst[id], dstf[id]);
+          }
+    return 0;
+  }
+  static int fill_conn(hgpu_ctx* c, double* dst, int tsel) {
+    const Params& P = c->P;
+    const double t = level_time(c->X, tsel);
+    for (int i = NG; i < NG + P.n1; i++)
+      for (int j = NG; j < NG + P.n2; j++)
+        for (int k = NG; k < NG + P.n3; k++) {
+          double Xc[4];
+          zone_X(P, c->pn, i, j, k, CENT, Xc);
+          pn_conn(c->pn.B, t, Xc[1], Xc[2], Xc[3], dst + (long long)conn_id(P, i, j, k) * 64);
+        }
+    return 0;
+  }
   static int con2prim_batch(hgpu_ctx* c, long long n, const double* U, const double* geom, double* prim,
                             double* gamma, int* code, int* iters) {
     const Geom* G = reinterpret_cast<const Geom*>(geom);

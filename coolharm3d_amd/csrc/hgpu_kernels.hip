@@ -223,7 +223,8 @@ __global__ __launch_bounds__(BLOCK) void k_conserved(Params P, Ptrs X, int stage
     const double Dt = (stage == 0) ? 0.5*dt : dt;
     c = cidx(P, i, j, k);
     double U[NP];
-    zone_conserved<MIXED>(P, X, stage, Dt, i, j, k, geom_at(P, X, i, j, k, CENT), fr, U);
+    const int gid = geom_id(P, i, j, k, CENT);
+    zone_conserved<MIXED>(P, X, stage, Dt, i, j, k, X.geom_n + gid, X.geom + gid, fr, U);
 #pragma unroll
     for (int l = 0; l < NP; l++) X.Uslow[l*P.nt + c] = U[l];
   }
@@ -373,6 +374,38 @@ __global__ __launch_bounds__(BLOCK) void k_divb(Params P, Ptrs X, Box b, unsigne
   b.at(t, i, j, k);
   double v = divb_corner(P, X, X.p, i, j, k);
   if (v == v) atomicMax(out, dbl_bits(v));
+}
+
+// Time-dependent metric: geometry records of positions [pos0, pos0 + npos) of
+// every zone at time level tsel, recomputed per zone (and their FP32 copies).
+__global__ __launch_bounds__(BLOCK) void k_fill_geom(Params P, Ptrs X, PnGrid G, int tsel, int pos0, int npos,
+                                                    Geom* dst, GeomF* dstf) {
+  const long long z = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (z >= (long long)npos * P.nt) return;
+  const int pos = pos0 + (int)(z / P.nt);
+  const long long cell = z - (long long)(pos - pos0) * P.nt;
+  const int k = (int)(cell % P.n3t);
+  const int j = (int)((cell / P.n3t) % P.n2t);
+  const int i = (int)(cell / ((long long)P.n3t * P.n2t));
+  double Xc[4];
+  zone_X(P, G, i, j, k, pos, Xc);
+  Geom g;
+  pn_geom(G.B, level_time(X, tsel), Xc[1], Xc[2], Xc[3], &g);
+  const int id = geom_id(P, i, j, k, pos);
+  dst[id] = g;
+  if (dstf) geom_f32(g, dstf[id]);
+}
+
+// Connection at the interior zone centres at time level tsel.
+__global__ __launch_bounds__(BLOCK) void k_fill_conn(Params P, Ptrs X, PnGrid G, int tsel, double* dst) {
+  const long long z = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (z >= (long long)P.n1 * P.n2 * P.n3) return;
+  const int k = NG + (int)(z % P.n3);
+  const int j = NG + (int)((z / P.n3) % P.n2);
+  const int i = NG + (int)(z / ((long long)P.n3 * P.n2));
+  double Xc[4];
+  zone_X(P, G, i, j, k, CENT, Xc);
+  pn_conn(G.B, level_time(X, tsel), Xc[1], Xc[2], Xc[3], dst + (long long)conn_id(P, i, j, k) * 64);
 }
 
 // Batched Utoprim_2d_fast() on independent zones (SoA buffers).
@@ -699,7 +732,9 @@ struct GpuBE {
     hipStream_t st = priv(c)->stream;
     if (c->P.mixed) hipLaunchKernelGGL(k_conserved<true>, dim3(T.nblocks), dim3(T.bs), 0, st, c->P, c->X, stage, b, T);
     else hipLaunchKernelGGL(k_conserved<false>, dim3(T.nblocks), dim3(T.bs), 0, st, c->P, c->X, stage, b, T);
-    hipLaunchKernelGGL(k_invert, dim3(T2.nblocks), dim3(T2.bs), 0, st, c->P, c->X, stage, b, T2);
+    Ptrs Xn = c->X;
+    Xn.geom = c->X.geom_new;   // inversion at the result's time level
+    hipLaunchKernelGGL(k_invert, dim3(T2.nblocks), dim3(T2.bs), 0, st, c->P, Xn, stage, b, T2);
     return hip_check(hipGetLastError(), "k_update");
   }
   static int recover_queued(hgpu_ctx* c, int stage) {
@@ -754,6 +789,20 @@ struct GpuBE {
     release(c, d);
     *out = bits_dbl(h);
     return rc;
+  }
+  static int fill_geometry(hgpu_ctx* c, Geom* dst, GeomF* dstf, int tsel, int cent_only) {
+    Scope s(c, "metric");
+    const int pos0 = cent_only ? CENT : 0, npos = cent_only ? 1 : NPOS;
+    const long long n = (long long)npos * c->P.nt;
+    hipLaunchKernelGGL(k_fill_geom, dim3(nblocks(n)), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, c->pn, tsel, pos0,
+                       npos, dst, c->P.mixed ? dstf : nullptr);
+    return hip_check(hipGetLastError(), "k_fill_geom");
+  }
+  static int fill_conn(hgpu_ctx* c, double* dst, int tsel) {
+    Scope s(c, "metric");
+    const long long n = (long long)c->P.n1 * c->P.n2 * c->P.n3;
+    hipLaunchKernelGGL(k_fill_conn, dim3(nblocks(n)), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, c->pn, tsel, dst);
+    return hip_check(hipGetLastError(), "k_fill_conn");
   }
   static int con2prim_batch(hgpu_ctx* c, long long n, const double* U, const double* geom, double* prim,
                             double* gamma, int* code, int* iters) {

@@ -381,14 +381,16 @@ HD int u2d_prepare(const double* U, const Geom* geom, double gmog, double* prim,
 }
 
 // Initial (W, v^2) from an estimate W_last (src/utoprim_2d_fast.c:313-330):
-// W is raised tenfold until it lies above the physical root's lower bound.
-// vsq_guess >= 0 replaces v^2(W) when no increase was needed.
-HD void u2d_guess_w(const U2d& s, double W_last, double* x2d, double vsq_guess = -1.) {
+// W is raised by `factor` (the reference: tenfold, at most 10 times) until it
+// lies above the physical root's lower bound.  vsq_guess >= 0 replaces v^2(W)
+// when no increase was needed.
+HD void u2d_guess_w(const U2d& s, double W_last, double* x2d, double vsq_guess = -1., double factor = 10.,
+                    int max_increase = 10) {
   int i_increase = 0;
   while (((W_last*W_last*W_last * (W_last + 2.*s.Bsq)
            - s.QdotBsq*(2.*W_last + s.Bsq)) <= W_last*W_last*(s.Qtsq - s.Bsq*s.Bsq))
-         && (i_increase < 10)) {
-    W_last *= 10.;
+         && (i_increase < max_increase)) {
+    W_last *= factor;
     i_increase++;
   }
   x2d[0] = fabs(W_last);

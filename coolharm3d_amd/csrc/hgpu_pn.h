@@ -81,6 +81,19 @@ HD Dual4 pow(const Dual4& a, double p) { const double f = ::pow(a.v, p); return 
 using ad::Dual4;
 using ad::dc;
 using ad::dvar;
+HD double value(double x) { return x; }
+HD double value(const Dual4& x) { return x.v; }
+HD void set_value(double& x, double v) { x = v; }
+HD void set_value(Dual4& x, double v) { x.v = v; }
+
+// seesaw(x, 0, 1) of the theta coordinate, src/coord.c:4316-4324: ghost zones
+// beyond a pole reflect back into [0, 1]
+HD double seesaw_unit(double x) {
+  const double x0 = 0., xn = 1., xn2 = 2.*xn - x0;
+  const int shift = (x < x0) ? (1 + (int)(fabs(x - x0)/(xn2 - x0))) : (-(int)(fabs(x - x0)/(xn2 - x0)));
+  const double x1 = x + (xn2 - x0)*shift;
+  return (x1 < xn) ? (x1 - x0) : (xn2 - x1);
+}
 
 // ---------------------------------------------------------------------------
 struct PnBinary {
@@ -160,10 +173,15 @@ template <class T>
 HD void pn_gcov_num(const PnBinary& B, const T& t, const T& X1, const T& X2, const T& X3, T g[4][4]) {
   const T er = exp(X1);
   const T r = er + B.R0;
-  const T th = B.th_length * X2 + B.th_beg + B.h_slope * sin(2. * M_PI * X2);
+  // theta from the reflected X2, its derivative from X2 itself, as x_of_xp()
+  // and dx_dxp_calc() form them (src/coord.c:240-382)
+  const T X2s = X2 + (seesaw_unit(value(X2)) - value(X2));
+  const T th = B.th_length * X2s + B.th_beg + B.h_slope * sin(2. * M_PI * X2s);
   const T dr = er;
   const T dth = B.th_length + 2. * M_PI * B.h_slope * cos(2. * M_PI * X2);
-  const T st = sin(th), ct = cos(th), sp = sin(X3), cp = cos(X3);
+  T st = sin(th);
+  const T ct = cos(th), sp = sin(X3), cp = cos(X3);
+  if (fabs(value(st)) < 1.E-16) set_value(st, value(st) >= 0. ? 1.E-16 : -1.E-16);   // trigvalues(), src/metric.c:29-40
   T x[3] = {r * st * cp, r * st * sp, r * ct};
   T gh[4][4];
   pn_gcov_harmonic(B, t, x, gh);

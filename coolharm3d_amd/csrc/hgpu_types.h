@@ -97,6 +97,7 @@ struct Params {
   int n_spatial_dims;
   int track_nfail;         // keep nfail[15][NT] per-cell counters
   int mixed;               // 1: FP32 reconstruction, face fluxes and EMFs
+  int dyn;                 // 1: time-dependent metric, geometry levels refilled every step
   double dx[4], invdx[4];
   double gam, gam_m1_o_gam;
   double cour;
@@ -110,7 +111,9 @@ struct Ptrs {
   double* ph;              // half-step state ph (SoA, 8*NT)
   double* F;               // face fluxes [3][NFLUX][NT]
   double* emf;             // corner EMFs [3][NT]
-  const Geom* geom;        // geometry records
+  const Geom* geom;        // geometry records (the level of the stage being computed)
+  const Geom* geom_n;      // level of U^n (time t of the step; = geom for a static metric)
+  const Geom* geom_new;    // level of the stage's result (t + dt/2 or t + dt; = geom when static)
   const GeomF* geomf;      // their FP32 copy (mixed precision)
   float* Ff;               // FP32 face fluxes [3][NFLUX][NT] (mixed precision)
   float* emff;             // FP32 corner EMFs [3][NT] (mixed precision)
@@ -129,6 +132,18 @@ struct Ptrs {
   unsigned long long* status;  // [ST_WORDS]
   double* scal;            // [SC_WORDS]
 };
+
+// FP32 copy of a geometry record (mixed precision); 1 - alpha is formed in FP64.
+HD void geom_f32(const Geom& g, GeomF& f) {
+  for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) {
+    f.gcon[a][b] = (float)g.gcon[a][b];
+    f.gcov[a][b] = (float)g.gcov[a][b];
+  }
+  f.g = (float)g.g; f.g_inv = (float)g.g_inv; f.alpha = (float)g.alpha;
+  for (int a = 0; a < 3; a++) f.beta[a] = (float)g.beta[a];
+  for (int a = 0; a < 4; a++) f.ncon[a] = (float)g.ncon[a];
+  f.om_alpha = (float)(1. - g.alpha);
+}
 
 HD long long cidx(const Params& P, int i, int j, int k) {
   return ((long long)i * P.n2t + j) * P.n3t + k;

@@ -45,8 +45,9 @@ def config_for(grid: GridSpec, **kw) -> _lib.HgpuConfig:
     Kerr-Schild/MKS -> PPM+MC, PPM EMFs, spherical outflow, h/r cooling."""
     c = _lib.HgpuConfig()
     c.n1, c.n2, c.n3 = grid.n1, grid.n2, grid.n3
-    ks = grid.metric == "kerr_schild"
-    c.metric = 2 if ks else 0
+    bbh = grid.metric == "pn_binary"
+    ks = grid.metric == "kerr_schild" or bbh
+    c.metric = 3 if bbh else (2 if ks else 0)
     c.coords = 1 if grid.coords == "mks" else 0
     c.recon = kw.get("recon", RECON_PPM_MC if ks else RECON_MC)
     c.flux = kw.get("flux", FLUX_HLLE)
@@ -63,6 +64,10 @@ def config_for(grid: GridSpec, **kw) -> _lib.HgpuConfig:
     c.a, c.R0, c.h_slope = grid.a, grid.R0, grid.h_slope
     c.th_beg, c.th_length = grid.th_beg, grid.th_length
     c.r_isco = kw.get("r_isco", risco_prograde(grid.a) if ks else 0.0)
+    if bbh:
+        e = grid.extra
+        c.bbh_m1, c.bbh_m2, c.bbh_sep = e["m1"], e["m2"], e["sep"]
+        c.bbh_phi0, c.bbh_inspiral = e.get("phi0", 0.0), 1 if e.get("inspiral", True) else 0
     c.h_o_r = kw.get("h_o_r", 0.1 if ks else 0.05)
     if ks:
         c.rhomin, c.uumin = kw.get("rhomin", 1.0e-7), kw.get("uumin", 1.0e-9)

@@ -26,7 +26,9 @@ extern "C" {
  * honours, as run-time values, and the grid parameters of init_base(). */
 typedef struct hgpu_config {
   int n1, n2, n3;          /* interior zones N1, N2, N3 (src/harm_mpi.c:1470-1472)          */
-  int metric;              /* 0: METRIC_MINK_CARTESIAN, 2: METRIC_KS_SPHERICAL (src/options.h:73-83) */
+  int metric;              /* 0: METRIC_MINK_CARTESIAN, 2: METRIC_KS_SPHERICAL (src/options.h:73-83),
+                              3: the time-dependent binary-black-hole metric of hgpu_pn_geometry()
+                              (METRIC_GENERAL_DYNAMIC with a PN binary, src/metric.h:104-171)     */
   int coords;              /* 0: COORD_IDENTITY, 1: COORD_DIAGONAL (src/options.h:136-147)           */
   int recon;               /* 2: RECON_MC, 5: RECON_PPM_MC (src/options.h:151-157)                   */
   int flux;                /* 0: FLUX_LF, 1: FLUX_HLLE (src/options.h:169-172)                        */
@@ -50,6 +52,10 @@ typedef struct hgpu_config {
                               values), FP64 conserved update with compensated flux-divergence
                               sums, FP64 con-to-prim, metric, connection and cooling.
                               No counterpart in the reference.                                     */
+  /* metric = 3 only: hole masses, separation and orbital phase at t = 0, and whether the
+   * orbit shrinks by radiation reaction (1) or stays circular (0)                      */
+  double bbh_m1, bbh_m2, bbh_sep, bbh_phi0;
+  int bbh_inspiral;
 } hgpu_config;
 
 typedef struct hgpu_ctx hgpu_ctx;
