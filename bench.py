@@ -33,7 +33,12 @@ Also reported:
                        trained on the GPU on zones the library solved itself,
                        training time reported apart), one-pass and compacted
                        solves, with the reference's own Utoprim_2d_fast() timed
-                       on the host cores beside it.
+                       on the host cores beside it;
+  bbh               -- circumbinary disk around an inspiralling equal-mass binary
+                       (separation 20 M, 1PN near-zone metric recomputed on the
+                       device at three time levels every step) on 300x160x400
+                       zones, r in [1.5, 13] separations: mixed-precision step,
+                       autotuned launch shapes, graph-replayed steps.
 """
 from __future__ import annotations
 
@@ -63,15 +68,16 @@ ALGO_BYTES_MIXED = {"flux": 100.0, "emf": 36.0, "update": 0.5 * (376.0 + 440.0) 
 
 
 def measured_traffic(workload):
-    """HBM bytes per dispatch of each kernel, from the committed counter summary
-    profiles/hbm_traffic_<workload>.md (two rocprofv3 --pmc runs, FETCH_SIZE and
-    WRITE_SIZE, calibrated on a plain 1 GiB device copy); None when absent."""
+    """HBM bytes per launch of each timed scope (flux, emf, update), from the
+    committed counter summary profiles/hbm_traffic_<workload>.md (two rocprofv3
+    --pmc runs, FETCH_SIZE and WRITE_SIZE, calibrated on a plain 1 GiB device
+    copy); None when absent."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.md")
     if not os.path.exists(path):
         return None
     for ln in open(path):
-        if ln.startswith("hbm_bytes_per_dispatch "):
-            return json.loads(ln[len("hbm_bytes_per_dispatch "):])
+        if ln.startswith("hbm_bytes_per_launch "):
+            return json.loads(ln[len("hbm_bytes_per_launch "):])
     return None
 
 
@@ -248,8 +254,47 @@ def con2prim_report(args):
     return rep
 
 
+def bbh_report(args):
+    """Circumbinary disk on the time-dependent binary metric (one GPU's share)."""
+    import numpy as np
+    from coolharm3d_amd import Harm3D, bbh_grid, bbh_disk
+    from coolharm3d_amd.problems import bbh_gdet
+    from coolharm3d_amd.pn import PnMetric
+    shape = tuple(int(x) for x in args.bbh_grid.split("x"))
+    g = bbh_grid(*shape)
+    p0 = bbh_disk(g, bbh_gdet(g, PnMetric))
+    H = Harm3D(g, precision="mixed", track_nfail=0)
+    H.set_prims(p0)
+    del p0
+    H.fixup_bounds()
+    H.set_time(0.0, 1.0e-3)
+    H.step_ch(2)
+    H.profile_enable(True)
+    H.step_ch(3)
+    kern = {k: H.profile_read(k)[0] / 3 for k in ("metric", "flux", "emf", "update", "recover", "bounds", "fixup")}
+    H.profile_enable(False)
+    shapes = H.autotune(reps=1)
+    H.set_graph(True)
+    H.step_ch(args.warmup)
+    H.sync()
+    t0 = time.perf_counter()
+    H.step_ch(args.bbh_steps)
+    H.sync()
+    el = time.perf_counter() - t0
+    divb = H.divb_max()
+    finite = bool(np.isfinite(H.prims()).all())
+    H.close()
+    ncell = shape[0] * shape[1] * shape[2]
+    return {"workload": f"bbh_disk_{shape[0]}x{shape[1]}x{shape[2]}_mixed", "value": ncell * args.bbh_steps / el,
+            "unit": "zone-cycles/s", "ms_per_step": el / args.bbh_steps * 1e3, "steps": args.bbh_steps,
+            "kernel_ms_per_step_eager": kern, "launch_shapes": shapes, "graph": True, "finite": finite,
+            "divb_max": divb, "metric": "1PN near-zone, quasi-circular inspiral (Peters), separation 20 M"}
+
+
 def timed_run(args, g, p0, precision):
-    """Warm-up + timed step_ch() of one grid; returns timings and per-kernel figures."""
+    """Warm-up, launch-shape tuning, a profiled pass (per-kernel HIP-event times)
+    and the timed step_ch() calls (replayed from a captured graph unless
+    --graph off); returns timings and per-kernel figures."""
     from coolharm3d_amd import Harm3D
     H = Harm3D(g, track_nfail=1, precision=precision)
     H.set_prims(p0)
@@ -257,18 +302,26 @@ def timed_run(args, g, p0, precision):
     H.set_time(0.0, 1.0e-5)
     H.step_ch(args.warmup)
     H.sync()
+    shapes = H.autotune(reps=2) if args.tune == "on" else None
+    nprof = args.prof_steps
     H.profile_enable(True)
+    H.sync()
+    H.step_ch(nprof)
+    H.sync()
+    kern = {nm: H.profile_read(nm) for nm in ("flux", "emf", "update", "recover", "bounds", "fixup", "step")}
+    H.profile_enable(False)
+    if args.graph == "on":
+        H.set_graph(True)
+        H.step_ch(2)
     H.sync()
     t0 = time.perf_counter()
     H.step_ch(args.steps)
     H.sync()
     el = time.perf_counter() - t0
-    kern = {nm: H.profile_read(nm) for nm in ("flux", "emf", "update", "bounds", "fixup", "step")}
-    H.profile_enable(False)
     import numpy as np
     finite = bool(np.isfinite(H.prims()).all())
     H.close()
-    return el, kern, finite
+    return el, kern, finite, shapes
 
 
 def precision_report(shape, steps=(1, 10, 100)):
@@ -324,6 +377,13 @@ def main():
     ap.add_argument("--slab", type=int, default=4)
     ap.add_argument("--slab-steps", type=int, default=4)
     ap.add_argument("--cpu-kind", choices=["reference", "port"], default="reference")
+    ap.add_argument("--graph", choices=["on", "off"], default="on", help="replay timed steps from a HIP graph")
+    ap.add_argument("--tune", choices=["on", "off"], default="on", help="autotune launch shapes before timing")
+    ap.add_argument("--prof-steps", type=int, default=3, help="steps of the per-kernel profiled pass")
+    ap.add_argument("--bbh", choices=["auto", "off"], default="auto",
+                    help="circumbinary-disk report on the binary metric")
+    ap.add_argument("--bbh-grid", default="300x160x400")
+    ap.add_argument("--bbh-steps", type=int, default=20)
     ap.add_argument("--c2p", choices=["auto", "off"], default="auto",
                     help="batched con-to-prim report (2^c2p-log2 zones)")
     ap.add_argument("--c2p-log2", type=int, default=24)
@@ -362,7 +422,7 @@ def main():
         del src, dst
     g = torus_grid(n1, n2, n3)
     p0 = torus(g)
-    el, kern, finite = timed_run(args, g, p0, args.precision)
+    el, kern, finite, shapes = timed_run(args, g, p0, args.precision)
     assert finite, "non-finite state after the timed steps"
     ncell = n1 * n2 * n3
     value = ncell * args.steps / el
@@ -375,22 +435,24 @@ def main():
     achieved = algo * units[dom] / avg_s / 1e9
     workload = f"torus{n1}x{n2}x{n3}_{tag}"
     traffic_tab = measured_traffic(workload)
-    kname = {"flux": "k_flux", "emf": "k_emf", "update": "k_update"}[dom]
-    traffic = traffic_tab.get(kname) if traffic_tab else None
+    traffic = traffic_tab.get(dom) if traffic_tab else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                 "traffic_source": f"profiles/hbm_traffic_{workload}.md" if traffic else None,
                 "avg_launch_ms": avg_s * 1e3, "algo_bytes_per_launch": algo * units[dom],
-                "kernel_ms_per_step": {k: kern[k][0] / args.steps for k in ("flux", "emf", "update", "bounds", "fixup")},
-                "state_roofline_frac": value * 320.0 / (HBM_PEAK_GBS * 1e9)}
+                "kernel_ms_per_step": {k: kern[k][0] / args.prof_steps
+                                       for k in ("flux", "emf", "update", "recover", "bounds", "fixup")},
+                "state_roofline_frac": value * 320.0 / (HBM_PEAK_GBS * 1e9),
+                "launch_shapes": shapes, "timed_steps_graph": args.graph == "on"}
 
     mixed = None
     if args.mixed == "auto" and args.precision == "fp64":
-        el_m, kern_m, fin_m = timed_run(args, g, p0, "mixed")
+        el_m, kern_m, fin_m, shapes_m = timed_run(args, g, p0, "mixed")
         v_m = ncell * args.steps / el_m
         mixed = {"value": v_m, "unit": "zone-cycles/s", "ms_per_step": el_m / args.steps * 1e3,
-                 "speedup_vs_fp64": v_m / value, "finite": fin_m,
-                 "kernel_ms_per_step": {k: kern_m[k][0] / args.steps for k in ("flux", "emf", "update", "bounds", "fixup")}}
+                 "speedup_vs_fp64": v_m / value, "finite": fin_m, "launch_shapes": shapes_m,
+                 "kernel_ms_per_step": {k: kern_m[k][0] / args.prof_steps
+                                        for k in ("flux", "emf", "update", "recover", "bounds", "fixup")}}
 
     prec_rep = None
     if args.precision_report == "auto":
@@ -401,6 +463,7 @@ def main():
         cpu = cpu_baseline(args)
 
     c2p = con2prim_report(args) if args.c2p == "auto" else None
+    bbh = bbh_report(args) if args.bbh == "auto" else None
 
     line = {
         "metric": METRIC, "value": value, "unit": "zone-cycles/s", "n_gpus": 1,
@@ -416,6 +479,7 @@ def main():
         "mixed_precision": mixed,
         "precision_variant": prec_rep,
         "con2prim": c2p,
+        "bbh": bbh,
     }
     if rank == 0:
         print(json.dumps(line), flush=True)

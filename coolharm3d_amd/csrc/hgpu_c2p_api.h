@@ -8,7 +8,7 @@
 
 namespace hgpu {
 
-constexpr int C2P_BUDGET_FIRST = 8, C2P_BUDGET_NEXT = 4;
+constexpr int C2P_BUDGET_FIRST = 16, C2P_BUDGET_NEXT = 8;   // measured best on the 2^24-zone workload
 constexpr int MLP_GT = 64;                  // samples per gradient workgroup
 constexpr int MLP_PART = MLP_NTRAIN + 1;    // per-workgroup partial: gradient + loss
 

@@ -393,7 +393,7 @@ struct Api {
   // level 0; fixup_bounds() and the diagnostics use it.
   static int set_levels_at_state(hgpu_ctx* c) {
     Ptrs& X = c->X;
-    int rc = BE::fill_geometry(c, c->lev[0], c->levf[0], 0, 0);
+    int rc = BE::fill_geometry(c, c->lev[0], c->levf[0], 0, 1u << CENT, 0u);
     X.geom = X.geom_n = X.geom_new = c->lev[0];
     X.geomf = c->levf[0];
     X.conn = c->connlev[0];
@@ -432,13 +432,17 @@ struct Api {
   // time-dependent metric (set_general_geometry()/set_general_conn(),
   // src/metric.c:3390-3647): levels t and t + dt/2 at all positions with their
   // connections, t + dt at zone centres.
+  // Only what is read is filled: face records in FP32 for the mixed-precision
+  // fluxes (FP64 otherwise), centre records in FP64; no corner records.
   static int fill_levels(hgpu_ctx* c) {
+    const unsigned faces = (1u << FACE1) | (1u << FACE2) | (1u << FACE3), cent = 1u << CENT;
+    const unsigned m64 = c->P.mixed ? cent : (faces | cent), m32 = c->P.mixed ? faces : 0u;
     int rc = 0;
-    rc |= BE::fill_geometry(c, c->lev[0], c->levf[0], 0, 0);
+    rc |= BE::fill_geometry(c, c->lev[0], c->levf[0], 0, m64, m32);
     rc |= BE::fill_conn(c, c->connlev[0], 0);
-    rc |= BE::fill_geometry(c, c->lev[1], c->levf[1], 1, 0);
+    rc |= BE::fill_geometry(c, c->lev[1], c->levf[1], 1, m64, m32);
     rc |= BE::fill_conn(c, c->connlev[1], 1);
-    rc |= BE::fill_geometry(c, c->lev[2], nullptr, 2, 1);
+    rc |= BE::fill_geometry(c, c->lev[2], nullptr, 2, cent, 0u);
     return rc;
   }
 

@@ -179,19 +179,24 @@ struct HostBE {
     *out = m;
     return 0;
   }
-  static int fill_geometry(hgpu_ctx* c, Geom* dst, GeomF* dstf, int tsel, int cent_only) {
+  static int fill_geometry(hgpu_ctx* c, Geom* dst, GeomF* dstf, int tsel, unsigned mask64, unsigned mask32) {
     const Params& P = c->P;
     const double t = level_time(c->X, tsel);
-    for (int pos = cent_only ? CENT : 0; pos < (cent_only ? CENT + 1 : NPOS); pos++)
+    for (int pos = 0; pos < NPOS; pos++) {
+      const bool f64 = (mask64 >> pos) & 1u, f32 = ((mask32 >> pos) & 1u) && dstf;
+      if (!f64 && !f32) continue;
       for (int i = 0; i < P.n1t; i++)
         for (int j = 0; j < P.n2t; j++)
           for (int k = 0; k < P.n3t; k++) {
             double Xc[4];
             zone_X(P, c->pn, i, j, k, pos, Xc);
             const int id = geom_id(P, i, j, k, pos);
-            pn_geom(c->pn.B, t, Xc[1], Xc[2], Xc[3], &dst[id]);
-            if (P.mixed && dstf) geom_f32(dst[id], dstf[id]);
+            Geom g;
+            pn_geom(c->pn.B, t, Xc[1], Xc[2], Xc[3], &g);
+            if (f64) dst[id] = g;
+            if (f32) geom_f32(g, dstf[id]);
           }
+    }
     return 0;
   }
   static int fill_conn(hgpu_ctx* c, double* dst, int tsel) {

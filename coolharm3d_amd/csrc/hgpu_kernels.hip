@@ -376,24 +376,51 @@ __global__ __launch_bounds__(BLOCK) void k_divb(Params P, Ptrs X, Box b, unsigne
   if (v == v) atomicMax(out, dbl_bits(v));
 }
 
-// Time-dependent metric: geometry records of positions [pos0, pos0 + npos) of
-// every zone at time level tsel, recomputed per zone (and their FP32 copies).
-__global__ __launch_bounds__(BLOCK) void k_fill_geom(Params P, Ptrs X, PnGrid G, int tsel, int pos0, int npos,
-                                                    Geom* dst, GeomF* dstf) {
-  const long long z = (long long)blockIdx.x * BLOCK + threadIdx.x;
-  if (z >= (long long)npos * P.nt) return;
-  const int pos = pos0 + (int)(z / P.nt);
-  const long long cell = z - (long long)(pos - pos0) * P.nt;
-  const int k = (int)(cell % P.n3t);
-  const int j = (int)((cell / P.n3t) % P.n2t);
-  const int i = (int)(cell / ((long long)P.n3t * P.n2t));
-  double Xc[4];
-  zone_X(P, G, i, j, k, pos, Xc);
+// Time-dependent metric: geometry records of position pos of every zone at
+// time level tsel, recomputed per zone, in FP64 (dst) and/or FP32 (dstf).  One
+// wavefront computes 64 consecutive records, stages them in LDS and stores
+// them as one contiguous, coalesced run.
+constexpr int FILL_B = 64;
+constexpr int GEOMF_FLOATS = (int)(sizeof(GeomF) / sizeof(float));
+__global__ __launch_bounds__(FILL_B) void k_fill_geom(Params P, Ptrs X, PnGrid G, int tsel, int pos, Geom* dst,
+                                                     GeomF* dstf) {
+  __shared__ double sbuf[FILL_B * GEOM_DOUBLES];   // one staging buffer, FP64 or FP32 records
+  const long long nt = P.nt;
+  const long long z0 = (long long)blockIdx.x * FILL_B;
+  const long long z = z0 + threadIdx.x;
+  const long long nrec = (nt - z0 < FILL_B) ? nt - z0 : FILL_B;
   Geom g;
-  pn_geom(G.B, level_time(X, tsel), Xc[1], Xc[2], Xc[3], &g);
-  const int id = geom_id(P, i, j, k, pos);
-  dst[id] = g;
-  if (dstf) geom_f32(g, dstf[id]);
+  if (z < nt) {
+    const int k = (int)(z % P.n3t);
+    const int j = (int)((z / P.n3t) % P.n2t);
+    const int i = (int)(z / ((long long)P.n3t * P.n2t));
+    double Xc[4];
+    zone_X(P, G, i, j, k, pos, Xc);
+    pn_geom(G.B, level_time(X, tsel), Xc[1], Xc[2], Xc[3], &g);
+  }
+  // records of position pos are contiguous: geom_id = pos * NT + zone index
+  if (dst) {
+    if (z < nt) {
+      const double* gp = reinterpret_cast<const double*>(&g);
+      for (int l = 0; l < GEOM_DOUBLES; l++) sbuf[threadIdx.x * GEOM_DOUBLES + l] = gp[l];
+    }
+    __syncthreads();
+    double* out = reinterpret_cast<double*>(dst + (long long)pos * nt + z0);
+    for (long long m = threadIdx.x; m < nrec * GEOM_DOUBLES; m += FILL_B) out[m] = sbuf[m];
+    __syncthreads();
+  }
+  if (dstf) {
+    float* sf = reinterpret_cast<float*>(sbuf);
+    if (z < nt) {
+      GeomF f;
+      geom_f32(g, f);
+      const float* fp = reinterpret_cast<const float*>(&f);
+      for (int l = 0; l < GEOMF_FLOATS; l++) sf[threadIdx.x * GEOMF_FLOATS + l] = fp[l];
+    }
+    __syncthreads();
+    float* outf = reinterpret_cast<float*>(dstf + (long long)pos * nt + z0);
+    for (long long m = threadIdx.x; m < nrec * GEOMF_FLOATS; m += FILL_B) outf[m] = sf[m];
+  }
 }
 
 // Connection at the interior zone centres at time level tsel.
@@ -790,13 +817,20 @@ struct GpuBE {
     *out = bits_dbl(h);
     return rc;
   }
-  static int fill_geometry(hgpu_ctx* c, Geom* dst, GeomF* dstf, int tsel, int cent_only) {
+  // positions in mask64 to dst (FP64), in mask32 to dstf (FP32)
+  static int fill_geometry(hgpu_ctx* c, Geom* dst, GeomF* dstf, int tsel, unsigned mask64, unsigned mask32) {
     Scope s(c, "metric");
-    const int pos0 = cent_only ? CENT : 0, npos = cent_only ? 1 : NPOS;
-    const long long n = (long long)npos * c->P.nt;
-    hipLaunchKernelGGL(k_fill_geom, dim3(nblocks(n)), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, c->pn, tsel, pos0,
-                       npos, dst, c->P.mixed ? dstf : nullptr);
-    return hip_check(hipGetLastError(), "k_fill_geom");
+    const unsigned nb = (unsigned)((c->P.nt + FILL_B - 1) / FILL_B);
+    int rc = 0;
+    for (int pos = 0; pos < NPOS; pos++) {
+      Geom* d64 = (mask64 >> pos) & 1u ? dst : nullptr;
+      GeomF* d32 = ((mask32 >> pos) & 1u) && dstf ? dstf : nullptr;
+      if (!d64 && !d32) continue;
+      hipLaunchKernelGGL(k_fill_geom, dim3(nb), dim3(FILL_B), 0, priv(c)->stream, c->P, c->X, c->pn, tsel, pos, d64,
+                         d32);
+      rc |= hip_check(hipGetLastError(), "k_fill_geom");
+    }
+    return rc;
   }
   static int fill_conn(hgpu_ctx* c, double* dst, int tsel) {
     Scope s(c, "metric");
