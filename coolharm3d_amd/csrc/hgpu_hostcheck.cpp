@@ -40,7 +40,7 @@ struct HostBE {
   static bool graph_enabled(hgpu_ctx*) { return false; }
   static int invalidate_graph(hgpu_ctx*) { return 0; }
   // launch shapes do not apply to the host build (zones run one by one)
-  static int set_shape(hgpu_ctx*, int fam, int, int, int) {
+  static int set_shape(hgpu_ctx*, int fam, int, int, int) {   // bs, kc, flags ignored
     return (fam < 0 || fam >= 6) ? set_error(-2, "kernel family out of range") : 0;
   }
   static int get_shape(hgpu_ctx*, int fam, int* bs, int* kc, int* remap) {

@@ -18,6 +18,7 @@ struct LaunchShape {
   unsigned bs = 256;     // threads per workgroup: 64, 128 or 256
   unsigned kc = 0;       // k-extent of a 2D tile, dividing bs; 0 = flat tiles
   unsigned remap = 1;    // XCD-contiguous tile order
+  unsigned order = 0;    // 0: plane after plane; 1: x1 fastest (one (j,k) tile through consecutive planes)
 };
 enum { FAM_FLUX1 = 0, FAM_FLUX2, FAM_FLUX3, FAM_EMF, FAM_CONS, FAM_INV, N_FAM };
 
@@ -108,8 +109,11 @@ __device__ __forceinline__ void commit_fail(const Params& P, const Ptrs& X, cons
 // (kc > 0).  Hardware dispatches workgroup p to XCD p % 8, and each XCD has its
 // own L2; with remap set, the logical tile of workgroup p is chosen so that
 // every XCD gets one contiguous range of tiles (whole x1-slabs), processed in
-// order, so the +-3-zone stencils of neighbouring tiles hit the same L2.  The
-// shape of each kernel family is picked by the autotuner (or set by the caller).
+// order, so the +-3-zone stencils of neighbouring tiles hit the same L2.  With
+// order = 1 consecutive workgroups take one (j,k) tile through consecutive
+// x1-planes, so an x1 stencil finds the planes it shares with the previous
+// workgroup in L2.  The shape of each kernel family is picked by the autotuner
+// (or set by the caller).
 struct Tiling {
   unsigned ni, nj, nk;  // box extents
   unsigned bs, kc, jr;  // tile: kc x jr (kc > 0) or bs consecutive points (kc = 0)
@@ -117,6 +121,7 @@ struct Tiling {
   unsigned X;           // tiles per plane
   unsigned nblocks;     // X * ni, rounded up to a multiple of 8 when remapped
   unsigned remap;
+  unsigned order;
 };
 
 static inline Tiling make_tiling(const Box& b, const LaunchShape& L = LaunchShape()) {
@@ -126,6 +131,7 @@ static inline Tiling make_tiling(const Box& b, const LaunchShape& L = LaunchShap
   T.ni = (unsigned)(b.hi[0] - b.lo[0] + 1);
   T.bs = L.bs;
   T.remap = L.remap;
+  T.order = L.order;
   if (L.kc > 0 && L.kc < T.nk && L.bs % L.kc == 0) {
     T.kc = L.kc;
     T.jr = L.bs / L.kc;
@@ -145,9 +151,16 @@ static inline Tiling make_tiling(const Box& b, const LaunchShape& L = LaunchShap
 __device__ __forceinline__ bool tile_index(const Box& b, const Tiling& T, int& i, int& j, int& k) {
   const unsigned p = blockIdx.x;
   const unsigned L = T.remap ? (p & 7u) * (T.nblocks >> 3) + (p >> 3) : p;
-  const unsigned ii = L / T.X;
+  unsigned ii, t;
+  if (T.order == 0) {
+    ii = L / T.X;
+    t = L - ii * T.X;
+  } else {
+    t = L / T.ni;
+    ii = L - t * T.ni;
+    if (t >= T.X) return false;
+  }
   if (ii >= T.ni) return false;
-  const unsigned t = L - ii * T.X;
   unsigned jj, kk;
   if (T.kc == 0) {
     const unsigned q = t * T.bs + threadIdx.x;
@@ -603,7 +616,8 @@ struct GpuBE {
 
   // Launch-shape autotuner.  For each stencil family (flux in x1, x2, x3, EMF,
   // conserved update) every candidate shape -- workgroup size 64/128/256, flat
-  // or kc x (bs/kc) tiles with kc in {16, 32, 64}, XCD remap on/off -- is timed
+  // or kc x (bs/kc) tiles with kc in {16, 32, 64}, XCD remap on/off, plane-major
+  // or x1-fastest tile order -- is timed
   // with HIP events over `reps` launches on the current state, and the fastest
   // kept.  These kernels write only face fluxes, EMFs and the conserved
   // scratch, which every substep recomputes before use; the failure words and
@@ -612,14 +626,14 @@ struct GpuBE {
   static int check_shape(const LaunchShape& L) {
     if (L.bs != 64 && L.bs != 128 && L.bs != 256) return set_error(-2, "workgroup size must be 64, 128 or 256");
     if (L.kc != 0 && (L.kc > L.bs || L.bs % L.kc != 0)) return set_error(-2, "tile k-extent must divide the workgroup size");
-    if (L.remap > 1) return set_error(-2, "remap must be 0 or 1");
+    if (L.remap > 1 || L.order > 1) return set_error(-2, "remap and order must be 0 or 1");
     return 0;
   }
-  static int set_shape(hgpu_ctx* c, int fam, int bs, int kc, int remap) {
+  static int set_shape(hgpu_ctx* c, int fam, int bs, int kc, int flags) {
     if (fam < 0 || fam >= N_FAM) return set_error(-2, "kernel family out of range");
     LaunchShape L;
-    L.bs = (unsigned)bs; L.kc = (unsigned)kc; L.remap = (unsigned)remap;
-    if (bs < 0 || kc < 0 || remap < 0) return set_error(-2, "negative launch-shape field");
+    L.bs = (unsigned)bs; L.kc = (unsigned)kc; L.remap = (unsigned)flags & 1u; L.order = ((unsigned)flags >> 1) & 1u;
+    if (bs < 0 || kc < 0 || flags < 0 || flags > 3) return set_error(-2, "bad launch-shape field");
     if (int rc = check_shape(L)) return rc;
     int rc = invalidate_graph(c);
     priv(c)->shape[fam] = L;
@@ -630,7 +644,7 @@ struct GpuBE {
     const LaunchShape& L = priv(c)->shape[fam];
     if (bs) *bs = (int)L.bs;
     if (kc) *kc = (int)L.kc;
-    if (remap) *remap = (int)L.remap;
+    if (remap) *remap = (int)(L.remap | (L.order << 1));
     return 0;
   }
   static int conserved_only(hgpu_ctx* c) {
@@ -671,9 +685,9 @@ struct GpuBE {
     for (int fam = FAM_FLUX1; fam <= FAM_CONS && !rc; fam++) {
       LaunchShape best = g->shape[fam];
       float best_ms = 1e30f;
-      for (unsigned bs : bss) for (unsigned kc : kcs) for (unsigned remap = 0; remap <= 1; remap++) {
+      for (unsigned bs : bss) for (unsigned kc : kcs) for (unsigned flags = 0; flags < 4; flags++) {
         LaunchShape L;
-        L.bs = bs; L.kc = kc; L.remap = remap;
+        L.bs = bs; L.kc = kc; L.remap = flags & 1u; L.order = flags >> 1;
         if (check_shape(L)) { g_last_error.clear(); continue; }
         g->shape[fam] = L;
         rc |= launch_family(c, fam);   // warm-up
@@ -690,7 +704,7 @@ struct GpuBE {
       g->shape[fam] = best;
       if (report) {
         report[fam*4 + 0] = best.bs; report[fam*4 + 1] = best.kc;
-        report[fam*4 + 2] = best.remap; report[fam*4 + 3] = best_ms;
+        report[fam*4 + 2] = best.remap | (best.order << 1); report[fam*4 + 3] = best_ms;
       }
     }
     g->pool.push_back(e0);

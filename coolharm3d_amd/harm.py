@@ -154,15 +154,16 @@ class Harm3D:
     def autotune(self, reps=2):
         """Times the launch-shape candidates of the stencil kernels on the
         current state and keeps the fastest (hgpu_autotune).  Returns
-        {family: (bs, kc, remap, ms)}."""
+        {family: (bs, kc, flags, ms)}."""
         rep = np.zeros(20)
         self._chk(self.lib.hgpu_autotune(self.h, reps, _dp(rep)), "hgpu_autotune")
         return {f: (int(rep[4 * i]), int(rep[4 * i + 1]), int(rep[4 * i + 2]), float(rep[4 * i + 3]))
                 for i, f in enumerate(self.FAMILIES[:5])}
 
-    def set_launch_shape(self, family, bs=256, kc=0, remap=1):
+    def set_launch_shape(self, family, bs=256, kc=0, flags=1):
+        """flags bit 0: XCD-contiguous tile order; bit 1: x1-fastest tile order."""
         fam = self.FAMILIES.index(family) if isinstance(family, str) else int(family)
-        self._chk(self.lib.hgpu_set_launch_shape(self.h, fam, bs, kc, remap), "hgpu_set_launch_shape")
+        self._chk(self.lib.hgpu_set_launch_shape(self.h, fam, bs, kc, flags), "hgpu_set_launch_shape")
 
     def launch_shape(self, family):
         fam = self.FAMILIES.index(family) if isinstance(family, str) else int(family)

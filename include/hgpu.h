@@ -104,14 +104,15 @@ int hgpu_set_graph(hgpu_ctx *ctx, int on);
 /* Launch shapes of the stencil kernel families (0-2: face fluxes in x1, x2,
  * x3; 3: EMFs; 4: conserved update; 5: inversion): workgroup size bs (64, 128,
  * 256), tile k-extent kc (0 = bs consecutive (j,k) points, else kc x bs/kc
- * tiles; kc divides bs), XCD-contiguous tile order remap (0/1).
+ * tiles; kc divides bs), flags bit 0: XCD-contiguous tile order, bit 1: x1
+ * fastest (a (j,k) tile through consecutive x1-planes).
  * hgpu_autotune() times every candidate of families 0-4 on the current state
  * (reps launches each) and keeps the fastest; report (NULL or [5][4]) gets
- * bs, kc, remap, ms per launch.  The state, flags and counters are unchanged;
+ * bs, kc, flags, ms per launch.  The state, flags and counters are unchanged;
  * results do not depend on the shapes.  No counterpart in the reference. */
 int hgpu_autotune(hgpu_ctx *ctx, int reps, double *report);
-int hgpu_set_launch_shape(hgpu_ctx *ctx, int family, int bs, int kc, int remap);
-int hgpu_get_launch_shape(hgpu_ctx *ctx, int family, int *bs, int *kc, int *remap);
+int hgpu_set_launch_shape(hgpu_ctx *ctx, int family, int bs, int kc, int flags);
+int hgpu_get_launch_shape(hgpu_ctx *ctx, int family, int *bs, int *kc, int *flags);
 
 /* One advance() stage (0: predictor into ph, 1: corrector into p) followed by
  * bounds and fixups, as step_ch() sequences them (src/step_ch.c:92-180). */

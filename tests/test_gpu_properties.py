@@ -132,8 +132,8 @@ def test_launch_shapes_and_autotune_keep_results():
         H.set_time(0.0, 1.0e-5)
         H.step_ch(1)
         if mode == "shapes":
-            for fam, (bs, kc, remap) in zip(H.FAMILIES, [(64, 16, 0), (128, 32, 1), (256, 64, 0),
-                                                         (64, 0, 1), (128, 16, 0), (64, 32, 1)]):
+            for fam, (bs, kc, remap) in zip(H.FAMILIES, [(64, 16, 2), (128, 32, 1), (256, 64, 3),
+                                                         (64, 0, 1), (128, 16, 2), (64, 32, 3)]):
                 H.set_launch_shape(fam, bs, kc, remap)
                 assert H.launch_shape(fam) == (bs, kc, remap)
         if mode == "autotune":
