@@ -411,15 +411,20 @@ HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int
                        const Geom* gn, const Geom* g, FailRec& fr, double* U) {
   const long long NT = P.nt;
   const long long c = cidx(P, i, j, k);
-  {
-    // U^n = primtoflux(p_i) with geom_i (src/step_ch.c:319, :339-341)
+  if (stage == 0) {
+    // U^n = primtoflux(p_i) with geom_i (src/step_ch.c:319, :339-341), kept for
+    // the corrector as KEEP_CONSERVED_VARS keeps U_gf (src/step_ch.c:321-333)
     double pi[NP];
 #pragma unroll
     for (int v = 0; v < NP; v++) pi[v] = rt_val(P, X, v, i, X.p[v*NT + c]);
     State q;
-    int gf = get_state(pi, gn, P.gam, &q);
-    if (stage == 0 && gf) fr.nf[NFAIL_GAMMA_CALC]++;
+    if (get_state(pi, gn, P.gam, &q)) fr.nf[NFAIL_GAMMA_CALC]++;
     primtoflux(pi, &q, 0, gn, U);
+#pragma unroll
+    for (int v = 0; v < NP; v++) X.Un[v*NT + c] = U[v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < NP; v++) U[v] = X.Un[v*NT + c];
   }
   phase_fence();
   double dU[NP];

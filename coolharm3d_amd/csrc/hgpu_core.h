@@ -280,6 +280,7 @@ struct Api {
     rc |= BE::alloc(c, (void**)&X.gamma, sizeof(double) * NT);
     const long long nint = (long long)P.n1 * P.n2 * P.n3;
     rc |= BE::alloc(c, (void**)&X.Uslow, sizeof(double) * NP * NT);
+    rc |= BE::alloc(c, (void**)&X.Un, sizeof(double) * NP * NT);
     rc |= BE::alloc(c, (void**)&X.slow_idx, sizeof(unsigned) * nint);
     rc |= BE::alloc(c, (void**)&X.pflag, sizeof(int) * NT);
     rc |= BE::alloc(c, (void**)&X.nfail, sizeof(int) * (P.track_nfail ? N_NFAIL * NT : 1));
@@ -325,7 +326,7 @@ struct Api {
     // buffer allocated for them), so the levels are what is released
     void* ptrs[] = {X.p, X.ph, X.F, X.emf, (void*)X.omega, (void*)X.tr,
                     (void*)X.rscale, (void*)X.irscale, (void*)X.rhoflr, (void*)X.uuflr, X.gamma,
-                    X.pflag, X.nfail, X.status, X.scal, X.Ff, X.emff, X.Uslow, X.slow_idx,
+                    X.pflag, X.nfail, X.status, X.scal, X.Ff, X.emff, X.Uslow, X.Un, X.slow_idx,
                     c->lev[0], c->lev[1], c->lev[2], c->levf[0], c->levf[1], c->connlev[0], c->connlev[1]};
     for (void* p : ptrs) if (p) BE::release(c, p);
     BE::fini(c);

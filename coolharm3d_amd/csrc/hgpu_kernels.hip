@@ -19,6 +19,7 @@ struct LaunchShape {
   unsigned kc = 0;       // k-extent of a 2D tile, dividing bs; 0 = flat tiles
   unsigned remap = 1;    // XCD-contiguous tile order
   unsigned order = 0;    // 0: plane after plane; 1: x1 fastest (one (j,k) tile through consecutive planes)
+  unsigned occ = 0;      // 1: the register-capped kernel variant (flux, conserved update)
 };
 enum { FAM_FLUX1 = 0, FAM_FLUX2, FAM_FLUX3, FAM_EMF, FAM_CONS, FAM_INV, N_FAM };
 
@@ -184,8 +185,10 @@ __device__ __forceinline__ bool tile_index(const Box& b, const Tiling& T, int& i
 constexpr unsigned MASK_GAMMA = 1u << NFAIL_GAMMA_CALC;
 constexpr unsigned MASK_ALL = (1u << N_NFAIL) - 1u;
 
-template <int D>
-__global__ __launch_bounds__(BLOCK) void k_flux(Params P, Ptrs X, const double* __restrict__ src, Box b, Tiling T) {
+// W: minimum waves per SIMD the register allocation must allow (1 = the
+// compiler's choice; larger values cap the VGPRs, trading spills for occupancy).
+template <int D, int W>
+__global__ __launch_bounds__(BLOCK, W) void k_flux(Params P, Ptrs X, const double* __restrict__ src, Box b, Tiling T) {
   int i, j, k;
   const bool active = tile_index(b, T, i, j, k);
   FailRec fr; fr.clear();
@@ -197,9 +200,9 @@ __global__ __launch_bounds__(BLOCK) void k_flux(Params P, Ptrs X, const double* 
   commit_fail(P, X, fr, active, c, MASK_GAMMA, D + 1);
 }
 
-template <int D>
-__global__ __launch_bounds__(BLOCK) void k_flux_mixed(Params P, Ptrs X, const double* __restrict__ src, Box b,
-                                                      Tiling T) {
+template <int D, int W>
+__global__ __launch_bounds__(BLOCK, W) void k_flux_mixed(Params P, Ptrs X, const double* __restrict__ src, Box b,
+                                                         Tiling T) {
   int i, j, k;
   const bool active = tile_index(b, T, i, j, k);
   FailRec fr; fr.clear();
@@ -225,8 +228,8 @@ __global__ __launch_bounds__(BLOCK) void k_emf_mixed(Params P, Ptrs X, Box b, Ti
 
 // Conserved update U^{n+1} of every interior zone into the U buffer
 // ([8][NT], indexed like the state).
-template <bool MIXED>
-__global__ __launch_bounds__(BLOCK) void k_conserved(Params P, Ptrs X, int stage, Box b, Tiling T) {
+template <bool MIXED, int W>
+__global__ __launch_bounds__(BLOCK, W) void k_conserved(Params P, Ptrs X, int stage, Box b, Tiling T) {
   int i, j, k;
   const bool active = tile_index(b, T, i, j, k);
   FailRec fr; fr.clear();
@@ -626,14 +629,15 @@ struct GpuBE {
   static int check_shape(const LaunchShape& L) {
     if (L.bs != 64 && L.bs != 128 && L.bs != 256) return set_error(-2, "workgroup size must be 64, 128 or 256");
     if (L.kc != 0 && (L.kc > L.bs || L.bs % L.kc != 0)) return set_error(-2, "tile k-extent must divide the workgroup size");
-    if (L.remap > 1 || L.order > 1) return set_error(-2, "remap and order must be 0 or 1");
+    if (L.remap > 1 || L.order > 1 || L.occ > 1) return set_error(-2, "remap, order and occ must be 0 or 1");
     return 0;
   }
   static int set_shape(hgpu_ctx* c, int fam, int bs, int kc, int flags) {
     if (fam < 0 || fam >= N_FAM) return set_error(-2, "kernel family out of range");
     LaunchShape L;
     L.bs = (unsigned)bs; L.kc = (unsigned)kc; L.remap = (unsigned)flags & 1u; L.order = ((unsigned)flags >> 1) & 1u;
-    if (bs < 0 || kc < 0 || flags < 0 || flags > 3) return set_error(-2, "bad launch-shape field");
+    L.occ = ((unsigned)flags >> 2) & 1u;
+    if (bs < 0 || kc < 0 || flags < 0 || flags > 7) return set_error(-2, "bad launch-shape field");
     if (int rc = check_shape(L)) return rc;
     int rc = invalidate_graph(c);
     priv(c)->shape[fam] = L;
@@ -644,15 +648,26 @@ struct GpuBE {
     const LaunchShape& L = priv(c)->shape[fam];
     if (bs) *bs = (int)L.bs;
     if (kc) *kc = (int)L.kc;
-    if (remap) *remap = (int)(L.remap | (L.order << 1));
+    if (remap) *remap = (int)(L.remap | (L.order << 1) | (L.occ << 2));
     return 0;
+  }
+  static void launch_conserved(hgpu_ctx* c, const LaunchShape& L, const Tiling& T, const Box& b, const Ptrs& X,
+                               int stage) {
+    hipStream_t st = priv(c)->stream;
+    const dim3 grid(T.nblocks), blk(T.bs);
+    if (c->P.mixed) {
+      if (L.occ) hipLaunchKernelGGL((k_conserved<true, 4>), grid, blk, 0, st, c->P, X, stage, b, T);
+      else hipLaunchKernelGGL((k_conserved<true, 1>), grid, blk, 0, st, c->P, X, stage, b, T);
+    } else {
+      if (L.occ) hipLaunchKernelGGL((k_conserved<false, 4>), grid, blk, 0, st, c->P, X, stage, b, T);
+      else hipLaunchKernelGGL((k_conserved<false, 1>), grid, blk, 0, st, c->P, X, stage, b, T);
+    }
   }
   static int conserved_only(hgpu_ctx* c) {
     Box b = interior_box(c->P);
     const Tiling T = make_tiling(b, priv(c)->shape[FAM_CONS]);
     hipStream_t st = priv(c)->stream;
-    if (c->P.mixed) hipLaunchKernelGGL(k_conserved<true>, dim3(T.nblocks), dim3(T.bs), 0, st, c->P, c->X, 0, b, T);
-    else hipLaunchKernelGGL(k_conserved<false>, dim3(T.nblocks), dim3(T.bs), 0, st, c->P, c->X, 0, b, T);
+    launch_conserved(c, priv(c)->shape[FAM_CONS], T, b, c->X, 0);
     return hip_check(hipGetLastError(), "k_conserved");
   }
   static int launch_family(hgpu_ctx* c, int fam) {
@@ -685,10 +700,7 @@ struct GpuBE {
     for (int fam = FAM_FLUX1; fam <= FAM_CONS && !rc; fam++) {
       LaunchShape best = g->shape[fam];
       float best_ms = 1e30f;
-      for (unsigned bs : bss) for (unsigned kc : kcs) for (unsigned flags = 0; flags < 4; flags++) {
-        LaunchShape L;
-        L.bs = bs; L.kc = kc; L.remap = flags & 1u; L.order = flags >> 1;
-        if (check_shape(L)) { g_last_error.clear(); continue; }
+      auto time_shape = [&](const LaunchShape& L) -> float {
         g->shape[fam] = L;
         rc |= launch_family(c, fam);   // warm-up
         rc |= hip_check(hipEventRecord(e0, st), "hipEventRecord");
@@ -697,14 +709,26 @@ struct GpuBE {
         rc |= hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
         float ms = 0.f;
         rc |= hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        return rc ? 1e30f : ms / (float)reps;
+      };
+      for (unsigned bs : bss) for (unsigned kc : kcs) for (unsigned flags = 0; flags < 4; flags++) {
+        LaunchShape L;
+        L.bs = bs; L.kc = kc; L.remap = flags & 1u; L.order = flags >> 1;
+        if (check_shape(L)) { g_last_error.clear(); continue; }
+        const float ms = time_shape(L);
         if (rc) break;
-        ms /= (float)reps;
         if (ms < best_ms) { best_ms = ms; best = L; }
+      }
+      if (!rc && fam != FAM_EMF) {   // the register-capped variant of the best shape
+        LaunchShape L = best;
+        L.occ = 1;
+        const float ms = time_shape(L);
+        if (!rc && ms < best_ms) { best_ms = ms; best = L; }
       }
       g->shape[fam] = best;
       if (report) {
         report[fam*4 + 0] = best.bs; report[fam*4 + 1] = best.kc;
-        report[fam*4 + 2] = best.remap | (best.order << 1); report[fam*4 + 3] = best_ms;
+        report[fam*4 + 2] = best.remap | (best.order << 1) | (best.occ << 2); report[fam*4 + 3] = best_ms;
       }
     }
     g->pool.push_back(e0);
@@ -745,14 +769,27 @@ struct GpuBE {
     const Tiling T = make_tiling(b, priv(c)->shape[FAM_FLUX1 + d]);
     const dim3 grid(T.nblocks), blk(T.bs);
     hipStream_t st = priv(c)->stream;
+    const bool occ = priv(c)->shape[FAM_FLUX1 + d].occ != 0;
     if (c->P.mixed) {
-      if (d == 0) hipLaunchKernelGGL(k_flux_mixed<0>, grid, blk, 0, st, c->P, c->X, src, b, T);
-      else if (d == 1) hipLaunchKernelGGL(k_flux_mixed<1>, grid, blk, 0, st, c->P, c->X, src, b, T);
-      else hipLaunchKernelGGL(k_flux_mixed<2>, grid, blk, 0, st, c->P, c->X, src, b, T);
+      if (occ) {
+        if (d == 0) hipLaunchKernelGGL((k_flux_mixed<0, 6>), grid, blk, 0, st, c->P, c->X, src, b, T);
+        else if (d == 1) hipLaunchKernelGGL((k_flux_mixed<1, 6>), grid, blk, 0, st, c->P, c->X, src, b, T);
+        else hipLaunchKernelGGL((k_flux_mixed<2, 6>), grid, blk, 0, st, c->P, c->X, src, b, T);
+      } else {
+        if (d == 0) hipLaunchKernelGGL((k_flux_mixed<0, 1>), grid, blk, 0, st, c->P, c->X, src, b, T);
+        else if (d == 1) hipLaunchKernelGGL((k_flux_mixed<1, 1>), grid, blk, 0, st, c->P, c->X, src, b, T);
+        else hipLaunchKernelGGL((k_flux_mixed<2, 1>), grid, blk, 0, st, c->P, c->X, src, b, T);
+      }
     } else {
-      if (d == 0) hipLaunchKernelGGL(k_flux<0>, grid, blk, 0, st, c->P, c->X, src, b, T);
-      else if (d == 1) hipLaunchKernelGGL(k_flux<1>, grid, blk, 0, st, c->P, c->X, src, b, T);
-      else hipLaunchKernelGGL(k_flux<2>, grid, blk, 0, st, c->P, c->X, src, b, T);
+      if (occ) {
+        if (d == 0) hipLaunchKernelGGL((k_flux<0, 4>), grid, blk, 0, st, c->P, c->X, src, b, T);
+        else if (d == 1) hipLaunchKernelGGL((k_flux<1, 4>), grid, blk, 0, st, c->P, c->X, src, b, T);
+        else hipLaunchKernelGGL((k_flux<2, 4>), grid, blk, 0, st, c->P, c->X, src, b, T);
+      } else {
+        if (d == 0) hipLaunchKernelGGL((k_flux<0, 1>), grid, blk, 0, st, c->P, c->X, src, b, T);
+        else if (d == 1) hipLaunchKernelGGL((k_flux<1, 1>), grid, blk, 0, st, c->P, c->X, src, b, T);
+        else hipLaunchKernelGGL((k_flux<2, 1>), grid, blk, 0, st, c->P, c->X, src, b, T);
+      }
     }
     return hip_check(hipGetLastError(), "k_flux");
   }
@@ -771,8 +808,7 @@ struct GpuBE {
     const Tiling T2 = make_tiling(b, priv(c)->shape[FAM_INV]);
     hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_SLOW, 1);
     hipStream_t st = priv(c)->stream;
-    if (c->P.mixed) hipLaunchKernelGGL(k_conserved<true>, dim3(T.nblocks), dim3(T.bs), 0, st, c->P, c->X, stage, b, T);
-    else hipLaunchKernelGGL(k_conserved<false>, dim3(T.nblocks), dim3(T.bs), 0, st, c->P, c->X, stage, b, T);
+    launch_conserved(c, priv(c)->shape[FAM_CONS], T, b, c->X, stage);
     Ptrs Xn = c->X;
     Xn.geom = c->X.geom_new;   // inversion at the result's time level
     hipLaunchKernelGGL(k_invert, dim3(T2.nblocks), dim3(T2.bs), 0, st, c->P, Xn, stage, b, T2);

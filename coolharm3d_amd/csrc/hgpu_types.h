@@ -126,6 +126,7 @@ struct Ptrs {
   const double* uuflr;
   double* gamma;           // p_gamma [NT]
   double* Uslow;           // U^{n+1} of the stage [8][NT]
+  double* Un;              // U^n of the step, kept from the predictor for the corrector [8][NT]
   unsigned* slow_idx;      // zones queued for the full con-to-prim tree [n_interior]
   int* pflag;              // [NT]
   int* nfail;              // [15][NT]
