@@ -477,6 +477,10 @@ struct GpuBE {
       return set_error(-5, "no HIP device available (the hgpu library runs only on a GPU)");
     GpuPriv* g = new GpuPriv();
     c->be = g;
+    // untuned default: the x1 fluxes take one (j,k) tile through consecutive
+    // x1-planes, so their +-3-plane stencil stays in L2 (measured 1.7x faster
+    // in mixed precision, neutral in FP64)
+    g->shape[FAM_FLUX1].order = 1;
     return hip_check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
   }
   static void fini(hgpu_ctx* c) {
