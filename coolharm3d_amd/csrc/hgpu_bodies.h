@@ -236,13 +236,17 @@ HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, i
       double mhd[4][4];
       mhd_calc(&q, 0, mhd[0]); mhd_calc(&q, 1, mhd[1]);
       mhd_calc(&q, 2, mhd[2]); mhd_calc(&q, 3, mhd[3]);
-      const double* cn = X.conn + (long long)conn_id(P, i, j, k) * 64;
+      // static metric: records [zone][64]; time-dependent: components [64][zone]
+      // (coalesced across the wavefront)
+      const long long cid = conn_id(P, i, j, k);
+      const double* cn = P.dyn ? X.conn + cid : X.conn + cid * 64;
+      const long long cs = P.dyn ? P.nconn : 1;
       for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) {
-        dU[U1] += mhd[a][b]*cn[(b*4 + 1)*4 + a];
-        dU[U2] += mhd[a][b]*cn[(b*4 + 2)*4 + a];
+        dU[U1] += mhd[a][b]*cn[((b*4 + 1)*4 + a)*cs];
+        dU[U2] += mhd[a][b]*cn[((b*4 + 2)*4 + a)*cs];
         if (P.metric_dim > 2) {
-          dU[UU] += mhd[a][b]*cn[(b*4 + 0)*4 + a];
-          dU[U3] += mhd[a][b]*cn[(b*4 + 3)*4 + a];
+          dU[UU] += mhd[a][b]*cn[((b*4 + 0)*4 + a)*cs];
+          dU[U3] += mhd[a][b]*cn[((b*4 + 3)*4 + a)*cs];
         }
       }
     }

@@ -91,6 +91,7 @@ inline int make_params(const hgpu_config& c, Params& P) {
   P.metric_dim = (c.metric == 3) ? 3 : ((c.metric == 2) ? 2 : 0);
   P.flat = (c.metric == 0);
   P.dyn = (c.metric == 3);
+  P.nconn = (P.metric_dim == 3) ? (long long)c.n1 * c.n2 * c.n3 : ((P.metric_dim == 2) ? (long long)c.n1 * c.n2 : 1);
   P.recon = c.recon;
   P.flux = c.flux;
   P.ct_para = c.ct_para;

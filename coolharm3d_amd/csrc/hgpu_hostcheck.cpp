@@ -205,9 +205,11 @@ struct HostBE {
     for (int i = NG; i < NG + P.n1; i++)
       for (int j = NG; j < NG + P.n2; j++)
         for (int k = NG; k < NG + P.n3; k++) {
-          double Xc[4];
+          double Xc[4], cn[64];
           zone_X(P, c->pn, i, j, k, CENT, Xc);
-          pn_conn(c->pn.B, t, Xc[1], Xc[2], Xc[3], dst + (long long)conn_id(P, i, j, k) * 64);
+          pn_conn(c->pn.B, t, Xc[1], Xc[2], Xc[3], cn);
+          const long long cid = conn_id(P, i, j, k);
+          for (int l = 0; l < 64; l++) dst[l * P.nconn + cid] = cn[l];
         }
     return 0;
   }

@@ -448,7 +448,10 @@ __global__ __launch_bounds__(BLOCK) void k_fill_conn(Params P, Ptrs X, PnGrid G,
   const int i = NG + (int)(z / ((long long)P.n3 * P.n2));
   double Xc[4];
   zone_X(P, G, i, j, k, CENT, Xc);
-  pn_conn(G.B, level_time(X, tsel), Xc[1], Xc[2], Xc[3], dst + (long long)conn_id(P, i, j, k) * 64);
+  double cn[64];
+  pn_conn(G.B, level_time(X, tsel), Xc[1], Xc[2], Xc[3], cn);
+  const long long cid = conn_id(P, i, j, k);
+  for (int l = 0; l < 64; l++) dst[l * P.nconn + cid] = cn[l];   // structure-of-arrays levels
 }
 
 // Batched Utoprim_2d_fast() on independent zones (SoA buffers).

@@ -21,7 +21,9 @@
 //   coordinates X1 = log(r - R0), X2 in [0,1] with
 //   theta = th_length X2 + th_beg + h_slope sin(2 pi X2) (the modified
 //   Kerr-Schild map, src/coord.c:240-382), X3 = phi.  g in numerical
-//   coordinates is J^T g_harmonic J with J = d(t, x, y, z)/d(t, X1, X2, X3).
+//   coordinates is J^T g_harmonic J with J = d(t, x, y, z)/d(t, X1, X2, X3),
+//   which for this conformally flat spatial metric and orthogonal map has a
+//   closed form (pn_gcov_num()).
 // * Everything is templated on the scalar: double for values, Dual4 for values
 //   and the four partial derivatives d/d(t, X1, X2, X3) (forward-mode
 //   differentiation), from which the connection Gamma^a_bc, including the
@@ -134,9 +136,11 @@ HD void pn_orbit(const PnBinary& B, const T& t, T y1[3], T y2[3], T v1[3], T v2[
   v2[0] = f2 * vx; v2[1] = f2 * vy; v2[2] = t * 0.;
 }
 
-// 1PN near-zone metric in harmonic coordinates at Cartesian point x, time t.
+// Potentials of the 1PN near-zone metric at Cartesian point x, time t: the
+// Newtonian U, the 1PN V and the vector potential V_i, so that
+// g00 = -1 + 2V - 2U^2, g0i = -4 V_i, gij = (1 + 2U) delta_ij.
 template <class T>
-HD void pn_gcov_harmonic(const PnBinary& B, const T& t, const T x[3], T g[4][4]) {
+HD void pn_potentials(const PnBinary& B, const T& t, const T x[3], T& U, T& V, T Vi[3]) {
   T y[2][3], v[2][3];
   pn_orbit(B, t, y[0], y[1], v[0], v[1]);
   const double m[2] = {B.m1, B.m2};
@@ -150,7 +154,8 @@ HD void pn_gcov_harmonic(const PnBinary& B, const T& t, const T x[3], T g[4][4])
   const T ex = y[0][0] - y[1][0], ey = y[0][1] - y[1][1], ez = y[0][2] - y[1][2];
   const T r12 = sqrt(ex * ex + ey * ey + ez * ez);
   const T r12_3 = r12 * r12 * r12;
-  T U = t * 0., V = t * 0.;
+  U = t * 0.;
+  V = t * 0.;
   for (int a = 0; a < 2; a++) {
     const int b = 1 - a;
     const T mr = m[a] / r[a];
@@ -158,17 +163,29 @@ HD void pn_gcov_harmonic(const PnBinary& B, const T& t, const T x[3], T g[4][4])
     V = V + mr * (1. + 2. * vsq[a] - 0.5 * nv[a] * nv[a]) - m[a] * m[b] / (r[a] * r12)
           + (m[a] * m[b] / 4.) * (r[b] * r[b] - r[a] * r[a] - r12 * r12) / (r[a] * r12_3);
   }
-  g[0][0] = -1. + 2. * V - 2. * U * U;
   for (int i = 0; i < 3; i++) {
-    T Vi = t * 0.;
-    for (int a = 0; a < 2; a++) Vi = Vi + m[a] * v[a][i] / r[a];
-    g[0][i + 1] = g[i + 1][0] = -4. * Vi;
+    Vi[i] = t * 0.;
+    for (int a = 0; a < 2; a++) Vi[i] = Vi[i] + m[a] * v[a][i] / r[a];
   }
+}
+
+// 1PN near-zone metric in harmonic coordinates at Cartesian point x, time t.
+template <class T>
+HD void pn_gcov_harmonic(const PnBinary& B, const T& t, const T x[3], T g[4][4]) {
+  T U, V, Vi[3];
+  pn_potentials(B, t, x, U, V, Vi);
+  g[0][0] = -1. + 2. * V - 2. * U * U;
+  for (int i = 0; i < 3; i++) g[0][i + 1] = g[i + 1][0] = -4. * Vi[i];
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) g[i + 1][j + 1] = (i == j) ? 1. + 2. * U : t * 0.;
 }
 
 // Metric in the numerical coordinates (t, X1, X2, X3) of the spherical grid.
+// The spatial metric is conformally flat, (1 + 2U) delta_ij, and the map to
+// (r, theta, phi) is orthogonal, so J^T g J has the closed form
+//   g_tt = g00,  g_tn = -4 V_i J^i_n,
+//   g_11 = A r'^2,  g_22 = A (r theta')^2,  g_33 = A (r sin theta)^2,  A = 1 + 2U,
+// with no other spatial components (r' = dr/dX1, theta' = dtheta/dX2).
 template <class T>
 HD void pn_gcov_num(const PnBinary& B, const T& t, const T& X1, const T& X2, const T& X3, T g[4][4]) {
   const T er = exp(X1);
@@ -182,30 +199,24 @@ HD void pn_gcov_num(const PnBinary& B, const T& t, const T& X1, const T& X2, con
   T st = sin(th);
   const T ct = cos(th), sp = sin(X3), cp = cos(X3);
   if (fabs(value(st)) < 1.E-16) set_value(st, value(st) >= 0. ? 1.E-16 : -1.E-16);   // trigvalues(), src/metric.c:29-40
-  T x[3] = {r * st * cp, r * st * sp, r * ct};
-  T gh[4][4];
-  pn_gcov_harmonic(B, t, x, gh);
-  // J[mu][nu] = d x^mu / d X^nu (t row and column trivial)
-  T J[4][4];
-  for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) J[a][b] = t * 0.;
-  J[0][0] = t * 0. + 1.;
-  J[1][1] = st * cp * dr; J[1][2] = r * ct * cp * dth; J[1][3] = -r * st * sp;
-  J[2][1] = st * sp * dr; J[2][2] = r * ct * sp * dth; J[2][3] = r * st * cp;
-  J[3][1] = ct * dr;      J[3][2] = -r * st * dth;     J[3][3] = t * 0.;
-  T tmp[4][4];
-  for (int a = 0; a < 4; a++)
-    for (int n = 0; n < 4; n++) {
-      T s = t * 0.;
-      for (int b = 0; b < 4; b++) s = s + gh[a][b] * J[b][n];
-      tmp[a][n] = s;
-    }
-  for (int m2 = 0; m2 < 4; m2++)
-    for (int n = m2; n < 4; n++) {
-      T s = t * 0.;
-      for (int a = 0; a < 4; a++) s = s + J[a][m2] * tmp[a][n];
-      g[m2][n] = s;
-      g[n][m2] = s;
-    }
+  const T rs = r * st;
+  T x[3] = {rs * cp, rs * sp, r * ct};
+  T U, V, Vi[3];
+  pn_potentials(B, t, x, U, V, Vi);
+  // columns n = 1..3 of J = d(x, y, z)/d(X1, X2, X3)
+  const T J1[3] = {st * cp * dr, st * sp * dr, ct * dr};
+  const T rdth = r * dth;
+  const T J2[3] = {ct * cp * rdth, ct * sp * rdth, -st * rdth};
+  const T J3[3] = {-rs * sp, rs * cp, t * 0.};
+  const T A = 1. + 2. * U;
+  g[0][0] = -1. + 2. * V - 2. * U * U;
+  g[0][1] = g[1][0] = -4. * (Vi[0] * J1[0] + Vi[1] * J1[1] + Vi[2] * J1[2]);
+  g[0][2] = g[2][0] = -4. * (Vi[0] * J2[0] + Vi[1] * J2[1] + Vi[2] * J2[2]);
+  g[0][3] = g[3][0] = -4. * (Vi[0] * J3[0] + Vi[1] * J3[1]);
+  g[1][1] = A * dr * dr;
+  g[2][2] = A * rdth * rdth;
+  g[3][3] = A * rs * rs;
+  g[1][2] = g[2][1] = g[1][3] = g[3][1] = g[2][3] = g[3][2] = t * 0.;
 }
 
 // Inverse and determinant of a 4x4 matrix by the Laplace expansion in 2x2

@@ -98,6 +98,8 @@ struct Params {
   int track_nfail;         // keep nfail[15][NT] per-cell counters
   int mixed;               // 1: FP32 reconstruction, face fluxes and EMFs
   int dyn;                 // 1: time-dependent metric, geometry levels refilled every step
+  long long nconn;         // connection records (interior zones); with dyn the levels are
+                           // structure-of-arrays, component l of zone z at conn[l*nconn + z]
   double dx[4], invdx[4];
   double gam, gam_m1_o_gam;
   double cour;
