@@ -20,6 +20,7 @@ struct LaunchShape {
   unsigned remap = 1;    // XCD-contiguous tile order
   unsigned order = 0;    // 0: plane after plane; 1: x1 fastest (one (j,k) tile through consecutive planes)
   unsigned occ = 0;      // 1: the register-capped kernel variant (flux, conserved update)
+  unsigned fused = 0;    // conserved update only: 1 = update and inversion in one kernel
 };
 enum { FAM_FLUX1 = 0, FAM_FLUX2, FAM_FLUX3, FAM_EMF, FAM_CONS, FAM_INV, N_FAM };
 
@@ -267,6 +268,37 @@ __global__ __launch_bounds__(BLOCK) void k_invert(Params P, Ptrs X, int stage, B
       store_zone(P, X, stage, c, pf, pgam, fr);
     else
       queue = true;
+  }
+  const unsigned long long ballot = __ballot(queue);
+  if (ballot) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)ballot) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = (unsigned)atomicAdd(&X.status[ST_SLOW], (unsigned long long)__popcll(ballot));
+    base = __shfl(base, leader, 64);
+    if (queue) X.slow_idx[base + (unsigned)__popcll(ballot & ((1ull << lane) - 1ull))] = (unsigned)c;
+  }
+  commit_fail(P, X, fr, active, c, MASK_ALL);
+}
+
+// Conserved update and the common con-to-prim branch in one pass: U^{n+1}
+// stays in registers; only the zones queued for k_recover store it.
+template <bool MIXED, int W>
+__global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage, Box b, Tiling T) {
+  int i, j, k;
+  const bool active = tile_index(b, T, i, j, k);
+  FailRec fr; fr.clear();
+  long long c = 0;
+  bool queue = false;
+  if (active) {
+    const double dt = X.scal[SC_DT];
+    c = cidx(P, i, j, k);
+    double U[NP];
+    if (!update_zone_fast<MIXED>(P, X, stage, dt, i, j, k, fr, U)) {
+      queue = true;
+#pragma unroll
+      for (int l = 0; l < NP; l++) X.Uslow[l*P.nt + c] = U[l];
+    }
   }
   const unsigned long long ballot = __ballot(queue);
   if (ballot) {
@@ -636,7 +668,8 @@ struct GpuBE {
   static int check_shape(const LaunchShape& L) {
     if (L.bs != 64 && L.bs != 128 && L.bs != 256) return set_error(-2, "workgroup size must be 64, 128 or 256");
     if (L.kc != 0 && (L.kc > L.bs || L.bs % L.kc != 0)) return set_error(-2, "tile k-extent must divide the workgroup size");
-    if (L.remap > 1 || L.order > 1 || L.occ > 1) return set_error(-2, "remap, order and occ must be 0 or 1");
+    if (L.remap > 1 || L.order > 1 || L.occ > 1 || L.fused > 1)
+      return set_error(-2, "remap, order, occ and fused must be 0 or 1");
     return 0;
   }
   static int set_shape(hgpu_ctx* c, int fam, int bs, int kc, int flags) {
@@ -644,7 +677,9 @@ struct GpuBE {
     LaunchShape L;
     L.bs = (unsigned)bs; L.kc = (unsigned)kc; L.remap = (unsigned)flags & 1u; L.order = ((unsigned)flags >> 1) & 1u;
     L.occ = ((unsigned)flags >> 2) & 1u;
-    if (bs < 0 || kc < 0 || flags < 0 || flags > 7) return set_error(-2, "bad launch-shape field");
+    L.fused = ((unsigned)flags >> 3) & 1u;
+    if (bs < 0 || kc < 0 || flags < 0 || flags > 15) return set_error(-2, "bad launch-shape field");
+    if (L.fused && fam != FAM_CONS) return set_error(-2, "flag bit 3 (fused update) applies to the conserved update only");
     if (int rc = check_shape(L)) return rc;
     int rc = invalidate_graph(c);
     priv(c)->shape[fam] = L;
@@ -655,7 +690,7 @@ struct GpuBE {
     const LaunchShape& L = priv(c)->shape[fam];
     if (bs) *bs = (int)L.bs;
     if (kc) *kc = (int)L.kc;
-    if (remap) *remap = (int)(L.remap | (L.order << 1) | (L.occ << 2));
+    if (remap) *remap = (int)(L.remap | (L.order << 1) | (L.occ << 2) | (L.fused << 3));
     return 0;
   }
   static void launch_conserved(hgpu_ctx* c, const LaunchShape& L, const Tiling& T, const Box& b, const Ptrs& X,
@@ -670,17 +705,24 @@ struct GpuBE {
       else hipLaunchKernelGGL((k_conserved<false, 1>), grid, blk, 0, st, c->P, X, stage, b, T);
     }
   }
-  static int conserved_only(hgpu_ctx* c) {
-    Box b = interior_box(c->P);
-    const Tiling T = make_tiling(b, priv(c)->shape[FAM_CONS]);
+  static void launch_fused(hgpu_ctx* c, const LaunchShape& L, const Tiling& T, const Box& b, int stage) {
     hipStream_t st = priv(c)->stream;
-    launch_conserved(c, priv(c)->shape[FAM_CONS], T, b, c->X, 0);
-    return hip_check(hipGetLastError(), "k_conserved");
+    const dim3 grid(T.nblocks), blk(T.bs);
+    if (c->P.mixed) {
+      if (L.occ) hipLaunchKernelGGL((k_update<true, 4>), grid, blk, 0, st, c->P, c->X, stage, b, T);
+      else hipLaunchKernelGGL((k_update<true, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T);
+    } else {
+      if (L.occ) hipLaunchKernelGGL((k_update<false, 4>), grid, blk, 0, st, c->P, c->X, stage, b, T);
+      else hipLaunchKernelGGL((k_update<false, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T);
+    }
   }
+  // The autotuner times the conserved-update family as the whole predictor
+  // update (split pair or fused kernel), which writes only arrays that the
+  // next step rewrites before reading (ph, U^n, gamma, the slow-zone queue).
   static int launch_family(hgpu_ctx* c, int fam) {
     if (fam <= FAM_FLUX3) return flux(c, c->X.p, fam - FAM_FLUX1);
     if (fam == FAM_EMF) return emf(c);
-    return conserved_only(c);
+    return update(c, 0);
   }
   static int autotune(hgpu_ctx* c, int reps, double* report) {
     GpuPriv* g = priv(c);
@@ -732,10 +774,27 @@ struct GpuBE {
         const float ms = time_shape(L);
         if (!rc && ms < best_ms) { best_ms = ms; best = L; }
       }
+      if (!rc && fam == FAM_CONS) {  // the fused update over the same tile grid
+        for (unsigned bs : bss) for (unsigned kc : kcs) for (unsigned flags = 0; flags < 4; flags++) {
+          LaunchShape L;
+          L.bs = bs; L.kc = kc; L.remap = flags & 1u; L.order = flags >> 1; L.fused = 1;
+          if (check_shape(L)) { g_last_error.clear(); continue; }
+          const float ms = time_shape(L);
+          if (rc) break;
+          if (ms < best_ms) { best_ms = ms; best = L; }
+        }
+        if (!rc && best.fused) {
+          LaunchShape L = best;
+          L.occ = 1;
+          const float ms = time_shape(L);
+          if (!rc && ms < best_ms) { best_ms = ms; best = L; }
+        }
+      }
       g->shape[fam] = best;
       if (report) {
         report[fam*4 + 0] = best.bs; report[fam*4 + 1] = best.kc;
-        report[fam*4 + 2] = best.remap | (best.order << 1) | (best.occ << 2); report[fam*4 + 3] = best_ms;
+        report[fam*4 + 2] = best.remap | (best.order << 1) | (best.occ << 2) | (best.fused << 3);
+        report[fam*4 + 3] = best_ms;
       }
     }
     g->pool.push_back(e0);
@@ -815,6 +874,10 @@ struct GpuBE {
     const Tiling T2 = make_tiling(b, priv(c)->shape[FAM_INV]);
     hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_SLOW, 1);
     hipStream_t st = priv(c)->stream;
+    if (priv(c)->shape[FAM_CONS].fused) {
+      launch_fused(c, priv(c)->shape[FAM_CONS], T, b, stage);
+      return hip_check(hipGetLastError(), "k_update");
+    }
     launch_conserved(c, priv(c)->shape[FAM_CONS], T, b, c->X, stage);
     Ptrs Xn = c->X;
     Xn.geom = c->X.geom_new;   // inversion at the result's time level

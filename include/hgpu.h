@@ -105,9 +105,12 @@ int hgpu_set_graph(hgpu_ctx *ctx, int on);
  * x3; 3: EMFs; 4: conserved update; 5: inversion): workgroup size bs (64, 128,
  * 256), tile k-extent kc (0 = bs consecutive (j,k) points, else kc x bs/kc
  * tiles; kc divides bs), flags bit 0: XCD-contiguous tile order, bit 1: x1
- * fastest (a (j,k) tile through consecutive x1-planes).
+ * fastest (a (j,k) tile through consecutive x1-planes), bit 2: the
+ * register-capped kernel variant (families 0-2 and 4), bit 3 (family 4 only):
+ * the conserved update and the inversion in one kernel.
  * hgpu_autotune() times every candidate of families 0-4 on the current state
- * (reps launches each) and keeps the fastest; report (NULL or [5][4]) gets
+ * (reps launches each; family 4 as the whole predictor update, split or fused)
+ * and keeps the fastest; report (NULL or [5][4]) gets
  * bs, kc, flags, ms per launch.  The state, flags and counters are unchanged;
  * results do not depend on the shapes.  No counterpart in the reference. */
 int hgpu_autotune(hgpu_ctx *ctx, int reps, double *report);

@@ -117,16 +117,18 @@ def test_mixed_precision_error_small_torus():
     assert np.all(l1 <= 1e-3)
 
 
-def test_launch_shapes_and_autotune_keep_results():
-    """Workgroup size, tile shape and tile order change nothing in the results;
-    the autotuner leaves state, flags and counters as they were."""
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_launch_shapes_and_autotune_keep_results(precision):
+    """Workgroup size, tile shape, tile order, the register-capped variants and
+    the fused update change nothing in the results; the autotuner leaves
+    state, flags and counters as they were."""
     from coolharm3d_amd import Harm3D, torus_grid, torus
     g = torus_grid(40, 32, 24)
     p0 = torus(g)
     out = []
-    for mode in ("default", "shapes", "autotune"):
-        print(f"[shapes] {mode}", flush=True)
-        H = Harm3D(g, track_nfail=1)
+    for mode in ("default", "shapes", "fused", "fused_capped", "autotune"):
+        print(f"[shapes] {precision} {mode}", flush=True)
+        H = Harm3D(g, track_nfail=1, precision=precision)
         H.set_prims(p0)
         H.fixup_bounds()
         H.set_time(0.0, 1.0e-5)
@@ -136,6 +138,12 @@ def test_launch_shapes_and_autotune_keep_results():
                                                          (64, 0, 1), (128, 16, 2), (64, 32, 3)]):
                 H.set_launch_shape(fam, bs, kc, remap)
                 assert H.launch_shape(fam) == (bs, kc, remap)
+        if mode.startswith("fused"):
+            cons = (256, 0, 8) if mode == "fused" else (128, 32, 15)
+            for fam, (bs, kc, flags) in zip(H.FAMILIES, [(256, 0, 4), (256, 0, 5), (128, 0, 6),
+                                                         (256, 0, 1), cons, (256, 0, 1)]):
+                H.set_launch_shape(fam, bs, kc, flags)
+                assert H.launch_shape(fam) == (bs, kc, flags)
         if mode == "autotune":
             fe0, v0 = H.status()
             before = (H.prims(), H.pflag(), H.nfail(), H.nfailtot(), v0)
