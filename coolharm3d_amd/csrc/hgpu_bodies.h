@@ -17,10 +17,9 @@ namespace hgpu {
 // template parameter so every 4-vector component the face selects (u^dir,
 // b^dir, g^{t dir}) is a compile-time index and stays in registers.
 template <int d>
-HD void flux_face(const Params& P, const Ptrs& X, const double* __restrict__ src,
-                  int i, int j, int k, FailRec& fr) {
+HD void flux_recon(const Params& P, const Ptrs& X, const double* __restrict__ src,
+                   int i, int j, int k, double* pl, double* pr) {
   const long long NT = P.nt;
-  double pl[NP], pr[NP];
   constexpr int di = (d == 0), dj = (d == 1), dk = (d == 2);
 
   if (P.degen[d]) {
@@ -86,7 +85,71 @@ HD void flux_face(const Params& P, const Ptrs& X, const double* __restrict__ src
       }
     }
   }
+}
 
+// PPM+MC face states of F consecutive faces along d starting at face (i,j,k),
+// each zone's limited parabola computed once (para_vect(), src/recon.c:994-1043,
+// evaluated per zone as the reference does; flux_recon() evaluates both
+// zones of one face).  Direction d must not be degenerate.  Zones past the
+// array end, which only faces beyond a partial group read, are clamped.
+template <int d, int F>
+HD void ppm_faces(const Params& P, const Ptrs& X, const double* __restrict__ src,
+                  int i, int j, int k, double (*pl)[NP], double (*pr)[NP]) {
+  const long long NT = P.nt;
+  constexpr int di = (d == 0), dj = (d == 1), dk = (d == 2);
+  constexpr int M = F + 5;
+  int ii = i, jj = j, kk = k;
+  if (d != 0 && P.degen[0]) ii = NG;
+  if (d != 1 && P.degen[1]) jj = NG;
+  if (d != 2 && P.degen[2]) kk = NG;
+  const int pos = (d == 0) ? ii : ((d == 1) ? jj : kk);
+  const int nd = (d == 0) ? P.n1t : ((d == 1) ? P.n2t : P.n3t);
+  const long long st = cidx(P, di, dj, dk) - cidx(P, 0, 0, 0);
+  const long long c0 = cidx(P, ii - 3*di, jj - 3*dj, kk - 3*dk);
+  int zm[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) zm[m] = imin(m, nd - 1 - (pos - 3));
+#pragma unroll
+  for (int v = 0; v < NP; v++) {
+    double y[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+      double x = src[v*NT + c0 + zm[m]*st];
+      if (d == 0) {
+        if (P.rescale_r) x *= X.rscale[rs_slot(v)*P.n1t + (ii - 3 + zm[m])];
+      } else {
+        x = rt_val(P, X, v, ii, x);
+      }
+      y[m] = x;
+    }
+    double dq[M - 1], fi[M - 2], le[M - 2], re[M - 2];
+#pragma unroll
+    for (int m = 1; m < M - 1; m++) dq[m] = mc_slope_vect(y[m - 1], y[m], y[m + 1]);
+#pragma unroll
+    for (int m = 1; m < M - 2; m++) fi[m] = 0.5*(y[m + 1] + y[m]) - PPM_FACTOR*(dq[m + 1] - dq[m]);
+#pragma unroll
+    for (int m = 2; m < M - 2; m++) ppm_limit(fi[m - 1], fi[m], y[m], &le[m], &re[m]);
+#pragma unroll
+    for (int f = 0; f < F; f++) {
+      double a = re[2 + f], b = le[3 + f];
+      if (d == 0 && P.rescale_r) {
+        const double fac = X.irscale[(3 + rs_slot(v))*P.n1t + imin(i + f, P.n1t - 1)];
+        a *= fac;
+        b *= fac;
+      }
+      pl[f][v] = a;
+      pr[f][v] = b;
+    }
+  }
+}
+
+// Riemann solve of face (i,j,k) from its two states: fluxes, the entropy flux
+// and the fastest interior speed (the rest of numerical_flux(),
+// src/step_ch.c:681-764).
+template <int d>
+HD void flux_riemann(const Params& P, const Ptrs& X, const double* pl, const double* pr,
+                     int i, int j, int k, FailRec& fr) {
+  const long long NT = P.nt;
   const Geom* g = geom_at(P, X, i, j, k, d);
   constexpr int dim = d + 1;
   State ql, qr;
@@ -107,7 +170,7 @@ HD void flux_face(const Params& P, const Ptrs& X, const double* __restrict__ src
   // fastest speed over interior faces (timestep(), src/step_ch.c:716-724),
   // reduced over the launch by the caller
   if (i >= NG && i < NG + P.n1 && j >= NG && j < NG + P.n2 && k >= NG && k < NG + P.n3 && ctop > 0.)
-    fr.vmax = ctop;
+    fr.vmax = dmax(fr.vmax, ctop);
 
   const long long c = cidx(P, i, j, k);
   double* F = X.F;
@@ -133,6 +196,14 @@ HD void flux_face(const Params& P, const Ptrs& X, const double* __restrict__ src
     else Fs = 0.5*(Gl + Gr - ctop*(Sr - Sl));
     F[(d*NFLUX + 8)*NT + c] = Fs;
   }
+}
+
+template <int d>
+HD void flux_face(const Params& P, const Ptrs& X, const double* __restrict__ src,
+                  int i, int j, int k, FailRec& fr) {
+  double pl[NP], pr[NP];
+  flux_recon<d>(P, X, src, i, j, k, pl, pr);
+  flux_riemann<d>(P, X, pl, pr, i, j, k, fr);
 }
 
 // --------------------------------------------------------------------------

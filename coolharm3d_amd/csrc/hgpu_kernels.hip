@@ -8,6 +8,7 @@
 #include "hgpu_core.h"
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <type_traits>
 
 namespace hgpu {
 
@@ -21,6 +22,7 @@ struct LaunchShape {
   unsigned order = 0;    // 0: plane after plane; 1: x1 fastest (one (j,k) tile through consecutive planes)
   unsigned occ = 0;      // 1: the register-capped kernel variant (flux, conserved update)
   unsigned fused = 0;    // conserved update only: 1 = update and inversion in one kernel
+  unsigned fpt = 1;      // face fluxes only: faces per thread along the direction (1, 2, 4)
 };
 enum { FAM_FLUX1 = 0, FAM_FLUX2, FAM_FLUX3, FAM_EMF, FAM_CONS, FAM_INV, N_FAM };
 
@@ -64,7 +66,8 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 // reduced in LDS so each status word receives at most one atomic per
 // workgroup.  Every thread of the block must call it.
 __device__ __forceinline__ void commit_fail(const Params& P, const Ptrs& X, const FailRec& fr,
-                                            bool active, long long c, unsigned mask, int vmax_slot = -1) {
+                                            bool active, long long c, unsigned mask, int vmax_slot = -1,
+                                            bool zone_map = true) {
   __shared__ unsigned s_cnt[N_NFAIL];
   __shared__ unsigned s_flags;
   __shared__ unsigned long long s_vmax;
@@ -78,7 +81,7 @@ __device__ __forceinline__ void commit_fail(const Params& P, const Ptrs& X, cons
       if (!(mask & (1u << l))) continue;
       const int v = fr.nf[l];
       if (v) {
-        if (P.track_nfail) X.nfail[l*NT + c] += v;
+        if (P.track_nfail && zone_map) X.nfail[l*NT + c] += v;
         atomicAdd(&s_cnt[l], (unsigned)v);
       }
     }
@@ -213,6 +216,81 @@ __global__ __launch_bounds__(BLOCK, W) void k_flux_mixed(Params P, Ptrs X, const
     c = cidx(P, i, j, k);
   }
   commit_fail(P, X, fr, active, c, MASK_GAMMA, D + 1);
+}
+
+// F consecutive faces along D per thread (PPM+MC, D not degenerate): the
+// thread's tile index along D counts groups of F faces from bg.lo[D]; faces
+// past hi_d (the flux box end) are skipped.  Each zone's parabola is computed
+// once per group.  The per-face fail() tally goes to the face's zone here,
+// the block totals through commit_fail().
+template <int D, int F, int W, bool MIXED>
+__global__ __launch_bounds__(BLOCK, W) void k_flux_multi(Params P, Ptrs X, const double* __restrict__ src, Box bg,
+                                                         Tiling T, int hi_d) {
+  int i, j, k;
+  const bool active = tile_index(bg, T, i, j, k);
+  FailRec fr; fr.clear();
+  if (active) {
+    constexpr int di = (D == 0), dj = (D == 1), dk = (D == 2);
+    const int lo = bg.lo[D];
+    if (D == 0) i = lo + (i - lo) * F;
+    else if (D == 1) j = lo + (j - lo) * F;
+    else k = lo + (k - lo) * F;
+    const int nvalid = min(F, hi_d - ((D == 0) ? i : ((D == 1) ? j : k)) + 1);
+    using Real = typename std::conditional<MIXED, float, double>::type;
+    Real pl[F][NP], pr[F][NP];
+    if constexpr (MIXED) ppm_faces_mixed<D, F>(P, X, src, i, j, k, pl, pr);
+    else ppm_faces<D, F>(P, X, src, i, j, k, pl, pr);
+#pragma unroll
+    for (int f = 0; f < F; f++) {
+      if (f >= nvalid) break;
+      const int g0 = fr.nf[NFAIL_GAMMA_CALC];
+      const int fi = i + f*di, fj = j + f*dj, fk = k + f*dk;
+      if constexpr (MIXED) flux_riemann_mixed<D>(P, X, pl[f], pr[f], fi, fj, fk, fr);
+      else flux_riemann<D>(P, X, pl[f], pr[f], fi, fj, fk, fr);
+      const int dv = fr.nf[NFAIL_GAMMA_CALC] - g0;
+      if (P.track_nfail && dv) X.nfail[NFAIL_GAMMA_CALC*P.nt + cidx(P, fi, fj, fk)] += dv;
+    }
+  }
+  commit_fail(P, X, fr, active, 0, MASK_GAMMA, D + 1, false);
+}
+
+// Face fluxes along D with launch shape L: one face per thread, or fpt faces
+// per thread (PPM+MC with D not degenerate) over the box of face groups.
+template <int D, int F, bool MIXED>
+static void launch_flux_multi(const Params& P, const Ptrs& X, const LaunchShape& L, const double* src, const Box& b,
+                              hipStream_t st) {
+  Box bg = b;
+  const int n = b.hi[D] - b.lo[D] + 1;
+  bg.hi[D] = b.lo[D] + (n + F - 1) / F - 1;
+  const Tiling T = make_tiling(bg, L);
+  const dim3 grid(T.nblocks), blk(T.bs);
+  // the register-capped variant exists for the FP32 solve (the FP64 one spills)
+  if (MIXED && L.occ) hipLaunchKernelGGL((k_flux_multi<D, F, 4, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D]);
+  else hipLaunchKernelGGL((k_flux_multi<D, F, 1, MIXED>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D]);
+}
+
+template <int D>
+static void launch_flux_dir(const Params& P, const Ptrs& X, const LaunchShape& L, const double* src, const Box& b,
+                            hipStream_t st) {
+  if (L.fpt > 1 && P.recon == RECON_PPM_MC && !P.degen[D]) {
+    if (P.mixed) {
+      if (L.fpt == 2) launch_flux_multi<D, 2, true>(P, X, L, src, b, st);
+      else launch_flux_multi<D, 4, true>(P, X, L, src, b, st);
+    } else {
+      if (L.fpt == 2) launch_flux_multi<D, 2, false>(P, X, L, src, b, st);
+      else launch_flux_multi<D, 4, false>(P, X, L, src, b, st);
+    }
+    return;
+  }
+  const Tiling T = make_tiling(b, L);
+  const dim3 grid(T.nblocks), blk(T.bs);
+  if (P.mixed) {
+    if (L.occ) hipLaunchKernelGGL((k_flux_mixed<D, 6>), grid, blk, 0, st, P, X, src, b, T);
+    else hipLaunchKernelGGL((k_flux_mixed<D, 1>), grid, blk, 0, st, P, X, src, b, T);
+  } else {
+    if (L.occ) hipLaunchKernelGGL((k_flux<D, 4>), grid, blk, 0, st, P, X, src, b, T);
+    else hipLaunchKernelGGL((k_flux<D, 1>), grid, blk, 0, st, P, X, src, b, T);
+  }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_emf(Params P, Ptrs X, Box b, Tiling T) {
@@ -665,11 +743,16 @@ struct GpuBE {
   // scratch, which every substep recomputes before use; the failure words and
   // counters they also touch are saved before and restored after.  The
   // inversion keeps its own shape (it rewrites the state) unless set.
+  static unsigned fpt_code(unsigned fpt) { return fpt == 4 ? 2u : (fpt == 2 ? 1u : 0u); }
+  static unsigned shape_flags(const LaunchShape& L) {
+    return L.remap | (L.order << 1) | (L.occ << 2) | (L.fused << 3) | (fpt_code(L.fpt) << 4);
+  }
   static int check_shape(const LaunchShape& L) {
     if (L.bs != 64 && L.bs != 128 && L.bs != 256) return set_error(-2, "workgroup size must be 64, 128 or 256");
     if (L.kc != 0 && (L.kc > L.bs || L.bs % L.kc != 0)) return set_error(-2, "tile k-extent must divide the workgroup size");
     if (L.remap > 1 || L.order > 1 || L.occ > 1 || L.fused > 1)
       return set_error(-2, "remap, order, occ and fused must be 0 or 1");
+    if (L.fpt != 1 && L.fpt != 2 && L.fpt != 4) return set_error(-2, "faces per thread must be 1, 2 or 4");
     return 0;
   }
   static int set_shape(hgpu_ctx* c, int fam, int bs, int kc, int flags) {
@@ -678,8 +761,11 @@ struct GpuBE {
     L.bs = (unsigned)bs; L.kc = (unsigned)kc; L.remap = (unsigned)flags & 1u; L.order = ((unsigned)flags >> 1) & 1u;
     L.occ = ((unsigned)flags >> 2) & 1u;
     L.fused = ((unsigned)flags >> 3) & 1u;
-    if (bs < 0 || kc < 0 || flags < 0 || flags > 15) return set_error(-2, "bad launch-shape field");
+    const unsigned fcode = ((unsigned)flags >> 4) & 3u;
+    L.fpt = 1u << fcode;
+    if (bs < 0 || kc < 0 || flags < 0 || flags > 63 || fcode > 2) return set_error(-2, "bad launch-shape field");
     if (L.fused && fam != FAM_CONS) return set_error(-2, "flag bit 3 (fused update) applies to the conserved update only");
+    if (L.fpt > 1 && fam > FAM_FLUX3) return set_error(-2, "flag bits 4-5 (faces per thread) apply to the face fluxes only");
     if (int rc = check_shape(L)) return rc;
     int rc = invalidate_graph(c);
     priv(c)->shape[fam] = L;
@@ -690,7 +776,7 @@ struct GpuBE {
     const LaunchShape& L = priv(c)->shape[fam];
     if (bs) *bs = (int)L.bs;
     if (kc) *kc = (int)L.kc;
-    if (remap) *remap = (int)(L.remap | (L.order << 1) | (L.occ << 2) | (L.fused << 3));
+    if (remap) *remap = (int)(L.remap | (L.order << 1) | (L.occ << 2) | (L.fused << 3) | (fpt_code(L.fpt) << 4));
     return 0;
   }
   static void launch_conserved(hgpu_ctx* c, const LaunchShape& L, const Tiling& T, const Box& b, const Ptrs& X,
@@ -760,9 +846,12 @@ struct GpuBE {
         rc |= hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
         return rc ? 1e30f : ms / (float)reps;
       };
+      const bool multi = fam <= FAM_FLUX3 && c->P.recon == RECON_PPM_MC && !c->P.degen[fam - FAM_FLUX1];
+      const unsigned nfpt = multi ? 3u : 1u;
+      for (unsigned fc = 0; fc < nfpt; fc++)
       for (unsigned bs : bss) for (unsigned kc : kcs) for (unsigned flags = 0; flags < 4; flags++) {
         LaunchShape L;
-        L.bs = bs; L.kc = kc; L.remap = flags & 1u; L.order = flags >> 1;
+        L.bs = bs; L.kc = kc; L.remap = flags & 1u; L.order = flags >> 1; L.fpt = 1u << fc;
         if (check_shape(L)) { g_last_error.clear(); continue; }
         const float ms = time_shape(L);
         if (rc) break;
@@ -793,7 +882,7 @@ struct GpuBE {
       g->shape[fam] = best;
       if (report) {
         report[fam*4 + 0] = best.bs; report[fam*4 + 1] = best.kc;
-        report[fam*4 + 2] = best.remap | (best.order << 1) | (best.occ << 2) | (best.fused << 3);
+        report[fam*4 + 2] = shape_flags(best);
         report[fam*4 + 3] = best_ms;
       }
     }
@@ -832,31 +921,11 @@ struct GpuBE {
   static int flux(hgpu_ctx* c, const double* src, int d) {
     Box b = flux_box(c->P, d);
     Scope s(c, "flux");
-    const Tiling T = make_tiling(b, priv(c)->shape[FAM_FLUX1 + d]);
-    const dim3 grid(T.nblocks), blk(T.bs);
+    const LaunchShape& L = priv(c)->shape[FAM_FLUX1 + d];
     hipStream_t st = priv(c)->stream;
-    const bool occ = priv(c)->shape[FAM_FLUX1 + d].occ != 0;
-    if (c->P.mixed) {
-      if (occ) {
-        if (d == 0) hipLaunchKernelGGL((k_flux_mixed<0, 6>), grid, blk, 0, st, c->P, c->X, src, b, T);
-        else if (d == 1) hipLaunchKernelGGL((k_flux_mixed<1, 6>), grid, blk, 0, st, c->P, c->X, src, b, T);
-        else hipLaunchKernelGGL((k_flux_mixed<2, 6>), grid, blk, 0, st, c->P, c->X, src, b, T);
-      } else {
-        if (d == 0) hipLaunchKernelGGL((k_flux_mixed<0, 1>), grid, blk, 0, st, c->P, c->X, src, b, T);
-        else if (d == 1) hipLaunchKernelGGL((k_flux_mixed<1, 1>), grid, blk, 0, st, c->P, c->X, src, b, T);
-        else hipLaunchKernelGGL((k_flux_mixed<2, 1>), grid, blk, 0, st, c->P, c->X, src, b, T);
-      }
-    } else {
-      if (occ) {
-        if (d == 0) hipLaunchKernelGGL((k_flux<0, 4>), grid, blk, 0, st, c->P, c->X, src, b, T);
-        else if (d == 1) hipLaunchKernelGGL((k_flux<1, 4>), grid, blk, 0, st, c->P, c->X, src, b, T);
-        else hipLaunchKernelGGL((k_flux<2, 4>), grid, blk, 0, st, c->P, c->X, src, b, T);
-      } else {
-        if (d == 0) hipLaunchKernelGGL((k_flux<0, 1>), grid, blk, 0, st, c->P, c->X, src, b, T);
-        else if (d == 1) hipLaunchKernelGGL((k_flux<1, 1>), grid, blk, 0, st, c->P, c->X, src, b, T);
-        else hipLaunchKernelGGL((k_flux<2, 1>), grid, blk, 0, st, c->P, c->X, src, b, T);
-      }
-    }
+    if (d == 0) launch_flux_dir<0>(c->P, c->X, L, src, b, st);
+    else if (d == 1) launch_flux_dir<1>(c->P, c->X, L, src, b, st);
+    else launch_flux_dir<2>(c->P, c->X, L, src, b, st);
     return hip_check(hipGetLastError(), "k_flux");
   }
   static int emf(hgpu_ctx* c) {

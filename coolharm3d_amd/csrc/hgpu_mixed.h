@@ -21,6 +21,34 @@
 namespace hgpu {
 
 // ---------------------------------------------------------------------------
+// FP32 reciprocal, square root and power with the hardware's single-instruction
+// forms on the device (v_rcp_f32, v_sqrt_f32: about 1 ulp; v_log_f32 and
+// v_exp_f32 for x^a = 2^(a log2 x), relative error about
+// |a log2 x| 2^-23 ln 2).  The FP32 face solve's own rounding is of the same
+// order; the host build uses the libm forms.
+HD float rcp_f(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.f/x;
+#endif
+}
+HD float sqrt_f(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+HD float pow_f(float x, float a) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_exp2f(a*__builtin_amdgcn_logf(x));
+#else
+  return powf(x, a);
+#endif
+}
+
+// ---------------------------------------------------------------------------
 // FP32 restatements of get_state()/primtoflux()/vchar_fast() (src/phys.c)
 struct StateF {
   float ucon[4], ucov[4], bcon[4], bcov[4];
@@ -40,7 +68,7 @@ HD int gamma_calc_f(const float* pr, const GeomF* g, float* gamma, float* qsq_ou
     if (fabsf(qsq) > 1.E-10f) { *gamma = 1.f; *qsq_out = 0.f; return 1; }
     qsq = 1.E-10f;
   }
-  *gamma = sqrtf(1.f + qsq);
+  *gamma = sqrt_f(1.f + qsq);
   *qsq_out = qsq;
   return 0;
 }
@@ -48,14 +76,14 @@ HD int gamma_calc_f(const float* pr, const GeomF* g, float* gamma, float* qsq_ou
 HD int get_state_f(const float* pr, const GeomF* g, float gam, StateF* q) {
   float gamma, qsq;
   int fl = gamma_calc_f(pr, g, &gamma, &qsq);
-  q->opu_t = gamma*g->om_alpha - qsq/(gamma + 1.f)
+  q->opu_t = gamma*g->om_alpha - qsq*rcp_f(gamma + 1.f)
            + (g->gcov[0][1]*pr[U1] + g->gcov[0][2]*pr[U2] + g->gcov[0][3]*pr[U3]);
   q->ucon[0] = gamma*g->ncon[0];
   q->ucon[1] = pr[U1] - q->ucon[0]*g->beta[0];
   q->ucon[2] = pr[U2] - q->ucon[0]*g->beta[1];
   q->ucon[3] = pr[U3] - q->ucon[0]*g->beta[2];
   lower_f(q->ucon, g, q->ucov);
-  float inv_ut = 1.f/q->ucon[0];
+  float inv_ut = rcp_f(q->ucon[0]);
   q->bcon[0] = pr[B1]*q->ucov[1] + pr[B2]*q->ucov[2] + pr[B3]*q->ucov[3];
   q->bcon[1] = (pr[B1] + q->bcon[0]*q->ucon[1])*inv_ut;
   q->bcon[2] = (pr[B2] + q->bcon[0]*q->ucon[2])*inv_ut;
@@ -91,7 +119,7 @@ HD void primtoflux_f(const float* pr, const StateF* q, const GeomF* g, float* fl
 
 template <int js>
 HD void vchar_f(const StateF* q, const GeomF* g, float gam, float* vmax, float* vmin) {
-  float cms2 = (gam*q->p + q->bsq)/q->eta;
+  float cms2 = (gam*q->p + q->bsq)*rcp_f(q->eta);
   if (cms2 < 0.f) cms2 = (float)SMALL;
   else if (cms2 > 1.f) cms2 = 1.f;
   const float Au2 = q->ucon[js]*q->ucon[js], Bu2 = q->ucon[0]*q->ucon[0], AuBu = q->ucon[js]*q->ucon[0];
@@ -100,8 +128,8 @@ HD void vchar_f(const StateF* q, const GeomF* g, float gam, float* vmax, float* 
   const float C = Au2 - (g->gcon[js][js] + Au2)*cms2;
   float discr = B*B - A*C;
   if (discr < 0.f) discr = 0.f;
-  discr = sqrtf(discr);
-  const float denom = 1.f/A;
+  discr = sqrt_f(discr);
+  const float denom = rcp_f(A);
   const float vp = -(-B + discr)*denom, vm = -(-B - discr)*denom;
   if (vp > vm) { *vmax = vp; *vmin = vm; }
   else { *vmax = vm; *vmin = vp; }
@@ -161,11 +189,10 @@ HD float mc_linear_diff(float a, float b) {
 // ---------------------------------------------------------------------------
 // Face flux in mixed precision (the FP64 flux_face<d> in FP32)
 template <int d>
-HD void flux_face_mixed(const Params& P, const Ptrs& X, const double* __restrict__ src,
-                        int i, int j, int k, FailRec& fr) {
+HD void flux_recon_mixed(const Params& P, const Ptrs& X, const double* __restrict__ src,
+                         int i, int j, int k, float* pl, float* pr) {
   const long long NT = P.nt;
   constexpr int di = (d == 0), dj = (d == 1), dk = (d == 2);
-  float pl[NP], pr[NP];
   if (P.degen[d]) {
     int ii = i, jj = j, kk = k;
     if (d == 0) { ii = NG; if (P.degen[1]) jj = NG; if (P.degen[2]) kk = NG; }
@@ -218,7 +245,67 @@ HD void flux_face_mixed(const Params& P, const Ptrs& X, const double* __restrict
       }
     }
   }
+}
 
+// PPM+MC states of F consecutive faces along d from face (i,j,k), each zone's
+// limited parabola computed once (ppm_offsets() split per zone).  Direction d
+// must not be degenerate; zones past the array end are clamped (they feed
+// only faces beyond a partial group).
+template <int d, int F>
+HD void ppm_faces_mixed(const Params& P, const Ptrs& X, const double* __restrict__ src,
+                        int i, int j, int k, float (*pl)[NP], float (*pr)[NP]) {
+  const long long NT = P.nt;
+  constexpr int di = (d == 0), dj = (d == 1), dk = (d == 2);
+  constexpr int M = F + 5;
+  const float K = (float)PPM_FACTOR;
+  int ii = i, jj = j, kk = k;
+  if (d != 0 && P.degen[0]) ii = NG;
+  if (d != 1 && P.degen[1]) jj = NG;
+  if (d != 2 && P.degen[2]) kk = NG;
+  const int pos = (d == 0) ? ii : ((d == 1) ? jj : kk);
+  const int nd = (d == 0) ? P.n1t : ((d == 1) ? P.n2t : P.n3t);
+  const long long st = cidx(P, di, dj, dk) - cidx(P, 0, 0, 0);
+  const long long c0 = cidx(P, ii - 3*di, jj - 3*dj, kk - 3*dk);
+  const bool rs = (d == 0) && P.rescale_r;
+  int zm[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) zm[m] = imin(m, nd - 1 - (pos - 3));
+#pragma unroll
+  for (int v = 0; v < NP; v++) {
+    double y[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+      double x = src[v*NT + c0 + zm[m]*st];
+      if (rs) x *= X.rscale[rs_slot(v)*P.n1t + (ii - 3 + zm[m])];
+      y[m] = x;
+    }
+    float D[M - 1], dq[M - 1], ln[M - 2], rn[M - 2];
+#pragma unroll
+    for (int m = 0; m < M - 1; m++) D[m] = (float)(y[m + 1] - y[m]);
+#pragma unroll
+    for (int m = 1; m < M - 1; m++) dq[m] = mc_slope_diff(D[m - 1], D[m]);
+#pragma unroll
+    for (int m = 2; m < M - 2; m++)
+      ppm_limit_rel(-0.5f*D[m - 1] - K*(dq[m] - dq[m - 1]), 0.5f*D[m] - K*(dq[m + 1] - dq[m]), &ln[m], &rn[m]);
+#pragma unroll
+    for (int f = 0; f < F; f++) {
+      double l = y[2 + f] + (double)rn[2 + f], r = y[3 + f] + (double)ln[3 + f];
+      if (rs) {
+        const double fac = X.irscale[(3 + rs_slot(v))*P.n1t + imin(i + f, P.n1t - 1)];
+        l *= fac;
+        r *= fac;
+      }
+      pl[f][v] = (float)l;
+      pr[f][v] = (float)r;
+    }
+  }
+}
+
+// FP32 Riemann solve of face (i,j,k) from its two states
+template <int d>
+HD void flux_riemann_mixed(const Params& P, const Ptrs& X, const float* pl, const float* pr,
+                           int i, int j, int k, FailRec& fr) {
+  const long long NT = P.nt;
   const GeomF* g = X.geomf + geom_id(P, i, j, k, d);
   constexpr int dim = d + 1;
   const float gam = (float)P.gam;
@@ -232,7 +319,7 @@ HD void flux_face_mixed(const Params& P, const Ptrs& X, const double* __restrict
   const float cmin = fabsf(fmaxf(fmaxf(0.f, -cminl), -cminr));
   const float ctop = fmaxf(cmin, cmax);
   if (i >= NG && i < NG + P.n1 && j >= NG && j < NG + P.n2 && k >= NG && k < NG + P.n3 && ctop > 0.f)
-    fr.vmax = (double)ctop;
+    fr.vmax = dmax(fr.vmax, (double)ctop);
 
   float Ul[NP], Ur[NP], Fl[NP], Fr[NP];
   primtoflux_f<0>(pl, &ql, g, Ul);
@@ -242,7 +329,7 @@ HD void flux_face_mixed(const Params& P, const Ptrs& X, const double* __restrict
   const long long c = cidx(P, i, j, k);
   float* F = X.Ff;
   if (P.flux == FLUX_HLLE) {
-    const float den = 1.f/(cmax + cmin + (float)SMALL);
+    const float den = rcp_f(cmax + cmin + (float)SMALL);
 #pragma unroll
     for (int l = 0; l < NP; l++)
       F[(d*NFLUX + l)*NT + c] = (cmax*Fl[l] + cmin*Fr[l] - cmax*cmin*(Ur[l] - Ul[l]))*den;
@@ -253,15 +340,23 @@ HD void flux_face_mixed(const Params& P, const Ptrs& X, const double* __restrict
   // entropy face flux (src/fixup.c:363-398)
   {
     const float gm1 = gam - 1.f;
-    float s = gm1*pl[UU]*powf(pl[RHO], -gm1);
+    float s = gm1*pl[UU]*pow_f(pl[RHO], -gm1);
     const float Sl = s*(g->g*ql.ucon[0]), Gl = s*(g->g*ql.ucon[dim]);
-    s = gm1*pr[UU]*powf(pr[RHO], -gm1);
+    s = gm1*pr[UU]*pow_f(pr[RHO], -gm1);
     const float Sr = s*(g->g*qr.ucon[0]), Gr = s*(g->g*qr.ucon[dim]);
     float Fs;
-    if (P.flux == FLUX_HLLE) Fs = (cmax*Gl + cmin*Gr - cmax*cmin*(Sr - Sl))/(cmax + cmin + (float)SMALL);
+    if (P.flux == FLUX_HLLE) Fs = (cmax*Gl + cmin*Gr - cmax*cmin*(Sr - Sl))*rcp_f(cmax + cmin + (float)SMALL);
     else Fs = 0.5f*(Gl + Gr - ctop*(Sr - Sl));
     F[(d*NFLUX + 8)*NT + c] = Fs;
   }
+}
+
+template <int d>
+HD void flux_face_mixed(const Params& P, const Ptrs& X, const double* __restrict__ src,
+                        int i, int j, int k, FailRec& fr) {
+  float pl[NP], pr[NP];
+  flux_recon_mixed<d>(P, X, src, i, j, k, pl, pr);
+  flux_riemann_mixed<d>(P, X, pl, pr, i, j, k, fr);
 }
 
 // Corner EMFs from the FP32 face fluxes (flux_ct_para_fast2(), src/flux_ct.c:780-846;

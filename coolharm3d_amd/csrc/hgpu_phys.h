@@ -934,6 +934,7 @@ HD int check_gamma(double gamma) {
 // small helpers
 
 HD double dmax(double a, double b) { return (a > b) ? a : b; }   // MAX(), src/macros.h:474
+HD int imin(int a, int b) { return (a < b) ? a : b; }
 
 HD unsigned long long dbl_bits(double x) {
   union { double d; unsigned long long u; } c;

@@ -163,7 +163,8 @@ class Harm3D:
     def set_launch_shape(self, family, bs=256, kc=0, flags=1):
         """flags bit 0: XCD-contiguous tile order; bit 1: x1-fastest tile order;
         bit 2: register-capped kernel variant; bit 3 (conserved update only):
-        update and inversion fused in one kernel."""
+        update and inversion fused in one kernel; bits 4-5 (face fluxes, PPM):
+        0/1/2 for 1/2/4 faces per thread."""
         fam = self.FAMILIES.index(family) if isinstance(family, str) else int(family)
         self._chk(self.lib.hgpu_set_launch_shape(self.h, fam, bs, kc, flags), "hgpu_set_launch_shape")
 

@@ -126,7 +126,7 @@ def test_launch_shapes_and_autotune_keep_results(precision):
     g = torus_grid(40, 32, 24)
     p0 = torus(g)
     out = []
-    for mode in ("default", "shapes", "fused", "fused_capped", "autotune"):
+    for mode in ("default", "shapes", "fused", "fused_capped", "multi", "autotune"):
         print(f"[shapes] {precision} {mode}", flush=True)
         H = Harm3D(g, track_nfail=1, precision=precision)
         H.set_prims(p0)
@@ -138,6 +138,11 @@ def test_launch_shapes_and_autotune_keep_results(precision):
                                                          (64, 0, 1), (128, 16, 2), (64, 32, 3)]):
                 H.set_launch_shape(fam, bs, kc, remap)
                 assert H.launch_shape(fam) == (bs, kc, remap)
+        if mode == "multi":
+            # 2 or 4 faces per thread (flags bits 4-5), with and without the capped variant
+            for fam, (bs, kc, flags) in zip(H.FAMILIES[:3], [(256, 0, 16 | 3), (128, 32, 32 | 1), (64, 0, 16 | 4)]):
+                H.set_launch_shape(fam, bs, kc, flags)
+                assert H.launch_shape(fam) == (bs, kc, flags)
         if mode.startswith("fused"):
             cons = (256, 0, 8) if mode == "fused" else (128, 32, 15)
             for fam, (bs, kc, flags) in zip(H.FAMILIES, [(256, 0, 4), (256, 0, 5), (128, 0, 6),

@@ -186,14 +186,17 @@ def test_bbh_step_host_build(precision):
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_gpu_bbh_step_matches_host(precision):
-    """GPU vs the host build of the same bodies on the moving metric (they
-    differ only through the device's sin/cos/exp/pow in the metric)."""
+    """GPU vs the host build of the same bodies on the moving metric.  In FP64
+    they differ only through the device's sin/cos/exp/pow in the metric; the
+    mixed face solve also uses the device's single-instruction FP32
+    reciprocal, square root, log2 and exp2 (about 1 ulp each, the host build
+    the correctly rounded libm forms), so its bar is FP32 rounding."""
     G, _ = _bbh_run(False, precision)
     H, _ = _bbh_run(True, precision)
     a, b = G.prims()[3:-3, 3:-3, 3:-3].reshape(-1, 8), H.prims()[3:-3, 3:-3, 3:-3].reshape(-1, 8)
     l1 = np.abs(a - b).sum(axis=0) / np.maximum(np.abs(b).sum(axis=0), 1e-300)
     print(f"[bbh] {precision} GPU vs host rel L1 after 3 steps: {l1}", flush=True)
-    assert np.all(l1 <= 1e-9)
+    assert np.all(l1 <= (1e-9 if precision == "fp64" else 1e-6))
     assert G.time() == pytest.approx(H.time(), rel=1e-12)
 
 
