@@ -143,9 +143,19 @@ HD void ppm_faces(const Params& P, const Ptrs& X, const double* __restrict__ src
   }
 }
 
-// Riemann solve of face (i,j,k) from its two states: fluxes, the entropy flux
-// and the fastest interior speed (the rest of numerical_flux(),
-// src/step_ch.c:681-764).
+// Signal speeds of a face from its two states (src/step_ch.c:706-714)
+HD void face_speeds(const State& ql, const State& qr, const Geom* g, int dim, double gam,
+                    double* cmax, double* cmin, double* ctop) {
+  double cmaxl, cminl, cmaxr, cminr;
+  vchar_fast(&ql, g, dim, gam, &cmaxl, &cminl);
+  vchar_fast(&qr, g, dim, gam, &cmaxr, &cminr);
+  *cmax = fabs(dmax(dmax(0., cmaxl), cmaxr));
+  *cmin = fabs(dmax(dmax(0., -cminl), -cminr));
+  *ctop = dmax(*cmin, *cmax);
+}
+
+// Riemann solve of face (i,j,k) from its two states: fluxes and the fastest
+// interior speed (the rest of numerical_flux(), src/step_ch.c:681-764).
 template <int d>
 HD void flux_riemann(const Params& P, const Ptrs& X, const double* pl, const double* pr,
                      int i, int j, int k, FailRec& fr) {
@@ -160,12 +170,8 @@ HD void flux_riemann(const Params& P, const Ptrs& X, const double* pl, const dou
   primtoflux(pr, &qr, 0, g, Ur);
   primtoflux(pl, &ql, dim, g, Fl);
   primtoflux(pr, &qr, dim, g, Fr);
-  double cmaxl, cminl, cmaxr, cminr;
-  vchar_fast(&ql, g, dim, P.gam, &cmaxl, &cminl);
-  vchar_fast(&qr, g, dim, P.gam, &cmaxr, &cminr);
-  double cmax = fabs(dmax(dmax(0., cmaxl), cmaxr));
-  double cmin = fabs(dmax(dmax(0., -cminl), -cminr));
-  double ctop = dmax(cmin, cmax);
+  double cmax, cmin, ctop;
+  face_speeds(ql, qr, g, dim, P.gam, &cmax, &cmin, &ctop);
 
   // fastest speed over interior faces (timestep(), src/step_ch.c:716-724),
   // reduced over the launch by the caller
@@ -181,21 +187,35 @@ HD void flux_riemann(const Params& P, const Ptrs& X, const double* pl, const dou
     for (int l = 0; l < NP; l++)
       F[(d*NFLUX + l)*NT + c] = 0.5*(Fl[l] + Fr[l] - ctop*(Ur[l] - Ul[l]));
   }
+}
 
-  // entropy flux of this face for fixup_entropy_eq() (src/fixup.c:363-398),
-  // from the face data the reference stores in c_gf/ucon_L/ucon_R (src/step_ch.c:743-750)
-  {
-    double uL0 = g->g * ql.ucon[0], uL1 = g->g * ql.ucon[dim];
-    double uR0 = g->g * qr.ucon[0], uR1 = g->g * qr.ucon[dim];
-    double s = (P.gam - 1.) * pl[UU] * pow(pl[RHO], 1. - P.gam);
-    double Sl = s * uL0, Gl = s * uL1;
-    s = (P.gam - 1.) * pr[UU] * pow(pr[RHO], 1. - P.gam);
-    double Sr = s * uR0, Gr = s * uR1;
-    double Fs;
-    if (P.flux == FLUX_HLLE) Fs = (cmax*Gl + cmin*Gr - cmax*cmin*(Sr - Sl)) / (cmax + cmin + SMALL);
-    else Fs = 0.5*(Gl + Gr - ctop*(Sr - Sl));
-    F[(d*NFLUX + 8)*NT + c] = Fs;
-  }
+// Entropy flux of face (i,j,k) in direction d for fixup_entropy_eq()
+// (src/fixup.c:363-398), from the face data the reference keeps in
+// c_gf/ucon_L/ucon_R (src/step_ch.c:743-750).  Only the rare zones that take
+// the entropy fix read it, so it is computed here, from the stage's
+// reconstruction source and face geometry, with the arithmetic of the face
+// solve, instead of being stored by every face.
+template <int d>
+HD double entropy_face_d(const Params& P, const Ptrs& X, const double* src, int i, int j, int k) {
+  Ptrs Xs = X;
+  if (X.geom_s) Xs.geom = X.geom_s;
+  double pl[NP], pr[NP];
+  flux_recon<d>(P, Xs, src, i, j, k, pl, pr);
+  const Geom* g = geom_at(P, Xs, i, j, k, d);
+  constexpr int dim = d + 1;
+  State ql, qr;
+  get_state(pl, g, P.gam, &ql);   // failures were counted by the face solve
+  get_state(pr, g, P.gam, &qr);
+  double cmax, cmin, ctop;
+  face_speeds(ql, qr, g, dim, P.gam, &cmax, &cmin, &ctop);
+  double uL0 = g->g * ql.ucon[0], uL1 = g->g * ql.ucon[dim];
+  double uR0 = g->g * qr.ucon[0], uR1 = g->g * qr.ucon[dim];
+  double s = (P.gam - 1.) * pl[UU] * pow(pl[RHO], 1. - P.gam);
+  double Sl = s * uL0, Gl = s * uL1;
+  s = (P.gam - 1.) * pr[UU] * pow(pr[RHO], 1. - P.gam);
+  double Sr = s * uR0, Gr = s * uR1;
+  if (P.flux == FLUX_HLLE) return (cmax*Gl + cmin*Gr - cmax*cmin*(Sr - Sl)) / (cmax + cmin + SMALL);
+  return 0.5*(Gl + Gr - ctop*(Sr - Sl));
 }
 
 template <int d>
@@ -348,29 +368,33 @@ HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, i
 }
 
 // --------------------------------------------------------------------------
-// Entropy face flux of face (d, c) from the FP64 or the FP32 flux array
+// Entropy flux of face (i,j,k) in direction d, FP64 or FP32 face solve
 template <bool MIXED>
-HD double entropy_flux(const Ptrs& X, long long NT, int d, long long c) {
-  if (MIXED) return (double)X.Ff[(d*NFLUX + 8)*NT + c];
-  return X.F[(d*NFLUX + 8)*NT + c];
+HD double entropy_flux(const Params& P, const Ptrs& X, const double* src, int d, int i, int j, int k) {
+  if constexpr (MIXED) {
+    if (d == 0) return entropy_face_mixed<0>(P, X, src, i, j, k);
+    if (d == 1) return entropy_face_mixed<1>(P, X, src, i, j, k);
+    return entropy_face_mixed<2>(P, X, src, i, j, k);
+  } else {
+    if (d == 0) return entropy_face_d<0>(P, X, src, i, j, k);
+    if (d == 1) return entropy_face_d<1>(P, X, src, i, j, k);
+    return entropy_face_d<2>(P, X, src, i, j, k);
+  }
 }
 
 // fixup_entropy_eq(), src/fixup.c:335-454
 template <bool MIXED>
 HD int fixup_entropy_eq(const Params& P, const Ptrs& X, int i, int j, int k, double Dt,
                         const double* prim_old, double* prim, const Geom* g, const double* U,
-                        double* gamma_out, FailRec& fr) {
-  const long long NT = P.nt;
+                        double* gamma_out, const double* src, FailRec& fr) {
   double prim_backup[NP];
   for (int l = 0; l < NP; l++) prim_backup[l] = prim[l];
   double gamma;
   gamma_calc(prim_old, g, &gamma);
   double S_old = g->g * (P.gam - 1.)*prim_old[UU]*gamma / (g->alpha * pow(prim_old[RHO], (P.gam - 1.)));
-  const long long c = cidx(P, i, j, k);
-  const long long si = (long long)P.n2t * P.n3t, sj = P.n3t, sk = 1;
-  const double Fm0 = entropy_flux<MIXED>(X, NT, 0, c), Fp0 = entropy_flux<MIXED>(X, NT, 0, c + si);
-  const double Fm1 = entropy_flux<MIXED>(X, NT, 1, c), Fp1 = entropy_flux<MIXED>(X, NT, 1, c + sj);
-  const double Fm2 = entropy_flux<MIXED>(X, NT, 2, c), Fp2 = entropy_flux<MIXED>(X, NT, 2, c + sk);
+  const double Fm0 = entropy_flux<MIXED>(P, X, src, 0, i, j, k), Fp0 = entropy_flux<MIXED>(P, X, src, 0, i + 1, j, k);
+  const double Fm1 = entropy_flux<MIXED>(P, X, src, 1, i, j, k), Fp1 = entropy_flux<MIXED>(P, X, src, 1, i, j + 1, k);
+  const double Fm2 = entropy_flux<MIXED>(P, X, src, 2, i, j, k), Fp2 = entropy_flux<MIXED>(P, X, src, 2, i, j, k + 1);
   double S = S_old + Dt*(
       - (Fp0 - Fm0) * P.invdx[1]
       - (Fp1 - Fm1) * P.invdx[2]
@@ -396,10 +420,11 @@ HD int fixup_entropy_eq(const Params& P, const Ptrs& X, int i, int j, int k, dou
 
 // --------------------------------------------------------------------------
 // recover_primitives(), src/step_ch.c:801-1024 (FIXUP_TREE 3, entropy fix on)
+// src: the stage's reconstruction source, for the entropy face fluxes.
 template <bool MIXED>
 HD void recover_primitives(const Params& P, const Ptrs& X, int i, int j, int k, double Dt,
                            double* pf, const double* U, const double* p_old, const Geom* g,
-                           double rhoflr, double uuflr, double* p_gamma, FailRec& fr) {
+                           double rhoflr, double uuflr, double* p_gamma, const double* src, FailRec& fr) {
   double pg[NP], pf2[NP];
   for (int l = 0; l < NP; l++) pg[l] = pf[l];
   double gamma = 1., bsq = 0.;
@@ -429,7 +454,7 @@ HD void recover_primitives(const Params& P, const Ptrs& X, int i, int j, int k, 
     if (!ret) { for (int l = 0; l < NP; l++) pf2[l] = pf[l]; }
     else fail_set(P, fr, NFAIL_UTOPRIM, 0, i, j, k);
     for (int l = 0; l < B1; l++) pf[l] = pg[l];
-    int ret_ee = fixup_entropy_eq<MIXED>(P, X, i, j, k, Dt, p_old, pf, g, U, &gamma, fr);
+    int ret_ee = fixup_entropy_eq<MIXED>(P, X, i, j, k, Dt, p_old, pf, g, U, &gamma, src, fr);
     if (ret_ee) {
       if (ret) {
         fail_set(P, fr, NFAIL_UTOPRIM_EE, 1, i, j, k);
@@ -596,7 +621,7 @@ HD void recover_zone(const Params& P, const Ptrs& X, int stage, double dt, int i
   for (int v = 0; v < NP; v++) p_raw[v] = X.p[v*NT + c];
   double pgam = 1.;
   recover_primitives<MIXED>(P, X, i, j, k, Dt, pf, U, p_raw, g, floor_rho(P, X, i, c), floor_uu(P, X, i, c),
-                            &pgam, fr);
+                            &pgam, (stage == 0) ? X.p : X.ph, fr);
   store_zone(P, X, stage, c, pf, pgam, fr);
 }
 

@@ -415,6 +415,7 @@ struct Api {
       X.conn = c->connlev[stage];
       X.geom_n = c->lev[0];
       X.geom_new = c->lev[stage + 1];
+      X.geom_s = c->lev[stage];
     }
     rc |= BE::reset_vmax(c);
     for (int d = 0; d < 3; d++) rc |= BE::flux(c, src, d);

@@ -337,18 +337,36 @@ HD void flux_riemann_mixed(const Params& P, const Ptrs& X, const float* pl, cons
 #pragma unroll
     for (int l = 0; l < NP; l++) F[(d*NFLUX + l)*NT + c] = 0.5f*(Fl[l] + Fr[l] - ctop*(Ur[l] - Ul[l]));
   }
-  // entropy face flux (src/fixup.c:363-398)
-  {
-    const float gm1 = gam - 1.f;
-    float s = gm1*pl[UU]*pow_f(pl[RHO], -gm1);
-    const float Sl = s*(g->g*ql.ucon[0]), Gl = s*(g->g*ql.ucon[dim]);
-    s = gm1*pr[UU]*pow_f(pr[RHO], -gm1);
-    const float Sr = s*(g->g*qr.ucon[0]), Gr = s*(g->g*qr.ucon[dim]);
-    float Fs;
-    if (P.flux == FLUX_HLLE) Fs = (cmax*Gl + cmin*Gr - cmax*cmin*(Sr - Sl))*rcp_f(cmax + cmin + (float)SMALL);
-    else Fs = 0.5f*(Gl + Gr - ctop*(Sr - Sl));
-    F[(d*NFLUX + 8)*NT + c] = Fs;
-  }
+}
+
+// Entropy flux of face (i,j,k) for fixup_entropy_eq() (src/fixup.c:363-398)
+// from the FP32 face solve's states and speeds, computed for the zones that
+// take the entropy fix (the stage's face geometry is X.geomf throughout).
+template <int d>
+HD double entropy_face_mixed(const Params& P, const Ptrs& X, const double* __restrict__ src, int i, int j, int k) {
+  float pl[NP], pr[NP];
+  flux_recon_mixed<d>(P, X, src, i, j, k, pl, pr);
+  const GeomF* g = X.geomf + geom_id(P, i, j, k, d);
+  constexpr int dim = d + 1;
+  const float gam = (float)P.gam;
+  StateF ql, qr;
+  get_state_f(pl, g, gam, &ql);   // failures were counted by the face solve
+  get_state_f(pr, g, gam, &qr);
+  float cmaxl, cminl, cmaxr, cminr;
+  vchar_f<dim>(&ql, g, gam, &cmaxl, &cminl);
+  vchar_f<dim>(&qr, g, gam, &cmaxr, &cminr);
+  const float cmax = fabsf(fmaxf(fmaxf(0.f, cmaxl), cmaxr));
+  const float cmin = fabsf(fmaxf(fmaxf(0.f, -cminl), -cminr));
+  const float ctop = fmaxf(cmin, cmax);
+  const float gm1 = gam - 1.f;
+  float s = gm1*pl[UU]*pow_f(pl[RHO], -gm1);
+  const float Sl = s*(g->g*ql.ucon[0]), Gl = s*(g->g*ql.ucon[dim]);
+  s = gm1*pr[UU]*pow_f(pr[RHO], -gm1);
+  const float Sr = s*(g->g*qr.ucon[0]), Gr = s*(g->g*qr.ucon[dim]);
+  float Fs;
+  if (P.flux == FLUX_HLLE) Fs = (cmax*Gl + cmin*Gr - cmax*cmin*(Sr - Sl))*rcp_f(cmax + cmin + (float)SMALL);
+  else Fs = 0.5f*(Gl + Gr - ctop*(Sr - Sl));
+  return (double)Fs;
 }
 
 template <int d>

@@ -15,7 +15,7 @@ namespace hgpu {
 constexpr int NG = 3;
 constexpr int NP = 8;
 constexpr int NDIM = 4;
-constexpr int NFLUX = 9;      // 8 conserved fluxes + entropy flux (fixup_entropy_eq)
+constexpr int NFLUX = 8;      // conserved fluxes per face
 constexpr int N_NFAIL = 15;   // src/macros.h:236-251
 constexpr int GEOM_DOUBLES = 42;
 
@@ -116,6 +116,7 @@ struct Ptrs {
   const Geom* geom;        // geometry records (the level of the stage being computed)
   const Geom* geom_n;      // level of U^n (time t of the step; = geom for a static metric)
   const Geom* geom_new;    // level of the stage's result (t + dt/2 or t + dt; = geom when static)
+  const Geom* geom_s;      // stage's level for face records after geom moves to geom_new (null: geom)
   const GeomF* geomf;      // their FP32 copy (mixed precision)
   float* Ff;               // FP32 face fluxes [3][NFLUX][NT] (mixed precision)
   float* emff;             // FP32 corner EMFs [3][NT] (mixed precision)

@@ -212,7 +212,7 @@ class Harm3D:
         return fe.value, v
 
     def fluxes(self):
-        a = np.empty((3, 9) + self.shape)
+        a = np.empty((3, 8) + self.shape)
         self._chk(self.lib.hgpu_download_fluxes(self.h, _dp(a)), "hgpu_download_fluxes")
         return a
 

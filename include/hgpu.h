@@ -134,7 +134,8 @@ int hgpu_reset_nfail(hgpu_ctx *ctx);
 /* failure_exists and max_char_speed[1..3] of the last numerical_flux(). */
 int hgpu_get_status(hgpu_ctx *ctx, int *failure_exists, double *vmax4);
 
-/* Face fluxes of the last advance(): F[3][9][NT] (8 conserved + entropy). */
+/* Face fluxes of the last advance(): F[3][8][NT] (the conserved fluxes; the
+ * entropy face fluxes are evaluated only for zones that take the entropy fix). */
 int hgpu_download_fluxes(hgpu_ctx *ctx, double *host);
 
 /* Maximum |div B| at zone corners with the constrained-transport stencil
