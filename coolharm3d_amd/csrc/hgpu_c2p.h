@@ -396,6 +396,7 @@ HD int c2p_unpark(const C2pQueue& Q, long long q, double gmog, C2pCell& c) {
   c.it.n_iter = m & 63;
   c.it.i_extra = (m >> 6) & 7;
   c.it.doing_extra = (m >> 9) & 1;
+  c.it.n_extra = EXTRA_NEWT_ITER;
   c.attempt = (m >> 10) & 1;
   c.has_fallback = (m >> 11) & 1;
   c.total = m >> 12;

@@ -20,6 +20,11 @@ constexpr int MAX_NEWT_ITER = 30;         // src/u2p_defs.h:76-79
 constexpr double NEWT_TOL = 1.0e-10;
 constexpr double MIN_NEWT_TOL = 1.0e-10;
 constexpr int EXTRA_NEWT_ITER = 2;
+// Extra iterations after convergence in the mixed-precision step's inversion:
+// the test |dW/W| <= NEWT_TOL is on the step just taken, so with quadratic
+// convergence the accepted iterate is already at round-off and the reference's
+// two confirming iterations change nothing at the mixed step's precision.
+constexpr int MIXED_EXTRA_NEWT_ITER = 0;
 constexpr double NEWT_TOL2 = 1.0e-15;
 constexpr double W_TOO_BIG = 1.e20;
 constexpr double UTSQ_TOO_BIG = 1.e20;
@@ -286,12 +291,14 @@ struct NewtonIter {
   double x[2];
   double f, df, errx;
   int n_iter, i_extra, doing_extra;
+  int n_extra;   // confirming iterations after convergence (EXTRA_NEWT_ITER)
 };
 
-HD void u2d_newton_init(NewtonIter& it, const double* x) {
+HD void u2d_newton_init(NewtonIter& it, const double* x, int n_extra = EXTRA_NEWT_ITER) {
   it.x[0] = x[0]; it.x[1] = x[1];
   it.f = 1.; it.df = 1.; it.errx = 1.;
   it.n_iter = 0; it.i_extra = 0; it.doing_extra = 0;
+  it.n_extra = n_extra;
 }
 
 HD bool u2d_newton_step(const U2d& s, NewtonIter& it) {
@@ -307,9 +314,9 @@ HD bool u2d_newton_step(const U2d& s, NewtonIter& it) {
   else { if (it.x[0] > W_TOO_BIG) { it.x[0] = x_old[0]; } }
   if (it.x[1] < 0.) { it.x[1] = 0.; }
   else { if (it.x[1] > 1.) { it.x[1] = dv; } }
-  if ((fabs(it.errx) <= NEWT_TOL) && (it.doing_extra == 0) && (EXTRA_NEWT_ITER > 0)) it.doing_extra = 1;
+  if ((fabs(it.errx) <= NEWT_TOL) && (it.doing_extra == 0) && (it.n_extra > 0)) it.doing_extra = 1;
   if (it.doing_extra == 1) it.i_extra++;
-  bool keep = !(((fabs(it.errx) <= NEWT_TOL) && (it.doing_extra == 0)) || (it.i_extra > EXTRA_NEWT_ITER) ||
+  bool keep = !(((fabs(it.errx) <= NEWT_TOL) && (it.doing_extra == 0)) || (it.i_extra > it.n_extra) ||
                 (it.n_iter >= (MAX_NEWT_ITER - 1)));
   it.n_iter++;
   return keep;
@@ -322,9 +329,9 @@ HD int u2d_newton_result(const NewtonIter& it) {
   return 1;
 }
 
-HD int u2d_newton(const U2d& s, double* x, int* iters) {
+HD int u2d_newton(const U2d& s, double* x, int* iters, int n_extra = EXTRA_NEWT_ITER) {
   NewtonIter it;
-  u2d_newton_init(it, x);
+  u2d_newton_init(it, x, n_extra);
   while (u2d_newton_step(s, it)) {}
   x[0] = it.x[0]; x[1] = it.x[1];
   if (iters) *iters = it.n_iter;
@@ -455,15 +462,17 @@ HD int u2d_finish(const U2d& s, const U2dAux& aux, double gam, const double* x2d
 // prim: guess on input; rho,u,u~ replaced only on success, B always set from U
 // (as the reference does before the solve).  gam: adiabatic index,
 // gmog = (gam-1)/gam (gam_m1_o_gam, src/main.c:299).
+// n_extra: confirming iterations after convergence (the reference's
+// EXTRA_NEWT_ITER unless the caller asks for fewer).
 HD int utoprim_2d_fast(const double* U, const Geom* geom, double gam, double gmog, double* prim,
-                       double* gamma_out, double* bsq_out, int* iters) {
+                       double* gamma_out, double* bsq_out, int* iters, int n_extra = EXTRA_NEWT_ITER) {
   if (iters) *iters = 0;
   U2d s;
   U2dAux aux;
   double x2d[2];
   int rc = u2d_setup(U, geom, gam, gmog, prim, s, aux, x2d);
   if (rc) return rc;
-  int retval = u2d_newton(s, x2d, iters);
+  int retval = u2d_newton(s, x2d, iters, n_extra);
   return u2d_finish(s, aux, gam, x2d, retval, prim, gamma_out, bsq_out);
 }
 
