@@ -526,7 +526,6 @@ HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int
 #pragma unroll
     for (int v = 0; v < NP; v++) U[v] = X.Un[v*NT + c];
   }
-  phase_fence();
   double dU[NP];
   {
     double phz[NP];
@@ -535,7 +534,6 @@ HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int
     for (int v = 0; v < NP; v++) phz[v] = rt_val(P, X, v, i, src[v*NT + c]);
     fr.nf[NFAIL_GAMMA_CALC] += source(P, X, phz, g, i, j, k, dU);
   }
-  phase_fence();
   if (MIXED) {
 #pragma unroll
     for (int l = 0; l < NP; l++) {
@@ -635,7 +633,6 @@ HD bool update_zone_fast(const Params& P, const Ptrs& X, int stage, double dt, i
   const double Dt = (stage == 0) ? 0.5*dt : dt;
   const int gid = geom_id(P, i, j, k, CENT);
   zone_conserved<MIXED>(P, X, stage, Dt, i, j, k, X.geom_n + gid, X.geom + gid, fr, U);
-  phase_fence();
   double pf[NP];
   zone_guess(P, X, stage, i, c, pf);
   double pgam = 1.;

@@ -590,10 +590,20 @@ struct GpuBE {
       return set_error(-5, "no HIP device available (the hgpu library runs only on a GPU)");
     GpuPriv* g = new GpuPriv();
     c->be = g;
-    // untuned default: the x1 fluxes take one (j,k) tile through consecutive
-    // x1-planes, so their +-3-plane stencil stays in L2 (measured 1.7x faster
-    // in mixed precision, neutral in FP64)
+    // Untuned defaults: the autotuner's picks on the 192x192x128 and
+    // 256x256x192 tori.  The x1 fluxes take one (j,k) tile through consecutive
+    // x1-planes, so their +-3-plane stencil stays in L2; the fluxes run 4
+    // (FP64) or 2 (FP32 solve) faces per thread; the update is the fused
+    // kernel (register-capped in mixed precision).
+    const bool mixed = c->P.mixed != 0;
+    for (int d = 0; d < 3; d++) g->shape[FAM_FLUX1 + d].fpt = mixed ? 2u : 4u;
     g->shape[FAM_FLUX1].order = 1;
+    g->shape[FAM_FLUX2].occ = mixed ? 1u : 0u;
+    g->shape[FAM_FLUX3].remap = mixed ? 0u : 1u;
+    g->shape[FAM_EMF].order = mixed ? 1u : 0u;
+    g->shape[FAM_CONS].fused = 1;
+    g->shape[FAM_CONS].remap = 0;
+    g->shape[FAM_CONS].occ = mixed ? 1u : 0u;
     return hip_check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
   }
   static void fini(hgpu_ctx* c) {

@@ -64,15 +64,6 @@ HD void atomic_or_u64(unsigned long long* a, unsigned long long v) {
 
 HD bool is_finite(double x) { return isfinite(x); }
 
-// Compiler-level memory fence between phases of a long kernel body: values
-// loaded before it are not kept in registers across it (they are loaded
-// again, from cache), which shortens live ranges.  No effect on results.
-HD void phase_fence() {
-#ifdef __HIP_DEVICE_COMPILE__
-  asm volatile("" ::: "memory");
-#endif
-}
-
 // ---------------------------------------------------------------------------
 // Metric contractions, src/metric.c:338-383
 HD void lower(const double* ucon, const Geom* g, double* ucov) {
