@@ -57,14 +57,15 @@ METRIC = "zone-cycles/s per RK2 step, 1 MI355X, mixed vs FP64; con2prim solves/s
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic HBM bytes per unit of work of each kernel (FP64 state):
-#   flux   : per face   -- 8 primitives of the zone read once (64 B) + 9 face values written (72 B)
+#   flux   : per face   -- 8 primitives of the zone read once (64 B) + 8 face fluxes written (64 B)
 #   emf    : per edge   -- 6 B-field face fluxes read (48 B) + 3 EMFs written (24 B)
-#   update : per zone   -- p read (64 B), 27 face values + 3 EMFs read (240 B), result written (64 B),
-#                          gamma written (8 B); the corrector also reads ph (64 B): 376 / 440 B
-ALGO_BYTES = {"flux": 136.0, "emf": 72.0, "update": 0.5 * (376.0 + 440.0)}
-# mixed precision: face values and EMFs are FP32 (flux writes 36 B/face, EMF 24+12 B/edge,
-# update reads 27 face values + 3 EMFs at 4 B)
-ALGO_BYTES_MIXED = {"flux": 100.0, "emf": 36.0, "update": 0.5 * (376.0 + 440.0) - 120.0}
+#   update : per zone   -- p read (64 B); 15 non-magnetic face fluxes + 3 EMFs read (144 B); result
+#                          (64 B) and gamma (8 B) written; the predictor writes U^n (64 B), the
+#                          corrector reads U^n and ph (128 B): 344 / 408 B
+ALGO_BYTES = {"flux": 128.0, "emf": 72.0, "update": 0.5 * (344.0 + 408.0)}
+# mixed precision: face fluxes and EMFs are FP32 (flux writes 32 B/face, EMF 24+12 B/edge,
+# the update reads 18 FP32 values)
+ALGO_BYTES_MIXED = {"flux": 96.0, "emf": 36.0, "update": 0.5 * (344.0 + 408.0) - 72.0}
 
 
 def measured_traffic(workload):
