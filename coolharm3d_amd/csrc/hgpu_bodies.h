@@ -155,10 +155,11 @@ HD void face_speeds(const State& ql, const State& qr, const Geom* g, int dim, do
 }
 
 // Riemann solve of face (i,j,k) from its two states: fluxes and the fastest
-// interior speed (the rest of numerical_flux(), src/step_ch.c:681-764).
+// interior speed (the rest of numerical_flux(), src/step_ch.c:681-764).  The
+// fluxes go to X.F, or to out[8] when the caller stores them itself.
 template <int d>
 HD void flux_riemann(const Params& P, const Ptrs& X, const double* pl, const double* pr,
-                     int i, int j, int k, FailRec& fr) {
+                     int i, int j, int k, FailRec& fr, double* out = nullptr) {
   const long long NT = P.nt;
   const Geom* g = geom_at(P, X, i, j, k, d);
   constexpr int dim = d + 1;
@@ -178,14 +179,19 @@ HD void flux_riemann(const Params& P, const Ptrs& X, const double* pl, const dou
   if (i >= NG && i < NG + P.n1 && j >= NG && j < NG + P.n2 && k >= NG && k < NG + P.n3 && ctop > 0.)
     fr.vmax = dmax(fr.vmax, ctop);
 
-  const long long c = cidx(P, i, j, k);
-  double* F = X.F;
+  double Fv[NP];
   if (P.flux == FLUX_HLLE) {
     for (int l = 0; l < NP; l++)
-      F[(d*NFLUX + l)*NT + c] = (cmax*Fl[l] + cmin*Fr[l] - cmax*cmin*(Ur[l] - Ul[l])) / (cmax + cmin + SMALL);
+      Fv[l] = (cmax*Fl[l] + cmin*Fr[l] - cmax*cmin*(Ur[l] - Ul[l])) / (cmax + cmin + SMALL);
   } else {
     for (int l = 0; l < NP; l++)
-      F[(d*NFLUX + l)*NT + c] = 0.5*(Fl[l] + Fr[l] - ctop*(Ur[l] - Ul[l]));
+      Fv[l] = 0.5*(Fl[l] + Fr[l] - ctop*(Ur[l] - Ul[l]));
+  }
+  if (out) {
+    for (int l = 0; l < NP; l++) out[l] = Fv[l];
+  } else {
+    const long long c = cidx(P, i, j, k);
+    for (int l = 0; l < NP; l++) X.F[(d*NFLUX + l)*NT + c] = Fv[l];
   }
 }
 

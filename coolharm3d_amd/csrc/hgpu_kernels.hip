@@ -223,12 +223,22 @@ __global__ __launch_bounds__(BLOCK, W) void k_flux_mixed(Params P, Ptrs X, const
 // past hi_d (the flux box end) are skipped.  Each zone's parabola is computed
 // once per group.  The per-face fail() tally goes to the face's zone here,
 // the block totals through commit_fail().
+// Along x3 (the fastest index of the arrays) a lane's F faces are adjacent in
+// memory, so the fluxes go through LDS: slot s of lane l stores element
+// s*64 + l of the wavefront's F*64 faces, and each store instruction writes
+// consecutive faces.
 template <int D, int F, int W, bool MIXED>
 __global__ __launch_bounds__(BLOCK, W) void k_flux_multi(Params P, Ptrs X, const double* __restrict__ src, Box bg,
                                                          Tiling T, int hi_d) {
+  using Real = typename std::conditional<MIXED, float, double>::type;
+  constexpr bool TRANSPOSE = (D == 2);
+  __shared__ Real tbuf[TRANSPOSE ? BLOCK * F : 1];
   int i, j, k;
   const bool active = tile_index(bg, T, i, j, k);
   FailRec fr; fr.clear();
+  Real fo[F][NP];
+  long long cb = 0;
+  int nv = 0;
   if (active) {
     constexpr int di = (D == 0), dj = (D == 1), dk = (D == 2);
     const int lo = bg.lo[D];
@@ -236,7 +246,6 @@ __global__ __launch_bounds__(BLOCK, W) void k_flux_multi(Params P, Ptrs X, const
     else if (D == 1) j = lo + (j - lo) * F;
     else k = lo + (k - lo) * F;
     const int nvalid = min(F, hi_d - ((D == 0) ? i : ((D == 1) ? j : k)) + 1);
-    using Real = typename std::conditional<MIXED, float, double>::type;
     Real pl[F][NP], pr[F][NP];
     if constexpr (MIXED) ppm_faces_mixed<D, F>(P, X, src, i, j, k, pl, pr);
     else ppm_faces<D, F>(P, X, src, i, j, k, pl, pr);
@@ -245,10 +254,35 @@ __global__ __launch_bounds__(BLOCK, W) void k_flux_multi(Params P, Ptrs X, const
       if (f >= nvalid) break;
       const int g0 = fr.nf[NFAIL_GAMMA_CALC];
       const int fi = i + f*di, fj = j + f*dj, fk = k + f*dk;
-      if constexpr (MIXED) flux_riemann_mixed<D>(P, X, pl[f], pr[f], fi, fj, fk, fr);
-      else flux_riemann<D>(P, X, pl[f], pr[f], fi, fj, fk, fr);
+      if constexpr (MIXED) flux_riemann_mixed<D>(P, X, pl[f], pr[f], fi, fj, fk, fr, TRANSPOSE ? fo[f] : nullptr);
+      else flux_riemann<D>(P, X, pl[f], pr[f], fi, fj, fk, fr, TRANSPOSE ? fo[f] : nullptr);
       const int dv = fr.nf[NFAIL_GAMMA_CALC] - g0;
       if (P.track_nfail && dv) X.nfail[NFAIL_GAMMA_CALC*P.nt + cidx(P, fi, fj, fk)] += dv;
+    }
+    cb = cidx(P, i, j, k);
+    nv = nvalid;
+  }
+  if constexpr (TRANSPOSE) {
+    Real* Fa = MIXED ? (Real*)X.Ff : (Real*)X.F;
+    const int lane = threadIdx.x & 63;
+    Real* wbuf = tbuf + (threadIdx.x & ~63) * F;
+    long long cs[F];
+    bool ok[F];
+#pragma unroll
+    for (int s = 0; s < F; s++) {
+      const int e = s*64 + lane, sl = e / F, f = e % F;
+      cs[s] = __shfl(cb, sl, 64) + f;
+      ok[s] = f < __shfl(nv, sl, 64);
+    }
+#pragma unroll
+    for (int l = 0; l < NP; l++) {
+      __syncthreads();
+#pragma unroll
+      for (int f = 0; f < F; f++) wbuf[lane*F + f] = fo[f][l];
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < F; s++)
+        if (ok[s]) Fa[(2*NFLUX + l)*(long long)P.nt + cs[s]] = wbuf[s*64 + lane];
     }
   }
   commit_fail(P, X, fr, active, 0, MASK_GAMMA, D + 1, false);

@@ -301,10 +301,11 @@ HD void ppm_faces_mixed(const Params& P, const Ptrs& X, const double* __restrict
   }
 }
 
-// FP32 Riemann solve of face (i,j,k) from its two states
+// FP32 Riemann solve of face (i,j,k) from its two states; fluxes to X.Ff or
+// to out[8]
 template <int d>
 HD void flux_riemann_mixed(const Params& P, const Ptrs& X, const float* pl, const float* pr,
-                           int i, int j, int k, FailRec& fr) {
+                           int i, int j, int k, FailRec& fr, float* out = nullptr) {
   const long long NT = P.nt;
   const GeomF* g = X.geomf + geom_id(P, i, j, k, d);
   constexpr int dim = d + 1;
@@ -326,16 +327,22 @@ HD void flux_riemann_mixed(const Params& P, const Ptrs& X, const float* pl, cons
   primtoflux_f<0>(pr, &qr, g, Ur);
   primtoflux_f<dim>(pl, &ql, g, Fl);
   primtoflux_f<dim>(pr, &qr, g, Fr);
-  const long long c = cidx(P, i, j, k);
-  float* F = X.Ff;
+  float Fv[NP];
   if (P.flux == FLUX_HLLE) {
     const float den = rcp_f(cmax + cmin + (float)SMALL);
 #pragma unroll
-    for (int l = 0; l < NP; l++)
-      F[(d*NFLUX + l)*NT + c] = (cmax*Fl[l] + cmin*Fr[l] - cmax*cmin*(Ur[l] - Ul[l]))*den;
+    for (int l = 0; l < NP; l++) Fv[l] = (cmax*Fl[l] + cmin*Fr[l] - cmax*cmin*(Ur[l] - Ul[l]))*den;
   } else {
 #pragma unroll
-    for (int l = 0; l < NP; l++) F[(d*NFLUX + l)*NT + c] = 0.5f*(Fl[l] + Fr[l] - ctop*(Ur[l] - Ul[l]));
+    for (int l = 0; l < NP; l++) Fv[l] = 0.5f*(Fl[l] + Fr[l] - ctop*(Ur[l] - Ul[l]));
+  }
+  if (out) {
+#pragma unroll
+    for (int l = 0; l < NP; l++) out[l] = Fv[l];
+  } else {
+    const long long c = cidx(P, i, j, k);
+#pragma unroll
+    for (int l = 0; l < NP; l++) X.Ff[(d*NFLUX + l)*NT + c] = Fv[l];
   }
 }
 
