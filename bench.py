@@ -274,8 +274,11 @@ def bbh_report(args):
     H.step_ch(3)
     kern = {k: H.profile_read(k)[0] / 3 for k in ("metric", "flux", "emf", "update", "recover", "bounds", "fixup")}
     H.profile_enable(False)
-    shapes = H.autotune(reps=1)
-    H.set_graph(True)
+    if args.tune == "on":
+        shapes = H.autotune(reps=1)
+    else:
+        shapes = {f: H.launch_shape(f) for f in H.FAMILIES[:5]}
+    H.set_graph(args.graph == "on")
     H.step_ch(args.warmup)
     H.sync()
     t0 = time.perf_counter()
@@ -288,7 +291,7 @@ def bbh_report(args):
     ncell = shape[0] * shape[1] * shape[2]
     return {"workload": f"bbh_disk_{shape[0]}x{shape[1]}x{shape[2]}_mixed", "value": ncell * args.bbh_steps / el,
             "unit": "zone-cycles/s", "ms_per_step": el / args.bbh_steps * 1e3, "steps": args.bbh_steps,
-            "kernel_ms_per_step_eager": kern, "launch_shapes": shapes, "graph": True, "finite": finite,
+            "kernel_ms_per_step_eager": kern, "launch_shapes": shapes, "graph": args.graph == "on", "finite": finite,
             "divb_max": divb, "metric": "1PN near-zone, quasi-circular inspiral (Peters), separation 20 M"}
 
 
