@@ -7,8 +7,15 @@
 #pragma once
 #include "hgpu_phys.h"
 #include "hgpu_mixed.h"
+#include "hgpu_pn.h"
 
 namespace hgpu {
+
+// Binary metric evaluated in a kernel: the grid and the orbit of the level.
+struct FaceOtf {
+  PnGrid G;
+  const PnOrbit* o;
+};
 
 // --------------------------------------------------------------------------
 // numerical_flux() for one face, src/step_ch.c:636-790 (with the face states of
@@ -157,11 +164,11 @@ HD void face_speeds(const State& ql, const State& qr, const Geom* g, int dim, do
 // Riemann solve of face (i,j,k) from its two states: fluxes and the fastest
 // interior speed (the rest of numerical_flux(), src/step_ch.c:681-764).  The
 // fluxes go to X.F, or to out[8] when the caller stores them itself.
+// g: the face's geometry record (flux_riemann() reads the stored one).
 template <int d>
-HD void flux_riemann(const Params& P, const Ptrs& X, const double* pl, const double* pr,
-                     int i, int j, int k, FailRec& fr, double* out = nullptr) {
+HD void flux_riemann_g(const Params& P, const Ptrs& X, const double* pl, const double* pr,
+                       int i, int j, int k, FailRec& fr, double* out, const Geom* g) {
   const long long NT = P.nt;
-  const Geom* g = geom_at(P, X, i, j, k, d);
   constexpr int dim = d + 1;
   State ql, qr;
   if (get_state(pl, g, P.gam, &ql)) fr.nf[NFAIL_GAMMA_CALC]++;
@@ -195,6 +202,12 @@ HD void flux_riemann(const Params& P, const Ptrs& X, const double* pl, const dou
   }
 }
 
+template <int d>
+HD void flux_riemann(const Params& P, const Ptrs& X, const double* pl, const double* pr,
+                     int i, int j, int k, FailRec& fr, double* out = nullptr) {
+  flux_riemann_g<d>(P, X, pl, pr, i, j, k, fr, out, geom_at(P, X, i, j, k, d));
+}
+
 // Entropy flux of face (i,j,k) in direction d for fixup_entropy_eq()
 // (src/fixup.c:363-398), from the face data the reference keeps in
 // c_gf/ucon_L/ucon_R (src/step_ch.c:743-750).  Only the rare zones that take
@@ -202,12 +215,15 @@ HD void flux_riemann(const Params& P, const Ptrs& X, const double* pl, const dou
 // reconstruction source and face geometry, with the arithmetic of the face
 // solve, instead of being stored by every face.
 template <int d>
-HD double entropy_face_d(const Params& P, const Ptrs& X, const double* src, int i, int j, int k) {
+HD double entropy_face_d(const Params& P, const Ptrs& X, const double* src, int i, int j, int k,
+                         const FaceOtf* fo) {
   Ptrs Xs = X;
   if (X.geom_s) Xs.geom = X.geom_s;
   double pl[NP], pr[NP];
   flux_recon<d>(P, Xs, src, i, j, k, pl, pr);
-  const Geom* g = geom_at(P, Xs, i, j, k, d);
+  Geom gb;
+  if (fo) pn_zone_geom(P, fo->G, *fo->o, i, j, k, d, &gb);
+  const Geom* g = fo ? &gb : geom_at(P, Xs, i, j, k, d);
   constexpr int dim = d + 1;
   State ql, qr;
   get_state(pl, g, P.gam, &ql);   // failures were counted by the face solve
@@ -375,16 +391,26 @@ HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, i
 
 // --------------------------------------------------------------------------
 // Entropy flux of face (i,j,k) in direction d, FP64 or FP32 face solve
+// fo: the face records are evaluated (binary metric) rather than stored.
 template <bool MIXED>
-HD double entropy_flux(const Params& P, const Ptrs& X, const double* src, int d, int i, int j, int k) {
+HD double entropy_flux(const Params& P, const Ptrs& X, const double* src, int d, int i, int j, int k,
+                       const FaceOtf* fo) {
   if constexpr (MIXED) {
-    if (d == 0) return entropy_face_mixed<0>(P, X, src, i, j, k);
-    if (d == 1) return entropy_face_mixed<1>(P, X, src, i, j, k);
-    return entropy_face_mixed<2>(P, X, src, i, j, k);
+    GeomF gf;
+    const GeomF* gp = nullptr;
+    if (fo) {
+      Geom gb;
+      pn_zone_geom(P, fo->G, *fo->o, i, j, k, d, &gb);
+      geom_f32(gb, gf);
+      gp = &gf;
+    }
+    if (d == 0) return entropy_face_mixed<0>(P, X, src, i, j, k, gp);
+    if (d == 1) return entropy_face_mixed<1>(P, X, src, i, j, k, gp);
+    return entropy_face_mixed<2>(P, X, src, i, j, k, gp);
   } else {
-    if (d == 0) return entropy_face_d<0>(P, X, src, i, j, k);
-    if (d == 1) return entropy_face_d<1>(P, X, src, i, j, k);
-    return entropy_face_d<2>(P, X, src, i, j, k);
+    if (d == 0) return entropy_face_d<0>(P, X, src, i, j, k, fo);
+    if (d == 1) return entropy_face_d<1>(P, X, src, i, j, k, fo);
+    return entropy_face_d<2>(P, X, src, i, j, k, fo);
   }
 }
 
@@ -392,15 +418,15 @@ HD double entropy_flux(const Params& P, const Ptrs& X, const double* src, int d,
 template <bool MIXED>
 HD int fixup_entropy_eq(const Params& P, const Ptrs& X, int i, int j, int k, double Dt,
                         const double* prim_old, double* prim, const Geom* g, const double* U,
-                        double* gamma_out, const double* src, FailRec& fr) {
+                        double* gamma_out, const double* src, const FaceOtf* fo, FailRec& fr) {
   double prim_backup[NP];
   for (int l = 0; l < NP; l++) prim_backup[l] = prim[l];
   double gamma;
   gamma_calc(prim_old, g, &gamma);
   double S_old = g->g * (P.gam - 1.)*prim_old[UU]*gamma / (g->alpha * pow(prim_old[RHO], (P.gam - 1.)));
-  const double Fm0 = entropy_flux<MIXED>(P, X, src, 0, i, j, k), Fp0 = entropy_flux<MIXED>(P, X, src, 0, i + 1, j, k);
-  const double Fm1 = entropy_flux<MIXED>(P, X, src, 1, i, j, k), Fp1 = entropy_flux<MIXED>(P, X, src, 1, i, j + 1, k);
-  const double Fm2 = entropy_flux<MIXED>(P, X, src, 2, i, j, k), Fp2 = entropy_flux<MIXED>(P, X, src, 2, i, j, k + 1);
+  const double Fm0 = entropy_flux<MIXED>(P, X, src, 0, i, j, k, fo), Fp0 = entropy_flux<MIXED>(P, X, src, 0, i + 1, j, k, fo);
+  const double Fm1 = entropy_flux<MIXED>(P, X, src, 1, i, j, k, fo), Fp1 = entropy_flux<MIXED>(P, X, src, 1, i, j + 1, k, fo);
+  const double Fm2 = entropy_flux<MIXED>(P, X, src, 2, i, j, k, fo), Fp2 = entropy_flux<MIXED>(P, X, src, 2, i, j, k + 1, fo);
   double S = S_old + Dt*(
       - (Fp0 - Fm0) * P.invdx[1]
       - (Fp1 - Fm1) * P.invdx[2]
@@ -430,7 +456,8 @@ HD int fixup_entropy_eq(const Params& P, const Ptrs& X, int i, int j, int k, dou
 template <bool MIXED>
 HD void recover_primitives(const Params& P, const Ptrs& X, int i, int j, int k, double Dt,
                            double* pf, const double* U, const double* p_old, const Geom* g,
-                           double rhoflr, double uuflr, double* p_gamma, const double* src, FailRec& fr) {
+                           double rhoflr, double uuflr, double* p_gamma, const double* src, const FaceOtf* fo,
+                           FailRec& fr) {
   double pg[NP], pf2[NP];
   for (int l = 0; l < NP; l++) pg[l] = pf[l];
   double gamma = 1., bsq = 0.;
@@ -460,7 +487,7 @@ HD void recover_primitives(const Params& P, const Ptrs& X, int i, int j, int k, 
     if (!ret) { for (int l = 0; l < NP; l++) pf2[l] = pf[l]; }
     else fail_set(P, fr, NFAIL_UTOPRIM, 0, i, j, k);
     for (int l = 0; l < B1; l++) pf[l] = pg[l];
-    int ret_ee = fixup_entropy_eq<MIXED>(P, X, i, j, k, Dt, p_old, pf, g, U, &gamma, src, fr);
+    int ret_ee = fixup_entropy_eq<MIXED>(P, X, i, j, k, Dt, p_old, pf, g, U, &gamma, src, fo, fr);
     if (ret_ee) {
       if (ret) {
         fail_set(P, fr, NFAIL_UTOPRIM_EE, 1, i, j, k);
@@ -613,9 +640,11 @@ HD void store_zone(const Params& P, const Ptrs& X, int stage, long long c, const
 }
 
 // Full recover_primitives() tree for a zone whose U^{n+1} is known.
+// fo: the stage's face records are evaluated (binary metric) for the
+// entropy fluxes.
 template <bool MIXED>
 HD void recover_zone(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k, const double* U,
-                     FailRec& fr) {
+                     FailRec& fr, const FaceOtf* fo = nullptr) {
   const long long NT = P.nt;
   const long long c = cidx(P, i, j, k);
   const double Dt = (stage == 0) ? 0.5*dt : dt;
@@ -626,26 +655,34 @@ HD void recover_zone(const Params& P, const Ptrs& X, int stage, double dt, int i
   for (int v = 0; v < NP; v++) p_raw[v] = X.p[v*NT + c];
   double pgam = 1.;
   recover_primitives<MIXED>(P, X, i, j, k, Dt, pf, U, p_raw, g, floor_rho(P, X, i, c), floor_uu(P, X, i, c),
-                            &pgam, (stage == 0) ? X.p : X.ph, fr);
+                            &pgam, (stage == 0) ? X.p : X.ph, fo, fr);
   store_zone(P, X, stage, c, pf, pgam, fr);
 }
 
 // Whole zone in one pass (the host build's sequence; the device runs the two
 // passes).  Returns false and leaves U filled when the full tree is needed.
+// gn, g, gnew: the zone's records at U^n's level, the stage's and the result's.
 template <bool MIXED>
-HD bool update_zone_fast(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
-                         FailRec& fr, double* U) {
+HD bool update_zone_fast_g(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
+                           FailRec& fr, double* U, const Geom* gn, const Geom* g, const Geom* gnew) {
   const long long c = cidx(P, i, j, k);
   const double Dt = (stage == 0) ? 0.5*dt : dt;
-  const int gid = geom_id(P, i, j, k, CENT);
-  zone_conserved<MIXED>(P, X, stage, Dt, i, j, k, X.geom_n + gid, X.geom + gid, fr, U);
+  zone_conserved<MIXED>(P, X, stage, Dt, i, j, k, gn, g, fr, U);
   double pf[NP];
   zone_guess(P, X, stage, i, c, pf);
   double pgam = 1.;
-  if (!recover_fast(P, X, i, j, k, pf, U, X.geom_new + gid, floor_rho(P, X, i, c), floor_uu(P, X, i, c), &pgam, fr))
+  if (!recover_fast(P, X, i, j, k, pf, U, gnew, floor_rho(P, X, i, c), floor_uu(P, X, i, c), &pgam, fr))
     return false;
   store_zone(P, X, stage, c, pf, pgam, fr);
   return true;
+}
+
+template <bool MIXED>
+HD bool update_zone_fast(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
+                         FailRec& fr, double* U) {
+  const int gid = geom_id(P, i, j, k, CENT);
+  return update_zone_fast_g<MIXED>(P, X, stage, dt, i, j, k, fr, U, X.geom_n + gid, X.geom + gid,
+                                   X.geom_new + gid);
 }
 
 template <bool MIXED>

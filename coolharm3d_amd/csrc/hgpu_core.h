@@ -24,21 +24,6 @@ inline int set_error(int code, const std::string& msg) {
 }  // namespace hgpu
 
 namespace hgpu {
-// Time-dependent metric: the binary and the numerical coordinates of zone
-// (0,0,0)'s lower corner (startx).
-struct PnGrid {
-  PnBinary B;
-  double x0[4];
-};
-// Numerical coordinates of zone (i,j,k) at position pos (coord(), src/coord.c:182-228).
-HD void zone_X(const Params& P, const PnGrid& G, int i, int j, int k, int pos, double* X) {
-  const double h1 = (pos == FACE1 || pos == CORN) ? 0. : 0.5;
-  const double h2 = (pos == FACE2 || pos == CORN) ? 0. : 0.5;
-  const double h3 = (pos == FACE3 || pos == CORN) ? 0. : 0.5;
-  X[1] = G.x0[1] + (i + h1) * P.dx[1];
-  X[2] = G.x0[2] + (j + h2) * P.dx[2];
-  X[3] = G.x0[3] + (k + h3) * P.dx[3];
-}
 // Time of level tsel of the step: 0 t, 1 t + dt/2, 2 t + dt.
 HD double level_time(const Ptrs& X, int tsel) {
   const double t = X.scal[SC_T], dt = X.scal[SC_DT];
@@ -56,6 +41,8 @@ struct hgpu_ctx {
   hgpu::GeomF* levf[2] = {nullptr, nullptr};
   double* connlev[2] = {nullptr, nullptr};
   hgpu::PnGrid pn;
+  hgpu::PnOrbit* orbit = nullptr;  // device: the orbit at levels t, t + dt/2, t + dt
+  int otf_stage = 0;               // level of the running stage (binary metric evaluated in kernels)
   long long n_geom = 0, n_conn = 0;
   int n1_conn = 0;
   std::vector<double> stage;       // host staging buffer
@@ -248,6 +235,7 @@ struct Api {
     }
     rc = BE::init(c);
     if (rc) { delete c; return rc; }
+    c->P.otf = metric_otf_possible(c) ? 1 : 0;
     HostTables T;
     build_tables(*cfg, P, c->n_geom, c->n_conn, T);
     Ptrs& X = c->X;
@@ -270,6 +258,8 @@ struct Api {
       for (int l = 1; l < 3; l++) rc |= BE::alloc(c, (void**)&c->lev[l], sizeof(double) * GEOM_DOUBLES * c->n_geom);
       rc |= BE::alloc(c, (void**)&c->levf[1], sizeof(GeomF) * (P.mixed ? c->n_geom : 1));
       rc |= BE::alloc(c, (void**)&c->connlev[1], sizeof(double) * 64 * c->n_conn);
+      rc |= BE::alloc(c, (void**)&c->orbit, sizeof(PnOrbit) * 3);
+      X.orbit = c->orbit;
     }
     X.geom_n = X.geom_new = X.geom;
     rc |= BE::alloc(c, (void**)&X.omega, sizeof(double) * P.n1t);
@@ -328,7 +318,8 @@ struct Api {
     void* ptrs[] = {X.p, X.ph, X.F, X.emf, (void*)X.omega, (void*)X.tr,
                     (void*)X.rscale, (void*)X.irscale, (void*)X.rhoflr, (void*)X.uuflr, X.gamma,
                     X.pflag, X.nfail, X.status, X.scal, X.Ff, X.emff, X.Uslow, X.Un, X.slow_idx,
-                    c->lev[0], c->lev[1], c->lev[2], c->levf[0], c->levf[1], c->connlev[0], c->connlev[1]};
+                    c->lev[0], c->lev[1], c->lev[2], c->levf[0], c->levf[1], c->connlev[0], c->connlev[1],
+                    (void*)c->orbit};
     for (void* p : ptrs) if (p) BE::release(c, p);
     BE::fini(c);
     delete c;
@@ -416,6 +407,7 @@ struct Api {
       X.geom_n = c->lev[0];
       X.geom_new = c->lev[stage + 1];
       X.geom_s = c->lev[stage];
+      c->otf_stage = stage;
     }
     rc |= BE::reset_vmax(c);
     for (int d = 0; d < 3; d++) rc |= BE::flux(c, src, d);
@@ -437,10 +429,38 @@ struct Api {
   // connections, t + dt at zone centres.
   // Only what is read is filled: face records in FP32 for the mixed-precision
   // fluxes (FP64 otherwise), centre records in FP64; no corner records.
+  // The binary metric can be evaluated inside the flux and update kernels:
+  // the backend has those kernels, the reconstruction is PPM (the
+  // several-faces-per-thread kernels) and no direction is degenerate.
+  static bool metric_otf_possible(const hgpu_ctx* c) {
+    const Params& P = c->P;
+    return BE::HAS_METRIC_OTF && P.dyn && P.recon == RECON_PPM_MC && !P.degen[0] && !P.degen[1] && !P.degen[2];
+  }
+  static int set_metric_otf(hgpu_ctx* c, int on) {
+    if (on && !metric_otf_possible(c))
+      return set_error(-2, "the metric is evaluated in the kernels only for the binary metric with PPM "
+                           "reconstruction, no degenerate direction, on the GPU build");
+    int rc = BE::invalidate_graph(c);
+    c->P.otf = on ? 1 : 0;
+    return rc;
+  }
+
   static int fill_levels(hgpu_ctx* c) {
     const unsigned faces = (1u << FACE1) | (1u << FACE2) | (1u << FACE3), cent = 1u << CENT;
     const unsigned m64 = c->P.mixed ? cent : (faces | cent), m32 = c->P.mixed ? faces : 0u;
     int rc = 0;
+    if (c->P.otf) {
+      // the flux and update kernels evaluate face and stage records from the
+      // orbit; the stored centre levels serve the inversion tree, fixups,
+      // boundaries and diagnostics, the connection the source terms
+      rc |= BE::orbit(c);
+      rc |= BE::fill_geometry(c, c->lev[0], nullptr, 0, cent, 0u);
+      rc |= BE::fill_conn(c, c->connlev[0], 0);
+      rc |= BE::fill_geometry(c, c->lev[1], nullptr, 1, cent, 0u);
+      rc |= BE::fill_conn(c, c->connlev[1], 1);
+      rc |= BE::fill_geometry(c, c->lev[2], nullptr, 2, cent, 0u);
+      return rc;
+    }
     rc |= BE::fill_geometry(c, c->lev[0], c->levf[0], 0, m64, m32);
     rc |= BE::fill_conn(c, c->connlev[0], 0);
     rc |= BE::fill_geometry(c, c->lev[1], c->levf[1], 1, m64, m32);

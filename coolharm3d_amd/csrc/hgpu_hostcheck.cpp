@@ -199,6 +199,12 @@ struct HostBE {
     }
     return 0;
   }
+  // The host build reads stored levels only (the core never sets P.otf here).
+  static constexpr bool HAS_METRIC_OTF = false;
+  static int orbit(hgpu_ctx* c) {
+    for (int l = 0; l < 3; l++) pn_orbit_at(c->pn.B, level_time(c->X, l), &c->orbit[l]);
+    return 0;
+  }
   static int fill_conn(hgpu_ctx* c, double* dst, int tsel) {
     const Params& P = c->P;
     const double t = level_time(c->X, tsel);

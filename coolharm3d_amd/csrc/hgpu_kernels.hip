@@ -218,6 +218,23 @@ __global__ __launch_bounds__(BLOCK, W) void k_flux_mixed(Params P, Ptrs X, const
   commit_fail(P, X, fr, active, c, MASK_GAMMA, D + 1);
 }
 
+// Orbit record of a level: the same address in every lane, read through the
+// scalar cache (constant address space) so the 12 values sit in SGPRs.
+__device__ __forceinline__ PnOrbit orbit_load(const PnOrbit* p) {
+  const __attribute__((address_space(4))) double* q = (const __attribute__((address_space(4))) double*)p;
+  PnOrbit o;
+  double* d = &o.y[0][0];
+#pragma unroll
+  for (int m = 0; m < 12; m++) d[m] = q[m];
+  return o;
+}
+
+// The holes' positions and velocities at the step's three levels
+__global__ __launch_bounds__(64) void k_orbit(Ptrs X, PnBinary B, PnOrbit* out) {
+  const int l = threadIdx.x;
+  if (l < 3) pn_orbit_at(B, level_time(X, l), &out[l]);
+}
+
 // F consecutive faces along D per thread (PPM+MC, D not degenerate): the
 // thread's tile index along D counts groups of F faces from bg.lo[D]; faces
 // past hi_d (the flux box end) are skipped.  Each zone's parabola is computed
@@ -227,9 +244,11 @@ __global__ __launch_bounds__(BLOCK, W) void k_flux_mixed(Params P, Ptrs X, const
 // memory, so the fluxes go through LDS: slot s of lane l stores element
 // s*64 + l of the wavefront's F*64 faces, and each store instruction writes
 // consecutive faces.
-template <int D, int F, int W, bool MIXED>
+// OTF: the face records are evaluated from the orbit of level lvl (binary
+// metric) instead of read.
+template <int D, int F, int W, bool MIXED, bool OTF = false>
 __global__ __launch_bounds__(BLOCK, W) void k_flux_multi(Params P, Ptrs X, const double* __restrict__ src, Box bg,
-                                                         Tiling T, int hi_d) {
+                                                         Tiling T, int hi_d, PnGrid G, int lvl) {
   using Real = typename std::conditional<MIXED, float, double>::type;
   constexpr bool TRANSPOSE = (D == 2);
   __shared__ Real tbuf[TRANSPOSE ? BLOCK * F : 1];
@@ -249,13 +268,27 @@ __global__ __launch_bounds__(BLOCK, W) void k_flux_multi(Params P, Ptrs X, const
     Real pl[F][NP], pr[F][NP];
     if constexpr (MIXED) ppm_faces_mixed<D, F>(P, X, src, i, j, k, pl, pr);
     else ppm_faces<D, F>(P, X, src, i, j, k, pl, pr);
+    PnOrbit orb;
+    if constexpr (OTF) orb = orbit_load(X.orbit + lvl);
 #pragma unroll
     for (int f = 0; f < F; f++) {
       if (f >= nvalid) break;
       const int g0 = fr.nf[NFAIL_GAMMA_CALC];
       const int fi = i + f*di, fj = j + f*dj, fk = k + f*dk;
-      if constexpr (MIXED) flux_riemann_mixed<D>(P, X, pl[f], pr[f], fi, fj, fk, fr, TRANSPOSE ? fo[f] : nullptr);
-      else flux_riemann<D>(P, X, pl[f], pr[f], fi, fj, fk, fr, TRANSPOSE ? fo[f] : nullptr);
+      if constexpr (OTF) {
+        Geom gb;
+        pn_zone_geom(P, G, orb, fi, fj, fk, D, &gb);
+        if constexpr (MIXED) {
+          GeomF gfb;
+          geom_f32(gb, gfb);
+          flux_riemann_mixed_g<D>(P, X, pl[f], pr[f], fi, fj, fk, fr, TRANSPOSE ? fo[f] : nullptr, &gfb);
+        } else {
+          flux_riemann_g<D>(P, X, pl[f], pr[f], fi, fj, fk, fr, TRANSPOSE ? fo[f] : nullptr, &gb);
+        }
+      } else {
+        if constexpr (MIXED) flux_riemann_mixed<D>(P, X, pl[f], pr[f], fi, fj, fk, fr, TRANSPOSE ? fo[f] : nullptr);
+        else flux_riemann<D>(P, X, pl[f], pr[f], fi, fj, fk, fr, TRANSPOSE ? fo[f] : nullptr);
+      }
       const int dv = fr.nf[NFAIL_GAMMA_CALC] - g0;
       if (P.track_nfail && dv) X.nfail[NFAIL_GAMMA_CALC*P.nt + cidx(P, fi, fj, fk)] += dv;
     }
@@ -292,27 +325,34 @@ __global__ __launch_bounds__(BLOCK, W) void k_flux_multi(Params P, Ptrs X, const
 // per thread (PPM+MC with D not degenerate) over the box of face groups.
 template <int D, int F, bool MIXED>
 static void launch_flux_multi(const Params& P, const Ptrs& X, const LaunchShape& L, const double* src, const Box& b,
-                              hipStream_t st) {
+                              hipStream_t st, const PnGrid& G, int lvl) {
   Box bg = b;
   const int n = b.hi[D] - b.lo[D] + 1;
   bg.hi[D] = b.lo[D] + (n + F - 1) / F - 1;
   const Tiling T = make_tiling(bg, L);
   const dim3 grid(T.nblocks), blk(T.bs);
+  if (P.otf) {
+    hipLaunchKernelGGL((k_flux_multi<D, F, 1, MIXED, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
+    return;
+  }
   // the register-capped variant exists for the FP32 solve (the FP64 one spills)
-  if (MIXED && L.occ) hipLaunchKernelGGL((k_flux_multi<D, F, 4, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D]);
-  else hipLaunchKernelGGL((k_flux_multi<D, F, 1, MIXED>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D]);
+  if (MIXED && L.occ) hipLaunchKernelGGL((k_flux_multi<D, F, 4, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
+  else hipLaunchKernelGGL((k_flux_multi<D, F, 1, MIXED>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
 }
 
+// G, lvl: the binary metric's grid and the stage's level, read when P.otf
+// (which implies PPM and no degenerate direction: the face-group kernels,
+// with at least 2 faces per thread).
 template <int D>
 static void launch_flux_dir(const Params& P, const Ptrs& X, const LaunchShape& L, const double* src, const Box& b,
-                            hipStream_t st) {
-  if (L.fpt > 1 && P.recon == RECON_PPM_MC && !P.degen[D]) {
+                            hipStream_t st, const PnGrid& G, int lvl) {
+  if ((L.fpt > 1 || P.otf) && P.recon == RECON_PPM_MC && !P.degen[D]) {
     if (P.mixed) {
-      if (L.fpt == 2) launch_flux_multi<D, 2, true>(P, X, L, src, b, st);
-      else launch_flux_multi<D, 4, true>(P, X, L, src, b, st);
+      if (L.fpt == 4) launch_flux_multi<D, 4, true>(P, X, L, src, b, st, G, lvl);
+      else launch_flux_multi<D, 2, true>(P, X, L, src, b, st, G, lvl);
     } else {
-      if (L.fpt == 2) launch_flux_multi<D, 2, false>(P, X, L, src, b, st);
-      else launch_flux_multi<D, 4, false>(P, X, L, src, b, st);
+      if (L.fpt == 4) launch_flux_multi<D, 4, false>(P, X, L, src, b, st, G, lvl);
+      else launch_flux_multi<D, 2, false>(P, X, L, src, b, st, G, lvl);
     }
     return;
   }
@@ -395,8 +435,11 @@ __global__ __launch_bounds__(BLOCK) void k_invert(Params P, Ptrs X, int stage, B
 
 // Conserved update and the common con-to-prim branch in one pass: U^{n+1}
 // stays in registers; only the zones queued for k_recover store it.
-template <bool MIXED, int W>
-__global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage, Box b, Tiling T) {
+// OTF: the zone records at the stage's level (lvl, also U^n's in the
+// predictor) and the result's (lvl + 1) are evaluated from the orbit.
+template <bool MIXED, int W, bool OTF = false>
+__global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage, Box b, Tiling T, PnGrid G,
+                                                     int lvl) {
   int i, j, k;
   const bool active = tile_index(b, T, i, j, k);
   FailRec fr; fr.clear();
@@ -406,7 +449,16 @@ __global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage
     const double dt = X.scal[SC_DT];
     c = cidx(P, i, j, k);
     double U[NP];
-    if (!update_zone_fast<MIXED>(P, X, stage, dt, i, j, k, fr, U)) {
+    bool done;
+    if constexpr (OTF) {
+      Geom gs, gnew;
+      pn_zone_geom(P, G, orbit_load(X.orbit + lvl), i, j, k, CENT, &gs);
+      pn_zone_geom(P, G, orbit_load(X.orbit + lvl + 1), i, j, k, CENT, &gnew);
+      done = update_zone_fast_g<MIXED>(P, X, stage, dt, i, j, k, fr, U, &gs, &gs, &gnew);
+    } else {
+      done = update_zone_fast<MIXED>(P, X, stage, dt, i, j, k, fr, U);
+    }
+    if (!done) {
       queue = true;
 #pragma unroll
       for (int l = 0; l < NP; l++) X.Uslow[l*P.nt + c] = U[l];
@@ -426,8 +478,10 @@ __global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage
 
 // Full con-to-prim tree for the queued zones: a fixed grid strides over the
 // queue, so every wave works on queued zones only.
+// G, lvl: with P.otf, the entropy fix's face records are evaluated from the
+// orbit of level lvl.
 template <bool MIXED>
-__global__ __launch_bounds__(BLOCK) void k_recover(Params P, Ptrs X, int stage) {
+__global__ __launch_bounds__(BLOCK) void k_recover(Params P, Ptrs X, int stage, PnGrid G, int lvl) {
   const unsigned n = (unsigned)X.status[ST_SLOW];
   const unsigned stride = gridDim.x * BLOCK;
   const unsigned rounds = (n + stride - 1) / stride;
@@ -445,7 +499,9 @@ __global__ __launch_bounds__(BLOCK) void k_recover(Params P, Ptrs X, int stage) 
       double U[NP];
 #pragma unroll
       for (int l = 0; l < NP; l++) U[l] = X.Uslow[l*P.nt + c];
-      recover_zone<MIXED>(P, X, stage, dt, i, j, k, U, fr);
+      FaceOtf fo;
+      if (P.otf) { fo.G = G; fo.o = X.orbit + lvl; }
+      recover_zone<MIXED>(P, X, stage, dt, i, j, k, U, fr, P.otf ? &fo : nullptr);
     }
     commit_fail(P, X, fr, active, c, MASK_ALL);
   }
@@ -838,12 +894,19 @@ struct GpuBE {
   static void launch_fused(hgpu_ctx* c, const LaunchShape& L, const Tiling& T, const Box& b, int stage) {
     hipStream_t st = priv(c)->stream;
     const dim3 grid(T.nblocks), blk(T.bs);
+    const PnGrid& G = c->pn;
+    const int lvl = c->otf_stage;
+    if (c->P.otf) {
+      if (c->P.mixed) hipLaunchKernelGGL((k_update<true, 1, true>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      else hipLaunchKernelGGL((k_update<false, 1, true>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      return;
+    }
     if (c->P.mixed) {
-      if (L.occ) hipLaunchKernelGGL((k_update<true, 4>), grid, blk, 0, st, c->P, c->X, stage, b, T);
-      else hipLaunchKernelGGL((k_update<true, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T);
+      if (L.occ) hipLaunchKernelGGL((k_update<true, 4>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      else hipLaunchKernelGGL((k_update<true, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
     } else {
-      if (L.occ) hipLaunchKernelGGL((k_update<false, 4>), grid, blk, 0, st, c->P, c->X, stage, b, T);
-      else hipLaunchKernelGGL((k_update<false, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T);
+      if (L.occ) hipLaunchKernelGGL((k_update<false, 4>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      else hipLaunchKernelGGL((k_update<false, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
     }
   }
   // The autotuner times the conserved-update family as the whole predictor
@@ -967,9 +1030,9 @@ struct GpuBE {
     Scope s(c, "flux");
     const LaunchShape& L = priv(c)->shape[FAM_FLUX1 + d];
     hipStream_t st = priv(c)->stream;
-    if (d == 0) launch_flux_dir<0>(c->P, c->X, L, src, b, st);
-    else if (d == 1) launch_flux_dir<1>(c->P, c->X, L, src, b, st);
-    else launch_flux_dir<2>(c->P, c->X, L, src, b, st);
+    if (d == 0) launch_flux_dir<0>(c->P, c->X, L, src, b, st, c->pn, c->otf_stage);
+    else if (d == 1) launch_flux_dir<1>(c->P, c->X, L, src, b, st, c->pn, c->otf_stage);
+    else launch_flux_dir<2>(c->P, c->X, L, src, b, st, c->pn, c->otf_stage);
     return hip_check(hipGetLastError(), "k_flux");
   }
   static int emf(hgpu_ctx* c) {
@@ -987,7 +1050,7 @@ struct GpuBE {
     const Tiling T2 = make_tiling(b, priv(c)->shape[FAM_INV]);
     hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_SLOW, 1);
     hipStream_t st = priv(c)->stream;
-    if (priv(c)->shape[FAM_CONS].fused) {
+    if (priv(c)->shape[FAM_CONS].fused || c->P.otf) {   // the evaluated metric has the fused kernel only
       launch_fused(c, priv(c)->shape[FAM_CONS], T, b, stage);
       return hip_check(hipGetLastError(), "k_update");
     }
@@ -1001,8 +1064,10 @@ struct GpuBE {
     Scope s(c, "recover");
     const long long nint = (long long)c->P.n1 * c->P.n2 * c->P.n3;
     const unsigned nb = (unsigned)std::min<long long>((nint + BLOCK - 1) / BLOCK, 1024);
-    if (c->P.mixed) hipLaunchKernelGGL(k_recover<true>, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage);
-    else hipLaunchKernelGGL(k_recover<false>, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage);
+    if (c->P.mixed)
+      hipLaunchKernelGGL(k_recover<true>, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage, c->pn, stage);
+    else
+      hipLaunchKernelGGL(k_recover<false>, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage, c->pn, stage);
     return hip_check(hipGetLastError(), "k_recover");
   }
   static int bounds_dir(hgpu_ctx* c, double* S, int dir, int skip_mpi) {
@@ -1064,6 +1129,11 @@ struct GpuBE {
       rc |= hip_check(hipGetLastError(), "k_fill_geom");
     }
     return rc;
+  }
+  static constexpr bool HAS_METRIC_OTF = true;
+  static int orbit(hgpu_ctx* c) {
+    hipLaunchKernelGGL(k_orbit, dim3(1), dim3(64), 0, priv(c)->stream, c->X, c->pn.B, c->orbit);
+    return hip_check(hipGetLastError(), "k_orbit");
   }
   static int fill_conn(hgpu_ctx* c, double* dst, int tsel) {
     Scope s(c, "metric");

@@ -304,10 +304,9 @@ HD void ppm_faces_mixed(const Params& P, const Ptrs& X, const double* __restrict
 // FP32 Riemann solve of face (i,j,k) from its two states; fluxes to X.Ff or
 // to out[8]
 template <int d>
-HD void flux_riemann_mixed(const Params& P, const Ptrs& X, const float* pl, const float* pr,
-                           int i, int j, int k, FailRec& fr, float* out = nullptr) {
+HD void flux_riemann_mixed_g(const Params& P, const Ptrs& X, const float* pl, const float* pr,
+                             int i, int j, int k, FailRec& fr, float* out, const GeomF* g) {
   const long long NT = P.nt;
-  const GeomF* g = X.geomf + geom_id(P, i, j, k, d);
   constexpr int dim = d + 1;
   const float gam = (float)P.gam;
   StateF ql, qr;
@@ -346,14 +345,21 @@ HD void flux_riemann_mixed(const Params& P, const Ptrs& X, const float* pl, cons
   }
 }
 
+template <int d>
+HD void flux_riemann_mixed(const Params& P, const Ptrs& X, const float* pl, const float* pr,
+                           int i, int j, int k, FailRec& fr, float* out = nullptr) {
+  flux_riemann_mixed_g<d>(P, X, pl, pr, i, j, k, fr, out, X.geomf + geom_id(P, i, j, k, d));
+}
+
 // Entropy flux of face (i,j,k) for fixup_entropy_eq() (src/fixup.c:363-398)
 // from the FP32 face solve's states and speeds, computed for the zones that
 // take the entropy fix (the stage's face geometry is X.geomf throughout).
 template <int d>
-HD double entropy_face_mixed(const Params& P, const Ptrs& X, const double* __restrict__ src, int i, int j, int k) {
+HD double entropy_face_mixed(const Params& P, const Ptrs& X, const double* __restrict__ src, int i, int j, int k,
+                             const GeomF* gin) {
   float pl[NP], pr[NP];
   flux_recon_mixed<d>(P, X, src, i, j, k, pl, pr);
-  const GeomF* g = X.geomf + geom_id(P, i, j, k, d);
+  const GeomF* g = gin ? gin : X.geomf + geom_id(P, i, j, k, d);
   constexpr int dim = d + 1;
   const float gam = (float)P.gam;
   StateF ql, qr;

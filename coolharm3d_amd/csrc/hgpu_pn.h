@@ -136,13 +136,21 @@ HD void pn_orbit(const PnBinary& B, const T& t, T y1[3], T y2[3], T v1[3], T v2[
   v2[0] = f2 * vx; v2[1] = f2 * vy; v2[2] = t * 0.;
 }
 
-// Potentials of the 1PN near-zone metric at Cartesian point x, time t: the
-// Newtonian U, the 1PN V and the vector potential V_i, so that
-// g00 = -1 + 2V - 2U^2, g0i = -4 V_i, gij = (1 + 2U) delta_ij.
+// Positions and velocities of the holes at one time level (pn_orbit()), kept
+// so that metric evaluations at that level need not recompute the orbit.
+struct PnOrbit {
+  double y[2][3], v[2][3];
+};
+
+HD void pn_orbit_at(const PnBinary& B, double t, PnOrbit* o) {
+  pn_orbit(B, t, o->y[0], o->y[1], o->v[0], o->v[1]);
+}
+
+// Potentials of the 1PN near-zone metric at Cartesian point x for holes at y
+// moving with v: the Newtonian U, the 1PN V and the vector potential V_i, so
+// that g00 = -1 + 2V - 2U^2, g0i = -4 V_i, gij = (1 + 2U) delta_ij.
 template <class T>
-HD void pn_potentials(const PnBinary& B, const T& t, const T x[3], T& U, T& V, T Vi[3]) {
-  T y[2][3], v[2][3];
-  pn_orbit(B, t, y[0], y[1], v[0], v[1]);
+HD void pn_potentials_yv(const PnBinary& B, const T y[2][3], const T v[2][3], const T x[3], T& U, T& V, T Vi[3]) {
   const double m[2] = {B.m1, B.m2};
   T r[2], nv[2], vsq[2];
   for (int a = 0; a < 2; a++) {
@@ -154,8 +162,8 @@ HD void pn_potentials(const PnBinary& B, const T& t, const T x[3], T& U, T& V, T
   const T ex = y[0][0] - y[1][0], ey = y[0][1] - y[1][1], ez = y[0][2] - y[1][2];
   const T r12 = sqrt(ex * ex + ey * ey + ez * ez);
   const T r12_3 = r12 * r12 * r12;
-  U = t * 0.;
-  V = t * 0.;
+  U = r12 * 0.;
+  V = r12 * 0.;
   for (int a = 0; a < 2; a++) {
     const int b = 1 - a;
     const T mr = m[a] / r[a];
@@ -164,9 +172,17 @@ HD void pn_potentials(const PnBinary& B, const T& t, const T x[3], T& U, T& V, T
           + (m[a] * m[b] / 4.) * (r[b] * r[b] - r[a] * r[a] - r12 * r12) / (r[a] * r12_3);
   }
   for (int i = 0; i < 3; i++) {
-    Vi[i] = t * 0.;
+    Vi[i] = r12 * 0.;
     for (int a = 0; a < 2; a++) Vi[i] = Vi[i] + m[a] * v[a][i] / r[a];
   }
+}
+
+// The potentials at Cartesian point x, time t.
+template <class T>
+HD void pn_potentials(const PnBinary& B, const T& t, const T x[3], T& U, T& V, T Vi[3]) {
+  T y[2][3], v[2][3];
+  pn_orbit(B, t, y[0], y[1], v[0], v[1]);
+  pn_potentials_yv<T>(B, y, v, x, U, V, Vi);
 }
 
 // 1PN near-zone metric in harmonic coordinates at Cartesian point x, time t.
@@ -186,8 +202,9 @@ HD void pn_gcov_harmonic(const PnBinary& B, const T& t, const T x[3], T g[4][4])
 //   g_tt = g00,  g_tn = -4 V_i J^i_n,
 //   g_11 = A r'^2,  g_22 = A (r theta')^2,  g_33 = A (r sin theta)^2,  A = 1 + 2U,
 // with no other spatial components (r' = dr/dX1, theta' = dtheta/dX2).
-template <class T>
-HD void pn_gcov_num(const PnBinary& B, const T& t, const T& X1, const T& X2, const T& X3, T g[4][4]) {
+// pot(x, U, V, Vi) evaluates the potentials at the Cartesian point x.
+template <class T, class Pot>
+HD void pn_gcov_num_pot(const PnBinary& B, const T& X1, const T& X2, const T& X3, T g[4][4], const Pot& pot) {
   const T er = exp(X1);
   const T r = er + B.R0;
   // theta from the reflected X2, its derivative from X2 itself, as x_of_xp()
@@ -202,12 +219,12 @@ HD void pn_gcov_num(const PnBinary& B, const T& t, const T& X1, const T& X2, con
   const T rs = r * st;
   T x[3] = {rs * cp, rs * sp, r * ct};
   T U, V, Vi[3];
-  pn_potentials(B, t, x, U, V, Vi);
+  pot(x, U, V, Vi);
   // columns n = 1..3 of J = d(x, y, z)/d(X1, X2, X3)
   const T J1[3] = {st * cp * dr, st * sp * dr, ct * dr};
   const T rdth = r * dth;
   const T J2[3] = {ct * cp * rdth, ct * sp * rdth, -st * rdth};
-  const T J3[3] = {-rs * sp, rs * cp, t * 0.};
+  const T J3[3] = {-rs * sp, rs * cp, X1 * 0.};
   const T A = 1. + 2. * U;
   g[0][0] = -1. + 2. * V - 2. * U * U;
   g[0][1] = g[1][0] = -4. * (Vi[0] * J1[0] + Vi[1] * J1[1] + Vi[2] * J1[2]);
@@ -216,7 +233,19 @@ HD void pn_gcov_num(const PnBinary& B, const T& t, const T& X1, const T& X2, con
   g[1][1] = A * dr * dr;
   g[2][2] = A * rdth * rdth;
   g[3][3] = A * rs * rs;
-  g[1][2] = g[2][1] = g[1][3] = g[3][1] = g[2][3] = g[3][2] = t * 0.;
+  g[1][2] = g[2][1] = g[1][3] = g[3][1] = g[2][3] = g[3][2] = X1 * 0.;
+}
+
+template <class T>
+HD void pn_gcov_num(const PnBinary& B, const T& t, const T& X1, const T& X2, const T& X3, T g[4][4]) {
+  pn_gcov_num_pot(B, X1, X2, X3, g, [&](const T x[3], T& U, T& V, T Vi[3]) { pn_potentials(B, t, x, U, V, Vi); });
+}
+
+// The same at a time level whose orbit is known.
+HD void pn_gcov_num_orb(const PnBinary& B, const PnOrbit& o, double X1, double X2, double X3, double g[4][4]) {
+  pn_gcov_num_pot(B, X1, X2, X3, g, [&](const double x[3], double& U, double& V, double Vi[3]) {
+    pn_potentials_yv<double>(B, o.y, o.v, x, U, V, Vi);
+  });
 }
 
 // Inverse and determinant of a 4x4 matrix by the Laplace expansion in 2x2
@@ -252,9 +281,7 @@ HD double invert4(const double a[4][4], double b[4][4]) {
 // Geometry record (struct of_geom layout) at (t, X): metric, inverse, sqrt(-g),
 // lapse, shift and normal observer, as calc_all_geom() forms them
 // (src/metric.c:4221-4232).
-HD void pn_geom(const PnBinary& B, double t, double X1, double X2, double X3, Geom* G) {
-  double g[4][4];
-  pn_gcov_num(B, t, X1, X2, X3, g);
+HD void pn_geom_from_gcov(const double g[4][4], Geom* G) {
   for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) G->gcov[a][b] = g[a][b];
   const double det = invert4(g, G->gcon);
   G->g = ::sqrt(-det);
@@ -267,6 +294,39 @@ HD void pn_geom(const PnBinary& B, double t, double X1, double X2, double X3, Ge
   G->beta[0] = -G->gcon[0][1] / G->gcon[0][0];
   G->beta[1] = -G->gcon[0][2] / G->gcon[0][0];
   G->beta[2] = -G->gcon[0][3] / G->gcon[0][0];
+}
+HD void pn_geom(const PnBinary& B, double t, double X1, double X2, double X3, Geom* G) {
+  double g[4][4];
+  pn_gcov_num(B, t, X1, X2, X3, g);
+  pn_geom_from_gcov(g, G);
+}
+// The record at a time level whose orbit is known: the same values as
+// pn_geom() at that level's time.
+HD void pn_geom_orb(const PnBinary& B, const PnOrbit& o, double X1, double X2, double X3, Geom* G) {
+  double g[4][4];
+  pn_gcov_num_orb(B, o, X1, X2, X3, g);
+  pn_geom_from_gcov(g, G);
+}
+
+// Binary and grid of a context whose metric is the binary's: the numerical
+// coordinates of zone (i,j,k) at position pos (coord(), src/coord.c:182-228).
+struct PnGrid {
+  PnBinary B;
+  double x0[4];
+};
+HD void zone_X(const Params& P, const PnGrid& G, int i, int j, int k, int pos, double* X) {
+  const double h1 = (pos == FACE1 || pos == CORN) ? 0. : 0.5;
+  const double h2 = (pos == FACE2 || pos == CORN) ? 0. : 0.5;
+  const double h3 = (pos == FACE3 || pos == CORN) ? 0. : 0.5;
+  X[1] = G.x0[1] + (i + h1) * P.dx[1];
+  X[2] = G.x0[2] + (j + h2) * P.dx[2];
+  X[3] = G.x0[3] + (k + h3) * P.dx[3];
+}
+// Geometry record of zone (i,j,k) at position pos and the level of orbit o
+HD void pn_zone_geom(const Params& P, const PnGrid& G, const PnOrbit& o, int i, int j, int k, int pos, Geom* out) {
+  double X[4];
+  zone_X(P, G, i, j, k, pos, X);
+  pn_geom_orb(G.B, o, X[1], X[2], X[3], out);
 }
 
 // Connection Gamma^d_{ab} = g^{dc} (d_a g_cb + d_b g_ca - d_c g_ab) / 2 at

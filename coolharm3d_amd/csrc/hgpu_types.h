@@ -98,6 +98,8 @@ struct Params {
   int track_nfail;         // keep nfail[15][NT] per-cell counters
   int mixed;               // 1: FP32 reconstruction, face fluxes and EMFs
   int dyn;                 // 1: time-dependent metric, geometry levels refilled every step
+  int otf;                 // with dyn: face records and the update's zone records evaluated in
+                           // the kernels from X.orbit instead of read from stored levels
   long long nconn;         // connection records (interior zones); with dyn the levels are
                            // structure-of-arrays, component l of zone z at conn[l*nconn + z]
   double dx[4], invdx[4];
@@ -108,6 +110,8 @@ struct Params {
 };
 
 // Device pointers of one context.
+struct PnOrbit;
+
 struct Ptrs {
   double* p;               // primitive state p  (SoA, 8*NT)
   double* ph;              // half-step state ph (SoA, 8*NT)
@@ -117,6 +121,7 @@ struct Ptrs {
   const Geom* geom_n;      // level of U^n (time t of the step; = geom for a static metric)
   const Geom* geom_new;    // level of the stage's result (t + dt/2 or t + dt; = geom when static)
   const Geom* geom_s;      // stage's level for face records after geom moves to geom_new (null: geom)
+  const PnOrbit* orbit;    // binary metric: the holes' orbit at the step's levels t, t + dt/2, t + dt
   const GeomF* geomf;      // their FP32 copy (mixed precision)
   float* Ff;               // FP32 face fluxes [3][NFLUX][NT] (mixed precision)
   float* emff;             // FP32 corner EMFs [3][NT] (mixed precision)

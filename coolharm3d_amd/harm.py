@@ -149,6 +149,16 @@ class Harm3D:
         """Replay steps from a captured HIP graph (hgpu_set_graph)."""
         self._chk(self.lib.hgpu_set_graph(self.h, 1 if on else 0), "hgpu_set_graph")
 
+    def set_metric_eval(self, in_kernels=True):
+        """Binary metric: evaluate face and update records inside the kernels
+        (True) or fill and read stored levels (False) -- hgpu_set_metric_eval."""
+        self._chk(self.lib.hgpu_set_metric_eval(self.h, 1 if in_kernels else 0), "hgpu_set_metric_eval")
+
+    def metric_eval(self):
+        v = np.zeros(1, np.int32)
+        self._chk(self.lib.hgpu_get_metric_eval(self.h, _ip(v)), "hgpu_get_metric_eval")
+        return bool(v[0])
+
     FAMILIES = ("flux1", "flux2", "flux3", "emf", "conserved", "invert")
 
     def autotune(self, reps=2):

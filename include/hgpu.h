@@ -119,6 +119,17 @@ int hgpu_autotune(hgpu_ctx *ctx, int reps, double *report);
 int hgpu_set_launch_shape(hgpu_ctx *ctx, int family, int bs, int kc, int flags);
 int hgpu_get_launch_shape(hgpu_ctx *ctx, int family, int *bs, int *kc, int *flags);
 
+/* Binary metric (metric = 3): in_kernels = 1 evaluates the face records and
+ * the conserved update's zone records inside the flux and update kernels from
+ * the holes' orbit at each level (computed once per step); 0 fills and reads
+ * stored levels for them too.  Both give the same results.  Default 1 where
+ * possible: the GPU build with PPM reconstruction and no degenerate direction
+ * (an error is returned otherwise).  The centre levels used by the inversion
+ * tree, fixups and boundaries and the connection levels stay stored.  No
+ * counterpart in the reference (whose binary metric bodies are absent). */
+int hgpu_set_metric_eval(hgpu_ctx *ctx, int in_kernels);
+int hgpu_get_metric_eval(hgpu_ctx *ctx, int *in_kernels);
+
 /* One advance() stage (0: predictor into ph, 1: corrector into p) followed by
  * bounds and fixups, as step_ch() sequences them (src/step_ch.c:92-180). */
 int hgpu_substep(hgpu_ctx *ctx, int stage);

@@ -153,12 +153,14 @@ def test_gpu_metric_matches_host():
 # ---------------------------------------------------------------------------
 # the advance on the binary metric (geometry levels t, t + dt/2, t + dt per step)
 
-def _bbh_run(hostcheck, precision="fp64", steps=3, graph=False, shape=(20, 16, 24)):
+def _bbh_run(hostcheck, precision="fp64", steps=3, graph=False, shape=(20, 16, 24), in_kernels=None):
     from coolharm3d_amd import Harm3D, bbh_grid, bbh_disk
     from coolharm3d_amd.problems import bbh_gdet
     g = bbh_grid(*shape)
     p0 = bbh_disk(g, bbh_gdet(g, PnMetric, hostcheck=True))
-    H = Harm3D(g, hostcheck=hostcheck, precision=precision)
+    H = Harm3D(g, hostcheck=hostcheck, precision=precision, track_nfail=1)
+    if in_kernels is not None:
+        H.set_metric_eval(in_kernels)
     H.set_prims(p0)
     H.fixup_bounds()
     H.set_time(0.0, 1.0e-3)
@@ -198,6 +200,33 @@ def test_gpu_bbh_step_matches_host(precision):
     print(f"[bbh] {precision} GPU vs host rel L1 after 3 steps: {l1}", flush=True)
     assert np.all(l1 <= (1e-9 if precision == "fp64" else 1e-6))
     assert G.time() == pytest.approx(H.time(), rel=1e-12)
+
+
+def test_metric_eval_modes_host_build():
+    """The host build reads stored levels only: asking it to evaluate the
+    metric inside the kernels is an error, and it reports the stored mode."""
+    from coolharm3d_amd import Harm3D, bbh_grid
+    from coolharm3d_amd._lib import HgpuError
+    H = Harm3D(bbh_grid(12, 8, 8), hostcheck=True)
+    assert H.metric_eval() is False
+    with pytest.raises(HgpuError, match="evaluated in the kernels"):
+        H.set_metric_eval(True)
+    H.set_metric_eval(False)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_bbh_metric_in_kernels_matches_stored(precision):
+    """Face and update records evaluated inside the kernels from the level's
+    orbit give the same state, flags and counters, bit for bit, as the
+    stored levels (the same functions at the same times)."""
+    A, _ = _bbh_run(False, precision, steps=3, in_kernels=True)
+    B, _ = _bbh_run(False, precision, steps=3, in_kernels=False)
+    assert A.metric_eval() and not B.metric_eval()
+    assert np.array_equal(A.prims(), B.prims())
+    assert np.array_equal(A.pflag(), B.pflag())
+    assert np.array_equal(A.nfail(), B.nfail())
+    assert A.time() == B.time()
 
 
 @pytest.mark.gpu
