@@ -516,7 +516,18 @@ __global__ __launch_bounds__(BLOCK) void k_bounds(Params P, Ptrs X, double* S, i
   long long c = 0;
   if (active) {
     int sg, a, bb;
-    b.at(t, sg, a, bb);
+    if (dir == 2) {
+      // x3 ghosts: the 2 NG layers of one (i, j) row are the fastest index, so
+      // a wavefront's accesses stay within the rows' ends rather than one
+      // cache line per lane
+      const int nsg = b.hi[0] - b.lo[0] + 1, nb = b.hi[2] - b.lo[2] + 1;
+      sg = b.lo[0] + (int)(t % nsg);
+      const long long r = t / nsg;
+      bb = b.lo[2] + (int)(r % nb);
+      a = b.lo[1] + (int)(r / nb);
+    } else {
+      b.at(t, sg, a, bb);
+    }
     const int side = sg / NG, g = sg % NG;
     bound_zone(P, X, S, dir, side, g, a, bb, do_pflag, fr);
     // nfail of inflow_check() belongs to the ghost zone
