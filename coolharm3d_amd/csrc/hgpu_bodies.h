@@ -335,7 +335,15 @@ HD double face_flux(const Params& P, const Ptrs& X, int d, int l, int i, int j, 
 }
 
 // --------------------------------------------------------------------------
-// source(), src/phys.c:121-187, with cooling_func_hr_disk(), src/phys.c:649-784
+// source(), src/phys.c:121-187, with cooling_func_hr_disk(), src/phys.c:649-784.
+// The binary metric (P.dyn) contracts the stress tensor with the metric's
+// first derivatives, dU_nu = T^{mu lambda} d_nu g_{mu lambda} / 2, which equals
+// T^mu_kappa Gamma^kappa_{nu mu} for the metric connection; its derivative
+// records are the stage's stored level X.conn ([PN_NDG][zones]).
+// MET: -1 decides between the static and the binary metric at run time;
+// 0 / 1 compile only the static / binary branch (kernels built per metric, so
+// the branch not taken costs no registers).
+template <int MET = -1>
 HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, int i, int j, int k,
               double* dU) {
   for (int l = 0; l < NP; l++) dU[l] = 0.;
@@ -349,17 +357,34 @@ HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, i
       double mhd[4][4];
       mhd_calc(&q, 0, mhd[0]); mhd_calc(&q, 1, mhd[1]);
       mhd_calc(&q, 2, mhd[2]); mhd_calc(&q, 3, mhd[3]);
-      // static metric: records [zone][64]; time-dependent: components [64][zone]
-      // (coalesced across the wavefront)
       const long long cid = conn_id(P, i, j, k);
-      const double* cn = P.dyn ? X.conn + cid : X.conn + cid * 64;
-      const long long cs = P.dyn ? P.nconn : 1;
-      for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) {
-        dU[U1] += mhd[a][b]*cn[((b*4 + 1)*4 + a)*cs];
-        dU[U2] += mhd[a][b]*cn[((b*4 + 2)*4 + a)*cs];
-        if (P.metric_dim > 2) {
-          dU[UU] += mhd[a][b]*cn[((b*4 + 0)*4 + a)*cs];
-          dU[U3] += mhd[a][b]*cn[((b*4 + 3)*4 + a)*cs];
+      const bool dyn = (MET < 0) ? (P.dyn != 0) : (MET >= 1);
+      if (dyn) {
+        // T^{mu lambda} = T^mu_kappa g^{kappa lambda}, formed per component
+        // for the pairs the derivative records hold
+        auto Tu = [&](int a, int b) {
+          return mhd[a][0]*g->gcon[0][b] + mhd[a][1]*g->gcon[1][b] + mhd[a][2]*g->gcon[2][b]
+               + mhd[a][3]*g->gcon[3][b];
+        };
+        const double* dg = X.conn + cid;
+        const long long cs = P.nconn;
+        const double w[PN_NG] = {Tu(0, 0), Tu(0, 1) + Tu(1, 0), Tu(0, 2) + Tu(2, 0), Tu(0, 3) + Tu(3, 0),
+                                 Tu(1, 1), Tu(2, 2), Tu(3, 3)};
+        for (int nu = 0; nu < 4; nu++) {
+          double sum = 0.;
+          for (int m = 0; m < PN_NG; m++) sum += w[m]*dg[(nu*PN_NG + m)*cs];
+          dU[UU + nu] += 0.5*sum;
+        }
+      } else {
+        // static metric: records [zone][64] of Gamma^b_{nu a}
+        const double* cn = X.conn + cid * 64;
+        for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) {
+          dU[U1] += mhd[a][b]*cn[((b*4 + 1)*4 + a)];
+          dU[U2] += mhd[a][b]*cn[((b*4 + 2)*4 + a)];
+          if (P.metric_dim > 2) {
+            dU[UU] += mhd[a][b]*cn[((b*4 + 0)*4 + a)];
+            dU[U3] += mhd[a][b]*cn[((b*4 + 3)*4 + a)];
+          }
         }
       }
     }
@@ -539,7 +564,7 @@ HD double floor_uu(const Params& P, const Ptrs& X, int i, long long c) {
 // U^{n+1} of the zone (src/step_ch.c:330-375)
 // gn: geometry of U^n (time t); g: geometry of the stage (source terms).  With a
 // static metric both are the zone's one record.
-template <bool MIXED>
+template <bool MIXED, int MET = -1>
 HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int i, int j, int k,
                        const Geom* gn, const Geom* g, FailRec& fr, double* U) {
   const long long NT = P.nt;
@@ -565,7 +590,7 @@ HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int
     const double* src = (stage == 0) ? X.p : X.ph;
 #pragma unroll
     for (int v = 0; v < NP; v++) phz[v] = rt_val(P, X, v, i, src[v*NT + c]);
-    fr.nf[NFAIL_GAMMA_CALC] += source(P, X, phz, g, i, j, k, dU);
+    fr.nf[NFAIL_GAMMA_CALC] += source<MET>(P, X, phz, g, i, j, k, dU);
   }
   if (MIXED) {
 #pragma unroll
@@ -662,12 +687,12 @@ HD void recover_zone(const Params& P, const Ptrs& X, int stage, double dt, int i
 // Whole zone in one pass (the host build's sequence; the device runs the two
 // passes).  Returns false and leaves U filled when the full tree is needed.
 // gn, g, gnew: the zone's records at U^n's level, the stage's and the result's.
-template <bool MIXED>
+template <bool MIXED, int MET = -1>
 HD bool update_zone_fast_g(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
                            FailRec& fr, double* U, const Geom* gn, const Geom* g, const Geom* gnew) {
   const long long c = cidx(P, i, j, k);
   const double Dt = (stage == 0) ? 0.5*dt : dt;
-  zone_conserved<MIXED>(P, X, stage, Dt, i, j, k, gn, g, fr, U);
+  zone_conserved<MIXED, MET>(P, X, stage, Dt, i, j, k, gn, g, fr, U);
   double pf[NP];
   zone_guess(P, X, stage, i, c, pf);
   double pgam = 1.;
@@ -677,12 +702,12 @@ HD bool update_zone_fast_g(const Params& P, const Ptrs& X, int stage, double dt,
   return true;
 }
 
-template <bool MIXED>
+template <bool MIXED, int MET = -1>
 HD bool update_zone_fast(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
                          FailRec& fr, double* U) {
   const int gid = geom_id(P, i, j, k, CENT);
-  return update_zone_fast_g<MIXED>(P, X, stage, dt, i, j, k, fr, U, X.geom_n + gid, X.geom + gid,
-                                   X.geom_new + gid);
+  return update_zone_fast_g<MIXED, MET>(P, X, stage, dt, i, j, k, fr, U, X.geom_n + gid, X.geom + gid,
+                                        X.geom_new + gid);
 }
 
 template <bool MIXED>

@@ -250,14 +250,17 @@ struct Api {
     rc |= BE::alloc(c, (void**)&X.emff, sizeof(float) * (P.mixed ? 3 * NT : 2));
     rc |= BE::alloc(c, (void**)&c->levf[0], sizeof(GeomF) * (P.mixed ? c->n_geom : 1));
     rc |= BE::alloc(c, (void**)&c->lev[0], sizeof(double) * GEOM_DOUBLES * c->n_geom);
-    rc |= BE::alloc(c, (void**)&c->connlev[0], sizeof(double) * 64 * c->n_conn);
+    // connection records [zone][64] of a static metric; derivative levels
+    // [PN_NDG][zone] of the binary metric
+    const long long nconn_rec = P.dyn ? PN_NDG : 64;
+    rc |= BE::alloc(c, (void**)&c->connlev[0], sizeof(double) * nconn_rec * c->n_conn);
     X.geom = c->lev[0];
     X.geomf = c->levf[0];
     X.conn = c->connlev[0];
     if (P.dyn) {
       for (int l = 1; l < 3; l++) rc |= BE::alloc(c, (void**)&c->lev[l], sizeof(double) * GEOM_DOUBLES * c->n_geom);
       rc |= BE::alloc(c, (void**)&c->levf[1], sizeof(GeomF) * (P.mixed ? c->n_geom : 1));
-      rc |= BE::alloc(c, (void**)&c->connlev[1], sizeof(double) * 64 * c->n_conn);
+      rc |= BE::alloc(c, (void**)&c->connlev[1], sizeof(double) * nconn_rec * c->n_conn);
       rc |= BE::alloc(c, (void**)&c->orbit, sizeof(PnOrbit) * 3);
       X.orbit = c->orbit;
     }
@@ -386,7 +389,8 @@ struct Api {
   // level 0; fixup_bounds() and the diagnostics use it.
   static int set_levels_at_state(hgpu_ctx* c) {
     Ptrs& X = c->X;
-    int rc = BE::fill_geometry(c, c->lev[0], c->levf[0], 0, 1u << CENT, 0u);
+    int rc = BE::orbit(c);
+    rc |= BE::fill_geometry(c, c->lev[0], c->levf[0], 0, 1u << CENT, 0u);
     X.geom = X.geom_n = X.geom_new = c->lev[0];
     X.geomf = c->levf[0];
     X.conn = c->connlev[0];
@@ -448,12 +452,11 @@ struct Api {
   static int fill_levels(hgpu_ctx* c) {
     const unsigned faces = (1u << FACE1) | (1u << FACE2) | (1u << FACE3), cent = 1u << CENT;
     const unsigned m64 = c->P.mixed ? cent : (faces | cent), m32 = c->P.mixed ? faces : 0u;
-    int rc = 0;
+    int rc = BE::orbit(c);   // the holes at t, t + dt/2, t + dt, read by every fill
     if (c->P.otf) {
-      // the flux and update kernels evaluate face and stage records from the
+      // the flux and update kernels evaluate face and zone records from the
       // orbit; the stored centre levels serve the inversion tree, fixups,
-      // boundaries and diagnostics, the connection the source terms
-      rc |= BE::orbit(c);
+      // boundaries and diagnostics, the derivative levels the source terms
       rc |= BE::fill_geometry(c, c->lev[0], nullptr, 0, cent, 0u);
       rc |= BE::fill_conn(c, c->connlev[0], 0);
       rc |= BE::fill_geometry(c, c->lev[1], nullptr, 1, cent, 0u);

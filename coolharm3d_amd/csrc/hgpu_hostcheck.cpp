@@ -181,18 +181,15 @@ struct HostBE {
   }
   static int fill_geometry(hgpu_ctx* c, Geom* dst, GeomF* dstf, int tsel, unsigned mask64, unsigned mask32) {
     const Params& P = c->P;
-    const double t = level_time(c->X, tsel);
     for (int pos = 0; pos < NPOS; pos++) {
       const bool f64 = (mask64 >> pos) & 1u, f32 = ((mask32 >> pos) & 1u) && dstf;
       if (!f64 && !f32) continue;
       for (int i = 0; i < P.n1t; i++)
         for (int j = 0; j < P.n2t; j++)
           for (int k = 0; k < P.n3t; k++) {
-            double Xc[4];
-            zone_X(P, c->pn, i, j, k, pos, Xc);
             const int id = geom_id(P, i, j, k, pos);
             Geom g;
-            pn_geom(c->pn.B, t, Xc[1], Xc[2], Xc[3], &g);
+            pn_zone_geom(P, c->pn, c->orbit[tsel], i, j, k, pos, &g);
             if (f64) dst[id] = g;
             if (f32) geom_f32(g, dstf[id]);
           }
@@ -207,15 +204,14 @@ struct HostBE {
   }
   static int fill_conn(hgpu_ctx* c, double* dst, int tsel) {
     const Params& P = c->P;
-    const double t = level_time(c->X, tsel);
     for (int i = NG; i < NG + P.n1; i++)
       for (int j = NG; j < NG + P.n2; j++)
         for (int k = NG; k < NG + P.n3; k++) {
-          double Xc[4], cn[64];
+          double Xc[4], dg[PN_NDG];
           zone_X(P, c->pn, i, j, k, CENT, Xc);
-          pn_conn(c->pn.B, t, Xc[1], Xc[2], Xc[3], cn);
+          pn_metric_dg(c->pn.B, c->orbit[tsel], Xc[1], Xc[2], Xc[3], dg);
           const long long cid = conn_id(P, i, j, k);
-          for (int l = 0; l < 64; l++) dst[l * P.nconn + cid] = cn[l];
+          for (int l = 0; l < PN_NDG; l++) dst[l * P.nconn + cid] = dg[l];
         }
     return 0;
   }

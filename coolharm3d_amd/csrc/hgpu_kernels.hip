@@ -437,7 +437,9 @@ __global__ __launch_bounds__(BLOCK) void k_invert(Params P, Ptrs X, int stage, B
 // stays in registers; only the zones queued for k_recover store it.
 // OTF: the zone records at the stage's level (lvl, also U^n's in the
 // predictor) and the result's (lvl + 1) are evaluated from the orbit.
-template <bool MIXED, int W, bool OTF = false>
+// MET: 0 static metric, 1 binary metric (stored levels or, with OTF,
+// evaluated records); each kernel carries only its metric's source branch.
+template <bool MIXED, int W, bool OTF = false, int MET = 0>
 __global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage, Box b, Tiling T, PnGrid G,
                                                      int lvl) {
   int i, j, k;
@@ -451,12 +453,24 @@ __global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage
     double U[NP];
     bool done;
     if constexpr (OTF) {
-      Geom gs, gnew;
-      pn_zone_geom(P, G, orbit_load(X.orbit + lvl), i, j, k, CENT, &gs);
+      // the stage's record for U^{n+1} (the metric derivatives of the source
+      // term come from the stage's stored level: evaluating them here, by
+      // forward-mode differentiation, needs ~200 VGPRs on top of the
+      // update's), then the result's record for the inversion
+      {
+        Geom gs;
+        pn_zone_geom(P, G, orbit_load(X.orbit + lvl), i, j, k, CENT, &gs);
+        zone_conserved<MIXED, 1>(P, X, stage, (stage == 0) ? 0.5*dt : dt, i, j, k, &gs, &gs, fr, U);
+      }
+      Geom gnew;
       pn_zone_geom(P, G, orbit_load(X.orbit + lvl + 1), i, j, k, CENT, &gnew);
-      done = update_zone_fast_g<MIXED>(P, X, stage, dt, i, j, k, fr, U, &gs, &gs, &gnew);
+      double pf[NP];
+      zone_guess(P, X, stage, i, c, pf);
+      double pgam = 1.;
+      done = recover_fast(P, X, i, j, k, pf, U, &gnew, floor_rho(P, X, i, c), floor_uu(P, X, i, c), &pgam, fr);
+      if (done) store_zone(P, X, stage, c, pf, pgam, fr);
     } else {
-      done = update_zone_fast<MIXED>(P, X, stage, dt, i, j, k, fr, U);
+      done = update_zone_fast<MIXED, MET>(P, X, stage, dt, i, j, k, fr, U);
     }
     if (!done) {
       queue = true;
@@ -621,9 +635,7 @@ __global__ __launch_bounds__(FILL_B) void k_fill_geom(Params P, Ptrs X, PnGrid G
     const int k = (int)(z % P.n3t);
     const int j = (int)((z / P.n3t) % P.n2t);
     const int i = (int)(z / ((long long)P.n3t * P.n2t));
-    double Xc[4];
-    zone_X(P, G, i, j, k, pos, Xc);
-    pn_geom(G.B, level_time(X, tsel), Xc[1], Xc[2], Xc[3], &g);
+    pn_zone_geom(P, G, X.orbit[tsel], i, j, k, pos, &g);
   }
   // records of position pos are contiguous: geom_id = pos * NT + zone index
   if (dst) {
@@ -659,10 +671,10 @@ __global__ __launch_bounds__(BLOCK) void k_fill_conn(Params P, Ptrs X, PnGrid G,
   const int i = NG + (int)(z / ((long long)P.n3 * P.n2));
   double Xc[4];
   zone_X(P, G, i, j, k, CENT, Xc);
-  double cn[64];
-  pn_conn(G.B, level_time(X, tsel), Xc[1], Xc[2], Xc[3], cn);
+  double dg[PN_NDG];
+  pn_metric_dg(G.B, X.orbit[tsel], Xc[1], Xc[2], Xc[3], dg);
   const long long cid = conn_id(P, i, j, k);
-  for (int l = 0; l < 64; l++) dst[l * P.nconn + cid] = cn[l];   // structure-of-arrays levels
+  for (int l = 0; l < PN_NDG; l++) dst[l * P.nconn + cid] = dg[l];   // structure-of-arrays levels
 }
 
 // Batched Utoprim_2d_fast() on independent zones (SoA buffers).
@@ -908,8 +920,13 @@ struct GpuBE {
     const PnGrid& G = c->pn;
     const int lvl = c->otf_stage;
     if (c->P.otf) {
-      if (c->P.mixed) hipLaunchKernelGGL((k_update<true, 1, true>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
-      else hipLaunchKernelGGL((k_update<false, 1, true>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      if (c->P.mixed) hipLaunchKernelGGL((k_update<true, 1, true, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      else hipLaunchKernelGGL((k_update<false, 1, true, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      return;
+    }
+    if (c->P.dyn) {
+      if (c->P.mixed) hipLaunchKernelGGL((k_update<true, 1, false, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      else hipLaunchKernelGGL((k_update<false, 1, false, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
       return;
     }
     if (c->P.mixed) {

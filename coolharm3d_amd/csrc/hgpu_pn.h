@@ -136,14 +136,21 @@ HD void pn_orbit(const PnBinary& B, const T& t, T y1[3], T y2[3], T v1[3], T v2[
   v2[0] = f2 * vx; v2[1] = f2 * vy; v2[2] = t * 0.;
 }
 
-// Positions and velocities of the holes at one time level (pn_orbit()), kept
-// so that metric evaluations at that level need not recompute the orbit.
+// Positions and velocities of the holes at one time level (pn_orbit()) and
+// their time derivatives, kept so that metric evaluations at that level need
+// not recompute the orbit.  The first 12 values are the positions and
+// velocities.
 struct PnOrbit {
   double y[2][3], v[2][3];
+  double ydot[2][3], vdot[2][3];
 };
 
 HD void pn_orbit_at(const PnBinary& B, double t, PnOrbit* o) {
   pn_orbit(B, t, o->y[0], o->y[1], o->v[0], o->v[1]);
+  Dual4 yd[2][3], vd[2][3];
+  pn_orbit(B, dvar(t, 0), yd[0], yd[1], vd[0], vd[1]);
+  for (int a = 0; a < 2; a++)
+    for (int i = 0; i < 3; i++) { o->ydot[a][i] = yd[a][i].d[0]; o->vdot[a][i] = vd[a][i].d[0]; }
 }
 
 // Potentials of the 1PN near-zone metric at Cartesian point x for holes at y
@@ -248,6 +255,37 @@ HD void pn_gcov_num_orb(const PnBinary& B, const PnOrbit& o, double X1, double X
   });
 }
 
+// Components of the numerical metric that are not identically zero (the
+// spatial block is diagonal), in the order the derivative records use.
+constexpr int PN_NG = 7;
+HD void pn_gcomp(int m, int* a, int* b) {
+  const int A[PN_NG] = {0, 0, 0, 0, 1, 2, 3}, Bv[PN_NG] = {0, 1, 2, 3, 1, 2, 3};
+  *a = A[m]; *b = Bv[m];
+}
+constexpr int PN_NDG = 4 * PN_NG;   // dg[nu * PN_NG + m] = d g_{ab(m)} / dx^nu, x = (t, X1, X2, X3)
+
+// First derivatives of the numerical metric at (X1, X2, X3) and the level of
+// orbit o, by forward-mode differentiation with the orbit's time derivatives.
+HD void pn_metric_dg(const PnBinary& B, const PnOrbit& o, double X1, double X2, double X3, double* dg) {
+  Dual4 yd[2][3], vd[2][3];
+  for (int a = 0; a < 2; a++)
+    for (int i = 0; i < 3; i++) {
+      yd[a][i] = dc(o.y[a][i]); yd[a][i].d[0] = o.ydot[a][i];
+      vd[a][i] = dc(o.v[a][i]); vd[a][i].d[0] = o.vdot[a][i];
+    }
+  Dual4 g[4][4];
+  pn_gcov_num_pot(B, dvar(X1, 1), dvar(X2, 2), dvar(X3, 3), g,
+                  [&](const Dual4 x[3], Dual4& U, Dual4& V, Dual4 Vi[3]) {
+                    pn_potentials_yv<Dual4>(B, yd, vd, x, U, V, Vi);
+                  });
+  for (int nu = 0; nu < 4; nu++)
+    for (int m = 0; m < PN_NG; m++) {
+      int a, b;
+      pn_gcomp(m, &a, &b);
+      dg[nu * PN_NG + m] = g[a][b].d[nu];
+    }
+}
+
 // Inverse and determinant of a 4x4 matrix by the Laplace expansion in 2x2
 // minors (closed form: no pivoting, no indexed scratch).
 HD double invert4(const double a[4][4], double b[4][4]) {
@@ -332,21 +370,34 @@ HD void pn_zone_geom(const Params& P, const PnGrid& G, const PnOrbit& o, int i, 
 // Connection Gamma^d_{ab} = g^{dc} (d_a g_cb + d_b g_ca - d_c g_ab) / 2 at
 // (t, X), time derivatives included; conn[(d*4 + a)*4 + b] (conn[l][d][a][b]).
 HD void pn_conn(const PnBinary& B, double t, double X1, double X2, double X3, double* conn) {
-  Dual4 g[4][4];
-  pn_gcov_num(B, dvar(t, 0), dvar(X1, 1), dvar(X2, 2), dvar(X3, 3), g);
-  double gv[4][4], gi[4][4];
-  for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) gv[a][b] = g[a][b].v;
+  // the connection from the same metric derivatives the step's source term
+  // uses (pn_metric_dg()), at the orbit of time t
+  PnOrbit o;
+  pn_orbit_at(B, t, &o);
+  double gv[4][4], gi[4][4], dg[PN_NDG];
+  pn_gcov_num_orb(B, o, X1, X2, X3, gv);
+  pn_metric_dg(B, o, X1, X2, X3, dg);
   invert4(gv, gi);
+  double d[4][4][4];   // d[c][a][b] = d g_ab / dx^c
+  for (int c = 0; c < 4; c++)
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) d[c][a][b] = 0.;
+  for (int c = 0; c < 4; c++)
+    for (int m = 0; m < PN_NG; m++) {
+      int a, b;
+      pn_gcomp(m, &a, &b);
+      d[c][a][b] = d[c][b][a] = dg[c * PN_NG + m];
+    }
   double low[4][4][4];   // Gamma_{c a b}
   for (int c = 0; c < 4; c++)
     for (int a = 0; a < 4; a++)
-      for (int b = 0; b < 4; b++) low[c][a][b] = 0.5 * (g[c][b].d[a] + g[c][a].d[b] - g[a][b].d[c]);
-  for (int d = 0; d < 4; d++)
+      for (int b = 0; b < 4; b++) low[c][a][b] = 0.5 * (d[a][c][b] + d[b][c][a] - d[c][a][b]);
+  for (int e = 0; e < 4; e++)
     for (int a = 0; a < 4; a++)
       for (int b = 0; b < 4; b++) {
         double s = 0.;
-        for (int c = 0; c < 4; c++) s += gi[d][c] * low[c][a][b];
-        conn[(d * 4 + a) * 4 + b] = s;
+        for (int c = 0; c < 4; c++) s += gi[e][c] * low[c][a][b];
+        conn[(e * 4 + a) * 4 + b] = s;
       }
 }
 

@@ -127,6 +127,32 @@ def test_connection_against_finite_differences(inspiral):
         assert abs(conn[0, 0, 0]) > 0
 
 
+def test_source_from_metric_derivatives_equals_connection_form():
+    """The step's source term for the binary metric contracts the stress
+    tensor with the metric's derivatives, T^{mu lambda} d_nu g_{mu lambda} / 2;
+    for a metric connection that equals T^mu_kappa Gamma^kappa_{nu mu}.  Here
+    with this metric's Gamma (time derivatives included) and d g formed from
+    it, for random symmetric stress tensors (the identity needs T^{mu lambda}
+    symmetric, as the MHD stress tensor is)."""
+    rng = np.random.default_rng(5)
+    X, t = (math.log(40.0), 0.37, 1.3), 50.0
+    g42 = geom(X, t)
+    gcv, gcn = gcov(g42), gcon(g42)
+    G = PnMetric(one_zone(*X), hostcheck=True).connection(t).reshape(4, 4, 4)   # G[k][nu][mu]
+    dg = np.einsum("ak,kbc->cab", gcv, G) + np.einsum("bk,kac->cab", gcv, G)    # dg[c][a][b] = d_c g_ab
+    # the spatial block of this metric is diagonal: its off-diagonal derivatives vanish
+    for c in range(4):
+        for a, b in ((1, 2), (1, 3), (2, 3)):
+            assert abs(dg[c, a, b]) <= 1e-12 * np.abs(dg[c]).max()
+    for _ in range(5):
+        S = rng.normal(size=(4, 4))
+        Tuu = S + S.T                                      # T^{mu lambda}
+        Tmix = Tuu @ gcv                                   # T^mu_kappa
+        a = np.einsum("mk,knm->n", Tmix, G)
+        b = 0.5 * np.einsum("ml,nml->n", Tuu, dg)
+        assert np.allclose(a, b, rtol=1e-10, atol=1e-12 * np.abs(a).max())
+
+
 def test_entry_point_errors():
     lib = _lib.load(hostcheck=True)
     c = one_zone(*POINTS[0])
