@@ -42,6 +42,7 @@ struct hgpu_ctx {
   double* connlev[2] = {nullptr, nullptr};
   hgpu::PnGrid pn;
   hgpu::PnOrbit* orbit = nullptr;  // device: the orbit at levels t, t + dt/2, t + dt
+  double* pn_tab = nullptr;        // device: coordinate-function tables of the binary metric
   int otf_stage = 0;               // level of the running stage (binary metric evaluated in kernels)
   long long n_geom = 0, n_conn = 0;
   int n1_conn = 0;
@@ -263,6 +264,7 @@ struct Api {
       rc |= BE::alloc(c, (void**)&c->connlev[1], sizeof(double) * nconn_rec * c->n_conn);
       rc |= BE::alloc(c, (void**)&c->orbit, sizeof(PnOrbit) * 3);
       X.orbit = c->orbit;
+      rc |= BE::alloc(c, (void**)&c->pn_tab, sizeof(double) * pn_tab_size(P));
     }
     X.geom_n = X.geom_new = X.geom;
     rc |= BE::alloc(c, (void**)&X.omega, sizeof(double) * P.n1t);
@@ -282,6 +284,15 @@ struct Api {
     rc |= BE::alloc(c, (void**)&X.scal, sizeof(double) * SC_WORDS);
     if (rc) { destroy(c); return set_error(-3, "device allocation failed: " + g_last_error); }
     rc |= BE::h2d(c, (void*)X.geom, T.geom.data(), sizeof(double) * T.geom.size());
+    if (P.dyn) {
+      // coordinate functions along each direction, evaluated once here and
+      // read by every record evaluation of the binary metric
+      std::vector<double> tab(pn_tab_size(P));
+      PnGrid G0 = c->pn;
+      for (long long e = 0; e < pn_tab_entries(P); e++) pn_tab_entry(P, G0, tab.data(), e);
+      rc |= BE::h2d(c, c->pn_tab, tab.data(), sizeof(double) * tab.size());
+      c->pn.tab = c->pn_tab;
+    }
     rc |= BE::h2d(c, (void*)X.conn, T.conn.data(), sizeof(double) * T.conn.size());
     rc |= BE::h2d(c, (void*)X.omega, T.omega.data(), sizeof(double) * T.omega.size());
     rc |= BE::h2d(c, (void*)X.tr, T.tr.data(), sizeof(double) * T.tr.size());
@@ -322,7 +333,7 @@ struct Api {
                     (void*)X.rscale, (void*)X.irscale, (void*)X.rhoflr, (void*)X.uuflr, X.gamma,
                     X.pflag, X.nfail, X.status, X.scal, X.Ff, X.emff, X.Uslow, X.Un, X.slow_idx,
                     c->lev[0], c->lev[1], c->lev[2], c->levf[0], c->levf[1], c->connlev[0], c->connlev[1],
-                    (void*)c->orbit};
+                    (void*)c->orbit, c->pn_tab};
     for (void* p : ptrs) if (p) BE::release(c, p);
     BE::fini(c);
     delete c;

@@ -210,19 +210,40 @@ HD void pn_gcov_harmonic(const PnBinary& B, const T& t, const T x[3], T g[4][4])
 //   g_11 = A r'^2,  g_22 = A (r theta')^2,  g_33 = A (r sin theta)^2,  A = 1 + 2U,
 // with no other spatial components (r' = dr/dX1, theta' = dtheta/dX2).
 // pot(x, U, V, Vi) evaluates the potentials at the Cartesian point x.
-template <class T, class Pot>
-HD void pn_gcov_num_pot(const PnBinary& B, const T& X1, const T& X2, const T& X3, T g[4][4], const Pot& pot) {
+// The coordinate functions of the spherical map, one numerical coordinate
+// each: r and dr/dX1 of X1; sin, cos of theta and dtheta/dX2 of X2; sin, cos
+// of phi = X3.
+template <class T>
+struct PnCoords {
+  T r, dr, st, ct, dth, sp, cp;
+};
+template <class T>
+HD void pn_coord1(const PnBinary& B, const T& X1, T* r, T* dr) {
   const T er = exp(X1);
-  const T r = er + B.R0;
+  *r = er + B.R0;
+  *dr = er;
+}
+template <class T>
+HD void pn_coord2(const PnBinary& B, const T& X2, T* st, T* ct, T* dth) {
   // theta from the reflected X2, its derivative from X2 itself, as x_of_xp()
   // and dx_dxp_calc() form them (src/coord.c:240-382)
   const T X2s = X2 + (seesaw_unit(value(X2)) - value(X2));
   const T th = B.th_length * X2s + B.th_beg + B.h_slope * sin(2. * M_PI * X2s);
-  const T dr = er;
-  const T dth = B.th_length + 2. * M_PI * B.h_slope * cos(2. * M_PI * X2);
-  T st = sin(th);
-  const T ct = cos(th), sp = sin(X3), cp = cos(X3);
-  if (fabs(value(st)) < 1.E-16) set_value(st, value(st) >= 0. ? 1.E-16 : -1.E-16);   // trigvalues(), src/metric.c:29-40
+  *dth = B.th_length + 2. * M_PI * B.h_slope * cos(2. * M_PI * X2);
+  *st = sin(th);
+  *ct = cos(th);
+  if (fabs(value(*st)) < 1.E-16) set_value(*st, value(*st) >= 0. ? 1.E-16 : -1.E-16);   // trigvalues(), src/metric.c:29-40
+}
+template <class T>
+HD void pn_coord3(const T& X3, T* sp, T* cp) {
+  *sp = sin(X3);
+  *cp = cos(X3);
+}
+
+// The metric from the coordinate functions; zero is a zero of type T.
+template <class T, class Pot>
+HD void pn_gcov_from_coords(const PnCoords<T>& C, const T& zero, T g[4][4], const Pot& pot) {
+  const T r = C.r, dr = C.dr, st = C.st, ct = C.ct, dth = C.dth, sp = C.sp, cp = C.cp;
   const T rs = r * st;
   T x[3] = {rs * cp, rs * sp, r * ct};
   T U, V, Vi[3];
@@ -231,7 +252,7 @@ HD void pn_gcov_num_pot(const PnBinary& B, const T& X1, const T& X2, const T& X3
   const T J1[3] = {st * cp * dr, st * sp * dr, ct * dr};
   const T rdth = r * dth;
   const T J2[3] = {ct * cp * rdth, ct * sp * rdth, -st * rdth};
-  const T J3[3] = {-rs * sp, rs * cp, X1 * 0.};
+  const T J3[3] = {-rs * sp, rs * cp, zero};
   const T A = 1. + 2. * U;
   g[0][0] = -1. + 2. * V - 2. * U * U;
   g[0][1] = g[1][0] = -4. * (Vi[0] * J1[0] + Vi[1] * J1[1] + Vi[2] * J1[2]);
@@ -240,7 +261,16 @@ HD void pn_gcov_num_pot(const PnBinary& B, const T& X1, const T& X2, const T& X3
   g[1][1] = A * dr * dr;
   g[2][2] = A * rdth * rdth;
   g[3][3] = A * rs * rs;
-  g[1][2] = g[2][1] = g[1][3] = g[3][1] = g[2][3] = g[3][2] = X1 * 0.;
+  g[1][2] = g[2][1] = g[1][3] = g[3][1] = g[2][3] = g[3][2] = zero;
+}
+
+template <class T, class Pot>
+HD void pn_gcov_num_pot(const PnBinary& B, const T& X1, const T& X2, const T& X3, T g[4][4], const Pot& pot) {
+  PnCoords<T> C;
+  pn_coord1(B, X1, &C.r, &C.dr);
+  pn_coord2(B, X2, &C.st, &C.ct, &C.dth);
+  pn_coord3(X3, &C.sp, &C.cp);
+  pn_gcov_from_coords(C, X1 * 0., g, pot);
 }
 
 template <class T>
@@ -348,10 +378,16 @@ HD void pn_geom_orb(const PnBinary& B, const PnOrbit& o, double X1, double X2, d
 
 // Binary and grid of a context whose metric is the binary's: the numerical
 // coordinates of zone (i,j,k) at position pos (coord(), src/coord.c:182-228).
+// tab: the coordinate functions at every zone's lower face (h = 0) and centre
+// (h = 1) along each direction, [2][N1T][2] (r, dr), then [2][N2T][3]
+// (sin theta, cos theta, dtheta/dX2), then [2][N3T][2] (sin phi, cos phi);
+// null: evaluate them.
 struct PnGrid {
   PnBinary B;
   double x0[4];
+  const double* tab;
 };
+HD long long pn_tab_size(const Params& P) { return 2LL * (2 * P.n1t + 3 * P.n2t + 2 * P.n3t); }
 HD void zone_X(const Params& P, const PnGrid& G, int i, int j, int k, int pos, double* X) {
   const double h1 = (pos == FACE1 || pos == CORN) ? 0. : 0.5;
   const double h2 = (pos == FACE2 || pos == CORN) ? 0. : 0.5;
@@ -360,11 +396,57 @@ HD void zone_X(const Params& P, const PnGrid& G, int i, int j, int k, int pos, d
   X[2] = G.x0[2] + (j + h2) * P.dx[2];
   X[3] = G.x0[3] + (k + h3) * P.dx[3];
 }
+// Coordinate functions of zone (i,j,k) at position pos from the tables (the
+// values pn_gcov_num() forms from the coordinates).
+HD void pn_zone_coords(const Params& P, const PnGrid& G, int i, int j, int k, int pos, PnCoords<double>& C) {
+  const int h1 = (pos == FACE1 || pos == CORN) ? 0 : 1;
+  const int h2 = (pos == FACE2 || pos == CORN) ? 0 : 1;
+  const int h3 = (pos == FACE3 || pos == CORN) ? 0 : 1;
+  const double* t1 = G.tab + 2 * (h1 * P.n1t + i);
+  const double* t2 = G.tab + 4 * P.n1t + 3 * (h2 * P.n2t + j);
+  const double* t3 = G.tab + 4 * P.n1t + 6 * P.n2t + 2 * (h3 * P.n3t + k);
+  C.r = t1[0]; C.dr = t1[1];
+  C.st = t2[0]; C.ct = t2[1]; C.dth = t2[2];
+  C.sp = t3[0]; C.cp = t3[1];
+}
+// Fill of the tables (host or device; one call per entry index e < pn_tab_size / 2)
+HD void pn_tab_entry(const Params& P, const PnGrid& G, double* tab, long long e) {
+  if (e < 2LL * P.n1t) {
+    const int h = (int)(e / P.n1t), i = (int)(e % P.n1t);
+    pn_coord1(G.B, G.x0[1] + (i + 0.5 * h) * P.dx[1], &tab[2 * e], &tab[2 * e + 1]);
+    return;
+  }
+  e -= 2LL * P.n1t;
+  if (e < 2LL * P.n2t) {
+    const int h = (int)(e / P.n2t), j = (int)(e % P.n2t);
+    double* t = tab + 4 * P.n1t + 3 * e;
+    pn_coord2(G.B, G.x0[2] + (j + 0.5 * h) * P.dx[2], &t[0], &t[1], &t[2]);
+    return;
+  }
+  e -= 2LL * P.n2t;
+  if (e < 2LL * P.n3t) {
+    const int h = (int)(e / P.n3t), k = (int)(e % P.n3t);
+    double* t = tab + 4 * P.n1t + 6 * P.n2t + 2 * e;
+    pn_coord3(G.x0[3] + (k + 0.5 * h) * P.dx[3], &t[0], &t[1]);
+  }
+}
+HD long long pn_tab_entries(const Params& P) { return 2LL * (P.n1t + P.n2t + P.n3t); }
+
 // Geometry record of zone (i,j,k) at position pos and the level of orbit o
 HD void pn_zone_geom(const Params& P, const PnGrid& G, const PnOrbit& o, int i, int j, int k, int pos, Geom* out) {
-  double X[4];
-  zone_X(P, G, i, j, k, pos, X);
-  pn_geom_orb(G.B, o, X[1], X[2], X[3], out);
+  if (!G.tab) {
+    double X[4];
+    zone_X(P, G, i, j, k, pos, X);
+    pn_geom_orb(G.B, o, X[1], X[2], X[3], out);
+    return;
+  }
+  PnCoords<double> C;
+  pn_zone_coords(P, G, i, j, k, pos, C);
+  double g[4][4];
+  pn_gcov_from_coords(C, 0., g, [&](const double x[3], double& U, double& V, double Vi[3]) {
+    pn_potentials_yv<double>(G.B, o.y, o.v, x, U, V, Vi);
+  });
+  pn_geom_from_gcov(g, out);
 }
 
 // Connection Gamma^d_{ab} = g^{dc} (d_a g_cb + d_b g_ca - d_c g_ab) / 2 at
