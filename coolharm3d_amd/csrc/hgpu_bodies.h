@@ -17,6 +17,17 @@ struct FaceOtf {
   const PnOrbit* o;
 };
 
+// Zone-centre record at the level X.geom describes: read, or (binary metric
+// evaluated in the kernels, P.otf) evaluated into buf from the orbit of level
+// X.cent_lvl.
+HD const Geom* cent_rec(const Params& P, const Ptrs& X, int i, int j, int k, Geom& buf) {
+  if (P.otf) {
+    pn_zone_geom(P, X.pn, X.orbit[X.cent_lvl], i, j, k, CENT, &buf);
+    return &buf;
+  }
+  return geom_at(P, X, i, j, k, CENT);
+}
+
 // --------------------------------------------------------------------------
 // numerical_flux() for one face, src/step_ch.c:636-790 (with the face states of
 // reconstruct_fast(), src/recon.c:894-1103).  src is the state being
@@ -673,7 +684,8 @@ HD void recover_zone(const Params& P, const Ptrs& X, int stage, double dt, int i
   const long long NT = P.nt;
   const long long c = cidx(P, i, j, k);
   const double Dt = (stage == 0) ? 0.5*dt : dt;
-  const Geom* g = geom_at(P, X, i, j, k, CENT);
+  Geom gb;
+  const Geom* g = cent_rec(P, X, i, j, k, gb);
   double pf[NP], p_raw[NP];
   zone_guess(P, X, stage, i, c, pf);
 #pragma unroll
@@ -728,7 +740,8 @@ HD void update_zone(const Params& P, const Ptrs& X, int stage, double dt, int i,
 HD void inflow_check(const Params& P, const Ptrs& X, double* pr, int ii, int jj, int kk, int type,
                      FailRec& fr) {
   const int dir = 1;
-  const Geom* g = geom_at(P, X, ii, jj, kk, CENT);
+  Geom gb;
+  const Geom* g = cent_rec(P, X, ii, jj, kk, gb);
   double gamma;
   if (gamma_calc(pr, g, &gamma)) fr.nf[NFAIL_GAMMA_CALC]++;
   double ucon[4];
@@ -820,7 +833,8 @@ HD void fixup_interp_zone(const Params& P, const Ptrs& X, double* pv, int which,
     if (pfl[o + ii] && pfl[o - ii]) ngood++;
     else pfl[o + ii] = pfl[o - ii] = 0;
   }
-  const Geom* g = geom_at(P, X, i, j, k, CENT);
+  Geom gb;
+  const Geom* g = cent_rec(P, X, i, j, k, gb);
   const int lbeg = (which == PFLAG_INTERP_V) ? U1 : RHO;
   double p[NP], pold[NP];
   for (int v = 0; v < NP; v++) p[v] = pv[v*NT + c];
@@ -859,7 +873,8 @@ HD void fixup_interp_zone(const Params& P, const Ptrs& X, double* pv, int which,
           double pn[NP];
           for (int v = 0; v < NP; v++) pn[v] = pv[v*NT + cn];
           double f;
-          i_ret = gamma_calc(pn, geom_at(P, X, i + ii, j + jj, k + kk, CENT), &f);
+          Geom gn;
+          i_ret = gamma_calc(pn, cent_rec(P, X, i + ii, j + jj, k + kk, gn), &f);
           if (!i_ret) { gamma += f; ngood2++; }
         }
       }
@@ -900,7 +915,8 @@ HD void fixup_interp_zone(const Params& P, const Ptrs& X, double* pv, int which,
 HD void fixup_init_zone(const Params& P, const Ptrs& X, int i, int j, int k, FailRec& fr) {
   const long long NT = P.nt;
   const long long c = cidx(P, i, j, k);
-  const Geom* g = geom_at(P, X, i, j, k, CENT);
+  Geom gb;
+  const Geom* g = cent_rec(P, X, i, j, k, gb);
   double p[NP];
   for (int v = 0; v < NP; v++) p[v] = X.p[v*NT + c];
   double gamma;
@@ -943,7 +959,8 @@ HD double divb_corner(const Params& P, const Ptrs& X, const double* S, int i, in
   int id = 0;
   for (int di = -1; di <= 0; di++) for (int dj = -1; dj <= 0; dj++) for (int dk = -1; dk <= 0; dk++) {
     long long c = cidx(P, i + di, j + dj, k + dk);
-    det_g[id] = geom_at(P, X, i + di, j + dj, k + dk, CENT)->g;
+    Geom gb;
+    det_g[id] = cent_rec(P, X, i + di, j + dj, k + dk, gb)->g;
     double B1v = S[B1*NT + c], B2v = S[B2*NT + c], B3v = S[B3*NT + c];
     b1[id] = (di == 0) ? B1v : -B1v;
     b2[id] = (dj == 0) ? B2v : -B2v;

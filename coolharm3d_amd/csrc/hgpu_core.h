@@ -292,6 +292,7 @@ struct Api {
       for (long long e = 0; e < pn_tab_entries(P); e++) pn_tab_entry(P, G0, tab.data(), e);
       rc |= BE::h2d(c, c->pn_tab, tab.data(), sizeof(double) * tab.size());
       c->pn.tab = c->pn_tab;
+      X.pn = c->pn;
     }
     rc |= BE::h2d(c, (void*)X.conn, T.conn.data(), sizeof(double) * T.conn.size());
     rc |= BE::h2d(c, (void*)X.omega, T.omega.data(), sizeof(double) * T.omega.size());
@@ -401,8 +402,9 @@ struct Api {
   static int set_levels_at_state(hgpu_ctx* c) {
     Ptrs& X = c->X;
     int rc = BE::orbit(c);
-    rc |= BE::fill_geometry(c, c->lev[0], c->levf[0], 0, 1u << CENT, 0u);
+    if (!c->P.otf) rc |= BE::fill_geometry(c, c->lev[0], c->levf[0], 0, 1u << CENT, 0u);
     X.geom = X.geom_n = X.geom_new = c->lev[0];
+    X.cent_lvl = 0;
     X.geomf = c->levf[0];
     X.conn = c->connlev[0];
     return rc;
@@ -422,13 +424,14 @@ struct Api {
       X.geom_n = c->lev[0];
       X.geom_new = c->lev[stage + 1];
       X.geom_s = c->lev[stage];
+      X.cent_lvl = stage;
       c->otf_stage = stage;
     }
     rc |= BE::reset_vmax(c);
     for (int d = 0; d < 3; d++) rc |= BE::flux(c, src, d);
     rc |= BE::emf(c);
     rc |= BE::update(c, stage);
-    if (c->P.dyn) X.geom = X.geom_new;
+    if (c->P.dyn) { X.geom = X.geom_new; X.cent_lvl = stage + 1; }
     rc |= BE::recover_queued(c, stage);
     rc |= bounds(c, dst, 0);
     rc |= BE::fixup_interp(c, dst, PFLAG_INTERP_V);
@@ -456,8 +459,17 @@ struct Api {
       return set_error(-2, "the metric is evaluated in the kernels only for the binary metric with PPM "
                            "reconstruction, no degenerate direction, on the GPU build");
     int rc = BE::invalidate_graph(c);
+    const bool was = c->P.otf != 0;
     c->P.otf = on ? 1 : 0;
+    // leaving the evaluated mode: the state's centre level must exist in memory
+    if (was && !on && c->P.dyn) rc |= BE::fill_geometry(c, const_cast<Geom*>(c->X.geom), nullptr, c->X.cent_lvl, 1u << CENT, 0u);
     return rc;
+  }
+  // Centre records of the state's level in memory (the evaluated mode keeps
+  // none): for hgpu_download_geometry.
+  static int materialize_state_geometry(hgpu_ctx* c) {
+    if (!c->P.dyn || !c->P.otf) return 0;
+    return BE::fill_geometry(c, const_cast<Geom*>(c->X.geom), nullptr, c->X.cent_lvl, 1u << CENT, 0u);
   }
 
   static int fill_levels(hgpu_ctx* c) {
@@ -465,14 +477,10 @@ struct Api {
     const unsigned m64 = c->P.mixed ? cent : (faces | cent), m32 = c->P.mixed ? faces : 0u;
     int rc = BE::orbit(c);   // the holes at t, t + dt/2, t + dt, read by every fill
     if (c->P.otf) {
-      // the flux and update kernels evaluate face and zone records from the
-      // orbit; the stored centre levels serve the inversion tree, fixups,
-      // boundaries and diagnostics, the derivative levels the source terms
-      rc |= BE::fill_geometry(c, c->lev[0], nullptr, 0, cent, 0u);
+      // every kernel evaluates the face and zone records it needs from the
+      // orbit; only the metric derivatives of the source terms are stored
       rc |= BE::fill_conn(c, c->connlev[0], 0);
-      rc |= BE::fill_geometry(c, c->lev[1], nullptr, 1, cent, 0u);
       rc |= BE::fill_conn(c, c->connlev[1], 1);
-      rc |= BE::fill_geometry(c, c->lev[2], nullptr, 2, cent, 0u);
       return rc;
     }
     rc |= BE::fill_geometry(c, c->lev[0], c->levf[0], 0, m64, m32);
@@ -488,7 +496,7 @@ struct Api {
     int rc = 0;
     if (c->P.dyn && stage == 0) rc |= fill_levels(c);
     rc |= substep(c, stage);
-    if (c->P.dyn && stage == 1) c->X.geom = c->X.geom_n = c->X.geom_new = c->lev[2];
+    if (c->P.dyn && stage == 1) { c->X.geom = c->X.geom_n = c->X.geom_new = c->lev[2]; c->X.cent_lvl = 2; }
     return rc;
   }
 
@@ -504,6 +512,7 @@ struct Api {
     if (c->P.dyn) {
       // the state is now at t + dt: level 2 (centres) describes it
       c->X.geom = c->X.geom_n = c->X.geom_new = c->lev[2];
+      c->X.cent_lvl = 2;
     }
     return rc;
   }

@@ -98,14 +98,6 @@ HD double seesaw_unit(double x) {
 }
 
 // ---------------------------------------------------------------------------
-struct PnBinary {
-  double m1, m2;          // masses (m1 + m2 = M)
-  double b0, phi0;        // separation and phase at t = 0
-  int inspiral;           // 1: radiation-reaction decay of b (when both masses are > 0);
-                          // 0: circular orbit of fixed b0
-  // numerical coordinates
-  double R0, th_beg, th_length, h_slope;
-};
 
 // Positions and velocities of the two holes at time t.
 template <class T>
@@ -349,9 +341,21 @@ HD double invert4(const double a[4][4], double b[4][4]) {
 // Geometry record (struct of_geom layout) at (t, X): metric, inverse, sqrt(-g),
 // lapse, shift and normal observer, as calc_all_geom() forms them
 // (src/metric.c:4221-4232).
+// The numerical metric of this binary has a diagonal spatial block D and a
+// shift row b, so its inverse is closed-form through the Schur complement
+// S = g_tt - sum b_i^2 / D_i (= -alpha^2): g^tt = 1/S, g^ti = -q_i/S,
+// g^ij = delta_ij / D_i + q_i q_j / S with q_i = b_i / D_i, det g = S D1 D2 D3.
 HD void pn_geom_from_gcov(const double g[4][4], Geom* G) {
   for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) G->gcov[a][b] = g[a][b];
-  const double det = invert4(g, G->gcon);
+  const double iD[3] = {1. / g[1][1], 1. / g[2][2], 1. / g[3][3]};
+  const double q[3] = {g[0][1] * iD[0], g[0][2] * iD[1], g[0][3] * iD[2]};
+  const double S = g[0][0] - (g[0][1] * q[0] + g[0][2] * q[1] + g[0][3] * q[2]);
+  const double iS = 1. / S;
+  G->gcon[0][0] = iS;
+  for (int a = 0; a < 3; a++) G->gcon[0][a + 1] = G->gcon[a + 1][0] = -q[a] * iS;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) G->gcon[a + 1][b + 1] = ((a == b) ? iD[a] : 0.) + q[a] * q[b] * iS;
+  const double det = S * g[1][1] * g[2][2] * g[3][3];
   G->g = ::sqrt(-det);
   G->g_inv = 1. / G->g;
   G->ncon[0] = ::sqrt(-G->gcon[0][0]);
@@ -378,15 +382,6 @@ HD void pn_geom_orb(const PnBinary& B, const PnOrbit& o, double X1, double X2, d
 
 // Binary and grid of a context whose metric is the binary's: the numerical
 // coordinates of zone (i,j,k) at position pos (coord(), src/coord.c:182-228).
-// tab: the coordinate functions at every zone's lower face (h = 0) and centre
-// (h = 1) along each direction, [2][N1T][2] (r, dr), then [2][N2T][3]
-// (sin theta, cos theta, dtheta/dX2), then [2][N3T][2] (sin phi, cos phi);
-// null: evaluate them.
-struct PnGrid {
-  PnBinary B;
-  double x0[4];
-  const double* tab;
-};
 HD long long pn_tab_size(const Params& P) { return 2LL * (2 * P.n1t + 3 * P.n2t + 2 * P.n3t); }
 HD void zone_X(const Params& P, const PnGrid& G, int i, int j, int k, int pos, double* X) {
   const double h1 = (pos == FACE1 || pos == CORN) ? 0. : 0.5;

@@ -112,6 +112,26 @@ struct Params {
 // Device pointers of one context.
 struct PnOrbit;
 
+// The binary of the time-dependent metric (hgpu_pn.h evaluates it).
+struct PnBinary {
+  double m1, m2;          // masses (m1 + m2 = M)
+  double b0, phi0;        // separation and phase at t = 0
+  int inspiral;           // 1: radiation-reaction decay of b (when both masses are > 0);
+                          // 0: circular orbit of fixed b0
+  // numerical coordinates
+  double R0, th_beg, th_length, h_slope;
+};
+// Binary and grid of a context whose metric is the binary's; x0: startx.
+// tab: the coordinate functions at every zone's lower face (h = 0) and centre
+// (h = 1) along each direction, [2][N1T][2] (r, dr), then [2][N2T][3]
+// (sin theta, cos theta, dtheta/dX2), then [2][N3T][2] (sin phi, cos phi);
+// null: evaluate them.
+struct PnGrid {
+  PnBinary B;
+  double x0[4];
+  const double* tab;
+};
+
 struct Ptrs {
   double* p;               // primitive state p  (SoA, 8*NT)
   double* ph;              // half-step state ph (SoA, 8*NT)
@@ -122,6 +142,8 @@ struct Ptrs {
   const Geom* geom_new;    // level of the stage's result (t + dt/2 or t + dt; = geom when static)
   const Geom* geom_s;      // stage's level for face records after geom moves to geom_new (null: geom)
   const PnOrbit* orbit;    // binary metric: the holes' orbit at the step's levels t, t + dt/2, t + dt
+  PnGrid pn;               // binary metric: its grid (with P.otf, zone-centre records are evaluated
+  int cent_lvl;            // at level cent_lvl of orbit instead of read from geom)
   const GeomF* geomf;      // their FP32 copy (mixed precision)
   float* Ff;               // FP32 face fluxes [3][NFLUX][NT] (mixed precision)
   float* emff;             // FP32 corner EMFs [3][NT] (mixed precision)

@@ -238,9 +238,12 @@ class Harm3D:
         return ng.value, nc.value
 
     def geometry(self):
+        """Geometry records [n_geom][42] and connection [n_conn][4][4][4]; for
+        the binary metric the records of the state's level and the metric
+        derivatives [28][n_conn] of the last stage's level."""
         ng, nc = self.geometry_counts()
         g = np.empty((ng, 42))
-        c = np.empty((nc, 4, 4, 4))
+        c = np.empty((28, nc)) if self.cfg.metric == 3 else np.empty((nc, 4, 4, 4))
         self._chk(self.lib.hgpu_download_geometry(self.h, _dp(g), _dp(c)), "hgpu_download_geometry")
         return g, c
 

@@ -78,7 +78,12 @@ int hgpu_download_prims(hgpu_ctx *ctx, int which, double *host);
 
 /* Replaces the computed geometry by the host's geom_arr / conn records
  * (struct of_geom layout, src/globals.h:90-98; conn[N_CONN][4][4][4]).
- * n_geom / n_conn must equal the counts hgpu_geometry_counts() reports. */
+ * n_geom / n_conn must equal the counts hgpu_geometry_counts() reports.
+ * With metric = 3 (no upload): the download gives the records of the state's
+ * time level (zone centres) and, for conn, the 28 metric derivatives of the
+ * last stage's level (t + dt/2 of the last step) stored [28][n_conn]
+ * (d g_ab / dx^nu for the non-zero components ab = tt, t1, t2, t3, 11, 22,
+ * 33, index nu*7 + component). */
 int hgpu_geometry_counts(const hgpu_ctx *ctx, long long *n_geom, long long *n_conn);
 int hgpu_upload_geometry(hgpu_ctx *ctx, const double *geom, const double *conn);
 int hgpu_download_geometry(hgpu_ctx *ctx, double *geom, double *conn);

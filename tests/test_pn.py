@@ -253,6 +253,13 @@ def test_gpu_bbh_metric_in_kernels_matches_stored(precision):
     assert np.array_equal(A.pflag(), B.pflag())
     assert np.array_equal(A.nfail(), B.nfail())
     assert A.time() == B.time()
+    assert A.divb_max() == B.divb_max()
+    # the evaluated mode keeps no centre level: the download forms the state's
+    ga, da = A.geometry()
+    gb, db = B.geometry()
+    cent = slice(3 * A.nt, 4 * A.nt)   # position 3 (zone centres) of [5][NT]
+    assert np.array_equal(ga[cent], gb[cent])
+    assert np.array_equal(da, db)
 
 
 @pytest.mark.gpu
