@@ -335,9 +335,13 @@ static void launch_flux_multi(const Params& P, const Ptrs& X, const LaunchShape&
     hipLaunchKernelGGL((k_flux_multi<D, F, 1, MIXED, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
     return;
   }
-  // the register-capped variant exists for the FP32 solve (the FP64 one spills)
-  if (MIXED && L.occ) hipLaunchKernelGGL((k_flux_multi<D, F, 4, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
-  else hipLaunchKernelGGL((k_flux_multi<D, F, 1, MIXED>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
+  // register-capped variants: 4 waves/SIMD for the FP32 solve, 3 for FP64
+  if (L.occ) {
+    if (MIXED) hipLaunchKernelGGL((k_flux_multi<D, F, 4, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
+    else hipLaunchKernelGGL((k_flux_multi<D, F, 3, false>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
+  } else {
+    hipLaunchKernelGGL((k_flux_multi<D, F, 1, MIXED>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
+  }
 }
 
 // G, lvl: the binary metric's grid and the stage's level, read when P.otf
@@ -705,11 +709,13 @@ struct GpuBE {
     c->be = g;
     // Untuned defaults: the autotuner's picks on the 192x192x128 and
     // 256x256x192 tori.  The x1 fluxes take one (j,k) tile through consecutive
-    // x1-planes, so their +-3-plane stencil stays in L2; the fluxes run 4
-    // (FP64) or 2 (FP32 solve) faces per thread; the update is the fused
-    // kernel (register-capped in mixed precision).
+    // x1-planes, so their +-3-plane stencil stays in L2; the x1 and x2 fluxes
+    // run 2 faces per thread, the x3 fluxes 4; the update is the fused kernel
+    // (register-capped in mixed precision).
     const bool mixed = c->P.mixed != 0;
-    for (int d = 0; d < 3; d++) g->shape[FAM_FLUX1 + d].fpt = mixed ? 2u : 4u;
+    g->shape[FAM_FLUX1].fpt = 2;
+    g->shape[FAM_FLUX2].fpt = 2;
+    g->shape[FAM_FLUX3].fpt = 4;
     g->shape[FAM_FLUX1].order = 1;
     g->shape[FAM_FLUX2].occ = mixed ? 1u : 0u;
     g->shape[FAM_FLUX3].remap = mixed ? 0u : 1u;
@@ -983,10 +989,16 @@ struct GpuBE {
       };
       const bool multi = fam <= FAM_FLUX3 && c->P.recon == RECON_PPM_MC && !c->P.degen[fam - FAM_FLUX1];
       const unsigned nfpt = multi ? 3u : 1u;
+      // face groups: every group size with and without the register cap
+      // (which size wins depends on the occupancy the cap buys)
+      const unsigned nocc = multi ? 2u : 1u;
       for (unsigned fc = 0; fc < nfpt; fc++)
+      for (unsigned oc = 0; oc < nocc; oc++)
       for (unsigned bs : bss) for (unsigned kc : kcs) for (unsigned flags = 0; flags < 4; flags++) {
         LaunchShape L;
         L.bs = bs; L.kc = kc; L.remap = flags & 1u; L.order = flags >> 1; L.fpt = 1u << fc;
+        L.occ = (fc > 0) ? oc : 0u;
+        if (fc == 0 && oc == 1) continue;
         if (check_shape(L)) { g_last_error.clear(); continue; }
         const float ms = time_shape(L);
         if (rc) break;
