@@ -263,6 +263,22 @@ def test_gpu_bbh_metric_in_kernels_matches_stored(precision):
 
 
 @pytest.mark.gpu
+def test_gpu_bbh_autotune_keeps_results():
+    """Launch shapes on the binary metric (evaluated records): the autotuner
+    leaves the state unchanged and the tuned steps equal the untuned ones."""
+    E, _ = _bbh_run(False, "mixed", steps=2)
+    T, _ = _bbh_run(False, "mixed", steps=2)
+    before = T.prims()
+    rep = T.autotune(reps=1)
+    print(f"[bbh] autotune picked {rep}", flush=True)
+    assert np.array_equal(before, T.prims())
+    E.step_ch(2)
+    T.step_ch(2)
+    assert np.array_equal(E.prims(), T.prims())
+    assert E.time() == T.time()
+
+
+@pytest.mark.gpu
 def test_gpu_bbh_graph_replay():
     E, _ = _bbh_run(False, "mixed", steps=4)
     G, _ = _bbh_run(False, "mixed", steps=4, graph=True)
