@@ -435,9 +435,7 @@ HD double entropy_flux(const Params& P, const Ptrs& X, const double* src, int d,
     GeomF gf;
     const GeomF* gp = nullptr;
     if (fo) {
-      Geom gb;
-      pn_zone_geom(P, fo->G, *fo->o, i, j, k, d, &gb);
-      geom_f32(gb, gf);
+      pn_zone_geomf(P, fo->G, *fo->o, i, j, k, d, &gf);
       gp = &gf;
     }
     if (d == 0) return entropy_face_mixed<0>(P, X, src, i, j, k, gp);

@@ -191,7 +191,7 @@ struct HostBE {
             Geom g;
             pn_zone_geom(P, c->pn, c->orbit[tsel], i, j, k, pos, &g);
             if (f64) dst[id] = g;
-            if (f32) geom_f32(g, dstf[id]);
+            if (f32) pn_zone_geomf(P, c->pn, c->orbit[tsel], i, j, k, pos, &dstf[id]);
           }
     }
     return 0;

@@ -276,13 +276,13 @@ __global__ __launch_bounds__(BLOCK, W) void k_flux_multi(Params P, Ptrs X, const
       const int g0 = fr.nf[NFAIL_GAMMA_CALC];
       const int fi = i + f*di, fj = j + f*dj, fk = k + f*dk;
       if constexpr (OTF) {
-        Geom gb;
-        pn_zone_geom(P, G, orb, fi, fj, fk, D, &gb);
         if constexpr (MIXED) {
           GeomF gfb;
-          geom_f32(gb, gfb);
+          pn_zone_geomf(P, G, orb, fi, fj, fk, D, &gfb);
           flux_riemann_mixed_g<D>(P, X, pl[f], pr[f], fi, fj, fk, fr, TRANSPOSE ? fo[f] : nullptr, &gfb);
         } else {
+          Geom gb;
+          pn_zone_geom(P, G, orb, fi, fj, fk, D, &gb);
           flux_riemann_g<D>(P, X, pl[f], pr[f], fi, fj, fk, fr, TRANSPOSE ? fo[f] : nullptr, &gb);
         }
       } else {
@@ -634,13 +634,11 @@ __global__ __launch_bounds__(FILL_B) void k_fill_geom(Params P, Ptrs X, PnGrid G
   const long long z0 = (long long)blockIdx.x * FILL_B;
   const long long z = z0 + threadIdx.x;
   const long long nrec = (nt - z0 < FILL_B) ? nt - z0 : FILL_B;
+  const int k = (int)(z % P.n3t);
+  const int j = (int)((z / P.n3t) % P.n2t);
+  const int i = (int)(z / ((long long)P.n3t * P.n2t));
   Geom g;
-  if (z < nt) {
-    const int k = (int)(z % P.n3t);
-    const int j = (int)((z / P.n3t) % P.n2t);
-    const int i = (int)(z / ((long long)P.n3t * P.n2t));
-    pn_zone_geom(P, G, X.orbit[tsel], i, j, k, pos, &g);
-  }
+  if (z < nt && dst) pn_zone_geom(P, G, X.orbit[tsel], i, j, k, pos, &g);
   // records of position pos are contiguous: geom_id = pos * NT + zone index
   if (dst) {
     if (z < nt) {
@@ -656,7 +654,7 @@ __global__ __launch_bounds__(FILL_B) void k_fill_geom(Params P, Ptrs X, PnGrid G
     float* sf = reinterpret_cast<float*>(sbuf);
     if (z < nt) {
       GeomF f;
-      geom_f32(g, f);
+      pn_zone_geomf(P, G, X.orbit[tsel], i, j, k, pos, &f);   // the FP32 face solve's own records
       const float* fp = reinterpret_cast<const float*>(&f);
       for (int l = 0; l < GEOMF_FLOATS; l++) sf[threadIdx.x * GEOMF_FLOATS + l] = fp[l];
     }

@@ -427,6 +427,79 @@ HD void pn_tab_entry(const Params& P, const PnGrid& G, double* tab, long long e)
 }
 HD long long pn_tab_entries(const Params& P) { return 2LL * (P.n1t + P.n2t + P.n3t); }
 
+// FP32 geometry record of zone (i,j,k) at position pos and the level of orbit
+// o, evaluated in FP32 from the tables (for the FP32 face solve).  1 - alpha
+// is formed without cancellation: 1 - alpha = (1 + S) / (1 + alpha) with
+// S = g_tt - sum b_i q_i = -alpha^2 and 1 + S = 2V - 2U^2 - sum b_i q_i.
+HD void pn_zone_geomf(const Params& P, const PnGrid& G, const PnOrbit& o, int i, int j, int k, int pos,
+                      GeomF* out) {
+  PnCoords<double> C;
+  pn_zone_coords(P, G, i, j, k, pos, C);
+  const float r = (float)C.r, dr = (float)C.dr, st = (float)C.st, ct = (float)C.ct, dth = (float)C.dth;
+  const float sp = (float)C.sp, cp = (float)C.cp;
+  const float rs = r * st;
+  const float x[3] = {rs * cp, rs * sp, r * ct};
+  const float m[2] = {(float)G.B.m1, (float)G.B.m2};
+  float y[2][3], v[2][3], ra[2], nv[2], vsq[2];
+  for (int a = 0; a < 2; a++)
+    for (int q = 0; q < 3; q++) { y[a][q] = (float)o.y[a][q]; v[a][q] = (float)o.v[a][q]; }
+  for (int a = 0; a < 2; a++) {
+    const float dx = x[0] - y[a][0], dy = x[1] - y[a][1], dz = x[2] - y[a][2];
+    ra[a] = sqrtf(dx * dx + dy * dy + dz * dz);
+    nv[a] = (dx * v[a][0] + dy * v[a][1] + dz * v[a][2]) / ra[a];
+    vsq[a] = v[a][0] * v[a][0] + v[a][1] * v[a][1] + v[a][2] * v[a][2];
+  }
+  const float ex = y[0][0] - y[1][0], ey = y[0][1] - y[1][1], ez = y[0][2] - y[1][2];
+  const float r12 = sqrtf(ex * ex + ey * ey + ez * ez);
+  const float r12_3 = r12 * r12 * r12;
+  float U = 0.f, V = 0.f, Vi[3] = {0.f, 0.f, 0.f};
+  for (int a = 0; a < 2; a++) {
+    const int b = 1 - a;
+    const float mr = m[a] / ra[a];
+    U = U + mr;
+    V = V + mr * (1.f + 2.f * vsq[a] - 0.5f * nv[a] * nv[a]) - m[a] * m[b] / (ra[a] * r12)
+          + (m[a] * m[b] / 4.f) * (ra[b] * ra[b] - ra[a] * ra[a] - r12 * r12) / (ra[a] * r12_3);
+  }
+  for (int q = 0; q < 3; q++)
+    for (int a = 0; a < 2; a++) Vi[q] = Vi[q] + m[a] * v[a][q] / ra[a];
+  const float J1[3] = {st * cp * dr, st * sp * dr, ct * dr};
+  const float rdth = r * dth;
+  const float J2[3] = {ct * cp * rdth, ct * sp * rdth, -st * rdth};
+  const float J3[2] = {-rs * sp, rs * cp};
+  const float A = 1.f + 2.f * U;
+  const float b[3] = {-4.f * (Vi[0] * J1[0] + Vi[1] * J1[1] + Vi[2] * J1[2]),
+                      -4.f * (Vi[0] * J2[0] + Vi[1] * J2[1] + Vi[2] * J2[2]),
+                      -4.f * (Vi[0] * J3[0] + Vi[1] * J3[1])};
+  const float D[3] = {A * dr * dr, A * rdth * rdth, A * rs * rs};
+  const float opg00 = 2.f * V - 2.f * U * U;          // 1 + g_tt
+  for (int a = 0; a < 4; a++)
+    for (int c = 0; c < 4; c++) out->gcov[a][c] = 0.f;
+  out->gcov[0][0] = opg00 - 1.f;
+  for (int a = 0; a < 3; a++) {
+    out->gcov[0][a + 1] = out->gcov[a + 1][0] = b[a];
+    out->gcov[a + 1][a + 1] = D[a];
+  }
+  const float iD[3] = {1.f / D[0], 1.f / D[1], 1.f / D[2]};
+  const float q[3] = {b[0] * iD[0], b[1] * iD[1], b[2] * iD[2]};
+  const float bq = b[0] * q[0] + b[1] * q[1] + b[2] * q[2];
+  const float S = out->gcov[0][0] - bq;
+  const float iS = 1.f / S;
+  out->gcon[0][0] = iS;
+  for (int a = 0; a < 3; a++) out->gcon[0][a + 1] = out->gcon[a + 1][0] = -q[a] * iS;
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++) out->gcon[a + 1][c + 1] = ((a == c) ? iD[a] : 0.f) + q[a] * q[c] * iS;
+  out->g = sqrtf(-(S * D[0] * D[1] * D[2]));
+  out->g_inv = 1.f / out->g;
+  const float alpha = sqrtf(-S);
+  out->alpha = alpha;
+  out->ncon[0] = 1.f / alpha;
+  for (int a = 0; a < 3; a++) {
+    out->ncon[a + 1] = -alpha * out->gcon[0][a + 1];
+    out->beta[a] = -out->gcon[0][a + 1] / out->gcon[0][0];
+  }
+  out->om_alpha = (opg00 - bq) / (1.f + alpha);
+}
+
 // Geometry record of zone (i,j,k) at position pos and the level of orbit o
 HD void pn_zone_geom(const Params& P, const PnGrid& G, const PnOrbit& o, int i, int j, int k, int pos, Geom* out) {
   if (!G.tab) {
