@@ -63,14 +63,14 @@ HD void flux_recon(const Params& P, const Ptrs& X, const double* __restrict__ sr
         for (int m = 0; m < 6; m++) {
           double x = src[v*NT + c0 + m*st];
           if (d == 0) {
-            if (P.rescale_r) x *= X.rscale[rs_slot(v)*P.n1t + (ii - 3 + m)];
+            x *= X.rscale[rs_slot(v)*P.n1t + (ii - 3 + m)];
           } else {
             x = rt_val(P, X, v, ii, x);
           }
           y[m] = x;
         }
         ppm_interface(y, &pl[v], &pr[v]);
-        if (d == 0 && P.rescale_r) {
+        if (d == 0) {
           double f = X.irscale[(3 + rs_slot(v))*P.n1t + i];
           pl[v] *= f;
           pr[v] *= f;
@@ -85,7 +85,7 @@ HD void flux_recon(const Params& P, const Ptrs& X, const double* __restrict__ sr
         for (int m = 0; m < 4; m++) {
           double x = src[v*NT + c0 + m*st];
           if (d == 0) {
-            if (P.rescale_r) x *= X.rscale[rs_slot(v)*P.n1t + (ii - 2 + m)];
+            x *= X.rscale[rs_slot(v)*P.n1t + (ii - 2 + m)];
           } else {
             x = rt_val(P, X, v, ii, x);
           }
@@ -95,7 +95,7 @@ HD void flux_recon(const Params& P, const Ptrs& X, const double* __restrict__ sr
         double dqr = 0.5*mc_slope(y[1], y[2], y[3]);
         pl[v] = y[1] + dql;
         pr[v] = y[2] - dqr;
-        if (d == 0 && P.rescale_r) {
+        if (d == 0) {
           double f = X.irscale[(3 + rs_slot(v))*P.n1t + i];
           pl[v] *= f;
           pr[v] *= f;
@@ -134,7 +134,7 @@ HD void ppm_faces(const Params& P, const Ptrs& X, const double* __restrict__ src
     for (int m = 0; m < M; m++) {
       double x = src[v*NT + c0 + zm[m]*st];
       if (d == 0) {
-        if (P.rescale_r) x *= X.rscale[rs_slot(v)*P.n1t + (ii - 3 + zm[m])];
+        x *= X.rscale[rs_slot(v)*P.n1t + (ii - 3 + zm[m])];
       } else {
         x = rt_val(P, X, v, ii, x);
       }
@@ -150,7 +150,7 @@ HD void ppm_faces(const Params& P, const Ptrs& X, const double* __restrict__ src
 #pragma unroll
     for (int f = 0; f < F; f++) {
       double a = re[2 + f], b = le[3 + f];
-      if (d == 0 && P.rescale_r) {
+      if (d == 0) {
         const double fac = X.irscale[(3 + rs_slot(v))*P.n1t + imin(i + f, P.n1t - 1)];
         a *= fac;
         b *= fac;
@@ -387,12 +387,16 @@ HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, i
           dU[UU + nu] += 0.5*sum;
         }
       } else {
-        // static metric: records [zone][64] of Gamma^b_{nu a}
+        // static metric: records [zone][64] of Gamma^b_{nu a}; each sum runs
+        // over (a, b) in the same order, the 3D test outside the loop so the
+        // record's loads issue together
         const double* cn = X.conn + cid * 64;
         for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) {
           dU[U1] += mhd[a][b]*cn[((b*4 + 1)*4 + a)];
           dU[U2] += mhd[a][b]*cn[((b*4 + 2)*4 + a)];
-          if (P.metric_dim > 2) {
+        }
+        if (P.metric_dim > 2) {
+          for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) {
             dU[UU] += mhd[a][b]*cn[((b*4 + 0)*4 + a)];
             dU[U3] += mhd[a][b]*cn[((b*4 + 3)*4 + a)];
           }

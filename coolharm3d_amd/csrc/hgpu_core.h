@@ -142,6 +142,7 @@ inline void build_tables(const hgpu_config& c, const Params& P, long long n_geom
   }
   // per-x1 tables
   T.omega.assign(P.n1t, 0.); T.tr.assign(P.n1t, 0.);
+  // ones without RESCALE_R: the kernels multiply by these tables unconditionally
   T.rscale.assign(6 * P.n1t, 1.); T.irscale.assign(6 * P.n1t, 1.);
   std::vector<double> rc(P.n1t), rf(P.n1t);
   for (int i = 0; i < P.n1t; i++) {

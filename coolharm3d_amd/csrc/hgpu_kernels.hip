@@ -332,7 +332,9 @@ static void launch_flux_multi(const Params& P, const Ptrs& X, const LaunchShape&
   const Tiling T = make_tiling(bg, L);
   const dim3 grid(T.nblocks), blk(T.bs);
   if (P.otf) {
-    hipLaunchKernelGGL((k_flux_multi<D, F, 1, MIXED, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
+    // three waves per SIMD fit the FP32 face solve without spilling
+    if (MIXED) hipLaunchKernelGGL((k_flux_multi<D, F, 3, true, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
+    else hipLaunchKernelGGL((k_flux_multi<D, F, 1, false, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
     return;
   }
   // register-capped variants: 4 waves/SIMD for the FP32 solve, 3 for FP64
@@ -705,22 +707,26 @@ struct GpuBE {
       return set_error(-5, "no HIP device available (the hgpu library runs only on a GPU)");
     GpuPriv* g = new GpuPriv();
     c->be = g;
-    // Untuned defaults: the autotuner's picks on the 192x192x128 and
-    // 256x256x192 tori.  The x1 fluxes take one (j,k) tile through consecutive
-    // x1-planes, so their +-3-plane stencil stays in L2; the x1 and x2 fluxes
-    // run 2 faces per thread, the x3 fluxes 4; the update is the fused kernel
-    // (register-capped in mixed precision).
+    // Untuned defaults: the autotuner's picks on the 192x192x128 torus.  The
+    // face fluxes run 2 faces per thread (4 along x3 in mixed precision) in
+    // the kernels capped at 3 (FP64) or 4 (FP32 solve) waves per SIMD; the
+    // update is the fused kernel capped at 3 waves per SIMD.
     const bool mixed = c->P.mixed != 0;
-    g->shape[FAM_FLUX1].fpt = 2;
-    g->shape[FAM_FLUX2].fpt = 2;
-    g->shape[FAM_FLUX3].fpt = 4;
-    g->shape[FAM_FLUX1].order = 1;
-    g->shape[FAM_FLUX2].occ = mixed ? 1u : 0u;
-    g->shape[FAM_FLUX3].remap = mixed ? 0u : 1u;
+    for (int f = FAM_FLUX1; f <= FAM_FLUX3; f++) {
+      g->shape[f].fpt = 2;
+      g->shape[f].occ = 1;
+      g->shape[f].remap = 0;
+    }
+    g->shape[FAM_FLUX2].remap = mixed ? 1u : 0u;
+    g->shape[FAM_FLUX2].order = mixed ? 0u : 1u;
+    if (mixed) {
+      g->shape[FAM_FLUX3].fpt = 4;
+      g->shape[FAM_FLUX3].occ = 0;
+    }
     g->shape[FAM_EMF].order = mixed ? 1u : 0u;
     g->shape[FAM_CONS].fused = 1;
     g->shape[FAM_CONS].remap = 0;
-    g->shape[FAM_CONS].occ = mixed ? 1u : 0u;
+    g->shape[FAM_CONS].occ = 1;
     return hip_check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
   }
   static void fini(hgpu_ctx* c) {
@@ -923,8 +929,8 @@ struct GpuBE {
     const dim3 grid(T.nblocks), blk(T.bs);
     const PnGrid& G = c->pn;
     const int lvl = c->otf_stage;
-    if (c->P.otf) {
-      if (c->P.mixed) hipLaunchKernelGGL((k_update<true, 1, true, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+    if (c->P.otf) {   // three waves per SIMD fit the FP32-face variant without spilling
+      if (c->P.mixed) hipLaunchKernelGGL((k_update<true, 3, true, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
       else hipLaunchKernelGGL((k_update<false, 1, true, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
       return;
     }
@@ -934,10 +940,10 @@ struct GpuBE {
       return;
     }
     if (c->P.mixed) {
-      if (L.occ) hipLaunchKernelGGL((k_update<true, 4>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      if (L.occ) hipLaunchKernelGGL((k_update<true, 3>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
       else hipLaunchKernelGGL((k_update<true, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
     } else {
-      if (L.occ) hipLaunchKernelGGL((k_update<false, 4>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      if (L.occ) hipLaunchKernelGGL((k_update<false, 3>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
       else hipLaunchKernelGGL((k_update<false, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
     }
   }

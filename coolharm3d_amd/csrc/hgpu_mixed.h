@@ -207,7 +207,7 @@ HD void flux_recon_mixed(const Params& P, const Ptrs& X, const double* __restric
     if (P.degen[1]) jj = NG;
     if (P.degen[2]) kk = NG;
     const long long st = cidx(P, di, dj, dk) - cidx(P, 0, 0, 0);
-    const bool rs = (d == 0) && P.rescale_r;
+    constexpr bool rs = (d == 0);   // ones in the tables without RESCALE_R
 #pragma unroll
     for (int v = 0; v < NP; v++) {
       double y[6];
@@ -266,7 +266,7 @@ HD void ppm_faces_mixed(const Params& P, const Ptrs& X, const double* __restrict
   const int nd = (d == 0) ? P.n1t : ((d == 1) ? P.n2t : P.n3t);
   const long long st = cidx(P, di, dj, dk) - cidx(P, 0, 0, 0);
   const long long c0 = cidx(P, ii - 3*di, jj - 3*dj, kk - 3*dk);
-  const bool rs = (d == 0) && P.rescale_r;
+  constexpr bool rs = (d == 0);   // ones in the tables without RESCALE_R
   int zm[M];
 #pragma unroll
   for (int m = 0; m < M; m++) zm[m] = imin(m, nd - 1 - (pos - 3));

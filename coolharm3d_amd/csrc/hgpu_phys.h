@@ -953,8 +953,10 @@ HD int rs_slot(int v) { return (v == RHO) ? 0 : ((v == UU) ? 2 : 1); }
 
 // Value of a primitive after the in-place x1 rescale/unscale round trip that
 // reconstruct_fast() applies to every zone (src/recon.c:996, :1008, :1173-1210).
+// Without RESCALE_R the tables hold exact ones, so the products return x
+// unchanged; applying them unconditionally keeps the kernels free of a
+// branch per loaded value, which would make each load wait for the last.
 HD double rt_val(const Params& P, const Ptrs& X, int v, int i, double x) {
-  if (!P.rescale_r) return x;
   int s = rs_slot(v);
   return (x * X.rscale[s*P.n1t + i]) * X.irscale[s*P.n1t + i];
 }
