@@ -262,16 +262,6 @@ HD void flux_face(const Params& P, const Ptrs& X, const double* __restrict__ src
 // --------------------------------------------------------------------------
 // Corner EMFs, flux_ct_para_fast2() (src/flux_ct.c:759-846) or flux_ct()
 // (src/flux_ct.c:26-41) for the edge (i,j,k).
-HD double ppm_line_pair(const double* __restrict__ base, long long c_center, long long stride) {
-  // returns (right edge of zone c-1) + (left edge of zone c) accumulated onto 0
-  double y[6];
-  for (int m = 0; m < 6; m++) y[m] = base[c_center + (m - 3)*stride];
-  double yl, yr;
-  ppm_interface(y, &yl, &yr);
-  double e = 0.0 + yl;
-  return e + yr;
-}
-
 HD void emf_edge(const Params& P, const Ptrs& X, int i, int j, int k) {
   const long long NT = P.nt;
   const long long c = cidx(P, i, j, k);
@@ -285,22 +275,23 @@ HD void emf_edge(const Params& P, const Ptrs& X, int i, int j, int k) {
   const double* F2B2 = F + (2*NFLUX + B2)*NT;
   double e0, e1, e2;
   if (P.ct_para) {
-    double y[6], yl, yr;
-    // emf0: + PPM of F1[B3] along x3, - PPM of F2[B2] along x2
-    e0 = ppm_line_pair(F1B3, c, sk);
-    for (int m = 0; m < 6; m++) y[m] = F2B2[c + (m - 3)*sj];
-    ppm_interface(y, &yl, &yr);
-    e0 = (e0 - yl) - yr;
-    // emf1: + PPM of F2[B1] along x1, - PPM of F0[B3] along x3
-    e1 = ppm_line_pair(F2B1, c, si);
-    for (int m = 0; m < 6; m++) y[m] = F0B3[c + (m - 3)*sk];
-    ppm_interface(y, &yl, &yr);
-    e1 = (e1 - yl) - yr;
-    // emf2: + PPM of F0[B2] along x2, - PPM of F1[B1] along x1
-    e2 = ppm_line_pair(F0B2, c, sj);
-    for (int m = 0; m < 6; m++) y[m] = F1B1[c + (m - 3)*si];
-    ppm_interface(y, &yl, &yr);
-    e2 = (e2 - yl) - yr;
+    // emf0: + PPM of F1[B3] along x3, - PPM of F2[B2] along x2;
+    // emf1: + PPM of F2[B1] along x1, - PPM of F0[B3] along x3;
+    // emf2: + PPM of F0[B2] along x2, - PPM of F1[B1] along x1.
+    // All six stencils are loaded before any limiter runs, so their loads
+    // are in flight together.
+    const double* line[6] = {F1B3, F2B2, F2B1, F0B3, F0B2, F1B1};
+    const long long str[6] = {sk, sj, si, sk, sj, si};
+    double y[6][6], yl[6], yr[6];
+#pragma unroll
+    for (int l = 0; l < 6; l++)
+#pragma unroll
+      for (int m = 0; m < 6; m++) y[l][m] = line[l][c + (m - 3)*str[l]];
+#pragma unroll
+    for (int l = 0; l < 6; l++) ppm_interface(y[l], &yl[l], &yr[l]);
+    e0 = (((0.0 + yl[0]) + yr[0]) - yl[1]) - yr[1];
+    e1 = (((0.0 + yl[2]) + yr[2]) - yl[3]) - yr[3];
+    e2 = (((0.0 + yl[4]) + yr[4]) - yl[5]) - yr[5];
   } else {
     e0 = 0.25*(F1B3[c] + F1B3[c - sk] - F2B2[c] - F2B2[c - sj]);
     e1 = 0.25*(F2B1[c] + F2B1[c - si] - F0B3[c] - F0B3[c - sk]);

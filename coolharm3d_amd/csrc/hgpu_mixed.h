@@ -392,10 +392,7 @@ HD void flux_face_mixed(const Params& P, const Ptrs& X, const double* __restrict
 
 // Corner EMFs from the FP32 face fluxes (flux_ct_para_fast2(), src/flux_ct.c:780-846;
 // flux_ct(), :31-41)
-HD float ppm_pair_f(const float* __restrict__ base, long long c, long long stride) {
-  float y[6];
-#pragma unroll
-  for (int m = 0; m < 6; m++) y[m] = base[c + (m - 3)*stride];
+HD float ppm_pair_f(const float* y) {
   float D[5];
 #pragma unroll
   for (int m = 0; m < 5; m++) D[m] = y[m + 1] - y[m];
@@ -417,9 +414,17 @@ HD void emf_edge_mixed(const Params& P, const Ptrs& X, int i, int j, int k) {
   const float* F2B2 = F + (2*NFLUX + B2)*NT;
   float e0, e1, e2;
   if (P.ct_para) {
-    e0 = ppm_pair_f(F1B3, c, sk) - ppm_pair_f(F2B2, c, sj);
-    e1 = ppm_pair_f(F2B1, c, si) - ppm_pair_f(F0B3, c, sk);
-    e2 = ppm_pair_f(F0B2, c, sj) - ppm_pair_f(F1B1, c, si);
+    // the six stencils loaded before any limiter runs (loads in flight together)
+    const float* line[6] = {F1B3, F2B2, F2B1, F0B3, F0B2, F1B1};
+    const long long str[6] = {sk, sj, si, sk, sj, si};
+    float y[6][6];
+#pragma unroll
+    for (int l = 0; l < 6; l++)
+#pragma unroll
+      for (int m = 0; m < 6; m++) y[l][m] = line[l][c + (m - 3)*str[l]];
+    e0 = ppm_pair_f(y[0]) - ppm_pair_f(y[1]);
+    e1 = ppm_pair_f(y[2]) - ppm_pair_f(y[3]);
+    e2 = ppm_pair_f(y[4]) - ppm_pair_f(y[5]);
   } else {
     e0 = 0.25f*(F1B3[c] + F1B3[c - sk] - F2B2[c] - F2B2[c - sj]);
     e1 = 0.25f*(F2B1[c] + F2B1[c - si] - F0B3[c] - F0B3[c - sk]);
