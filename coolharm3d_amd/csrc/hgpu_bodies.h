@@ -127,19 +127,34 @@ HD void ppm_faces(const Params& P, const Ptrs& X, const double* __restrict__ src
   int zm[M];
 #pragma unroll
   for (int m = 0; m < M; m++) zm[m] = imin(m, nd - 1 - (pos - 3));
+  // The next primitive's stencil is loaded before this one's limiters run
+  // (their branches keep the compiler from moving loads across them); this
+  // primitive's rescale factors are loaded before that, so waiting for them
+  // leaves the stencil loads in flight.  rt_val()'s round trip is spelled out
+  // with the factors loaded ahead.
+  double raw[2][M];
+#pragma unroll
+  for (int m = 0; m < M; m++) raw[0][m] = src[c0 + zm[m]*st];
 #pragma unroll
   for (int v = 0; v < NP; v++) {
+    const int s = rs_slot(v);
+    double fz[M], fo[F], ra = 1., ri = 1.;
+    if (d == 0) {
+#pragma unroll
+      for (int m = 0; m < M; m++) fz[m] = X.rscale[s*P.n1t + (ii - 3 + zm[m])];
+#pragma unroll
+      for (int f = 0; f < F; f++) fo[f] = X.irscale[(3 + s)*P.n1t + imin(i + f, P.n1t - 1)];
+    } else {
+      ra = X.rscale[s*P.n1t + ii];
+      ri = X.irscale[s*P.n1t + ii];
+    }
+    if (v + 1 < NP) {
+#pragma unroll
+      for (int m = 0; m < M; m++) raw[(v + 1) & 1][m] = src[(v + 1)*NT + c0 + zm[m]*st];
+    }
     double y[M];
 #pragma unroll
-    for (int m = 0; m < M; m++) {
-      double x = src[v*NT + c0 + zm[m]*st];
-      if (d == 0) {
-        x *= X.rscale[rs_slot(v)*P.n1t + (ii - 3 + zm[m])];
-      } else {
-        x = rt_val(P, X, v, ii, x);
-      }
-      y[m] = x;
-    }
+    for (int m = 0; m < M; m++) y[m] = (d == 0) ? raw[v & 1][m]*fz[m] : (raw[v & 1][m]*ra)*ri;
     double dq[M - 1], fi[M - 2], le[M - 2], re[M - 2];
 #pragma unroll
     for (int m = 1; m < M - 1; m++) dq[m] = mc_slope_vect(y[m - 1], y[m], y[m + 1]);
@@ -151,9 +166,8 @@ HD void ppm_faces(const Params& P, const Ptrs& X, const double* __restrict__ src
     for (int f = 0; f < F; f++) {
       double a = re[2 + f], b = le[3 + f];
       if (d == 0) {
-        const double fac = X.irscale[(3 + rs_slot(v))*P.n1t + imin(i + f, P.n1t - 1)];
-        a *= fac;
-        b *= fac;
+        a *= fo[f];
+        b *= fo[f];
       }
       pl[f][v] = a;
       pr[f][v] = b;

@@ -270,15 +270,28 @@ HD void ppm_faces_mixed(const Params& P, const Ptrs& X, const double* __restrict
   int zm[M];
 #pragma unroll
   for (int m = 0; m < M; m++) zm[m] = imin(m, nd - 1 - (pos - 3));
+  // the next primitive's stencil is loaded before this one's limiters run,
+  // this one's x1 rescale factors before that (waiting for them leaves the
+  // stencil loads in flight)
+  double raw[2][M];
+#pragma unroll
+  for (int m = 0; m < M; m++) raw[0][m] = src[c0 + zm[m]*st];
 #pragma unroll
   for (int v = 0; v < NP; v++) {
+    double fz[M], fo[F];
+    if (rs) {
+#pragma unroll
+      for (int m = 0; m < M; m++) fz[m] = X.rscale[rs_slot(v)*P.n1t + (ii - 3 + zm[m])];
+#pragma unroll
+      for (int f = 0; f < F; f++) fo[f] = X.irscale[(3 + rs_slot(v))*P.n1t + imin(i + f, P.n1t - 1)];
+    }
+    if (v + 1 < NP) {
+#pragma unroll
+      for (int m = 0; m < M; m++) raw[(v + 1) & 1][m] = src[(v + 1)*NT + c0 + zm[m]*st];
+    }
     double y[M];
 #pragma unroll
-    for (int m = 0; m < M; m++) {
-      double x = src[v*NT + c0 + zm[m]*st];
-      if (rs) x *= X.rscale[rs_slot(v)*P.n1t + (ii - 3 + zm[m])];
-      y[m] = x;
-    }
+    for (int m = 0; m < M; m++) y[m] = rs ? raw[v & 1][m]*fz[m] : raw[v & 1][m];
     float D[M - 1], dq[M - 1], ln[M - 2], rn[M - 2];
 #pragma unroll
     for (int m = 0; m < M - 1; m++) D[m] = (float)(y[m + 1] - y[m]);
@@ -291,9 +304,8 @@ HD void ppm_faces_mixed(const Params& P, const Ptrs& X, const double* __restrict
     for (int f = 0; f < F; f++) {
       double l = y[2 + f] + (double)rn[2 + f], r = y[3 + f] + (double)ln[3 + f];
       if (rs) {
-        const double fac = X.irscale[(3 + rs_slot(v))*P.n1t + imin(i + f, P.n1t - 1)];
-        l *= fac;
-        r *= fac;
+        l *= fo[f];
+        r *= fo[f];
       }
       pl[f][v] = (float)l;
       pr[f][v] = (float)r;
