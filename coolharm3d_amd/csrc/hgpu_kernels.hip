@@ -708,21 +708,18 @@ struct GpuBE {
     GpuPriv* g = new GpuPriv();
     c->be = g;
     // Untuned defaults: the autotuner's picks on the 192x192x128 torus.  The
-    // face fluxes run 2 faces per thread (4 along x3 in mixed precision) in
-    // the kernels capped at 3 (FP64) or 4 (FP32 solve) waves per SIMD; the
+    // face fluxes run 2 faces per thread, FP64 in the kernels capped at 3
+    // waves per SIMD, FP32 uncapped (3 waves) but along x2 (capped at 4); the
     // update is the fused kernel capped at 3 waves per SIMD.
     const bool mixed = c->P.mixed != 0;
     for (int f = FAM_FLUX1; f <= FAM_FLUX3; f++) {
       g->shape[f].fpt = 2;
-      g->shape[f].occ = 1;
+      g->shape[f].occ = mixed ? 0u : 1u;
       g->shape[f].remap = 0;
     }
+    g->shape[FAM_FLUX2].occ = 1;
     g->shape[FAM_FLUX2].remap = mixed ? 1u : 0u;
     g->shape[FAM_FLUX2].order = mixed ? 0u : 1u;
-    if (mixed) {
-      g->shape[FAM_FLUX3].fpt = 4;
-      g->shape[FAM_FLUX3].occ = 0;
-    }
     g->shape[FAM_EMF].order = mixed ? 1u : 0u;
     g->shape[FAM_CONS].fused = 1;
     g->shape[FAM_CONS].remap = 0;

@@ -20,6 +20,17 @@
 
 namespace hgpu {
 
+// The device build of this file's arithmetic fuses a multiply and an add
+// written in one expression into one FMA (one rounding instead of two, and
+// fewer instructions; within an expression only, so every inlined copy of a
+// body fuses alike); the FP64 code elsewhere keeps -ffp-contract=off so it
+// stays bit-identical to the reference build.  The compensated FP64 sum at
+// the end of the file is excluded (Neumaier's correction needs its
+// separately rounded steps).
+#ifdef __HIP_DEVICE_COMPILE__
+#pragma clang fp contract(on)
+#endif
+
 // ---------------------------------------------------------------------------
 // FP32 reciprocal, square root and power with the hardware's single-instruction
 // forms on the device (v_rcp_f32, v_sqrt_f32: about 1 ulp; v_log_f32 and
@@ -480,6 +491,10 @@ HD double face_flux_mixed(const Params& P, const Ptrs& X, int d, int l, int i, i
   }
   return f;
 }
+
+#ifdef __HIP_DEVICE_COMPILE__
+#pragma clang fp contract(off)
+#endif
 
 // Neumaier-compensated accumulation
 struct CompSum {
