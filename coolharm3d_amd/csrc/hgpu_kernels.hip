@@ -717,9 +717,10 @@ struct GpuBE {
       g->shape[f].occ = mixed ? 0u : 1u;
       g->shape[f].remap = 0;
     }
+    // x2 stencils share rows with the neighbouring tiles: XCD-contiguous
+    // tiles keep those in one L2 (half the HBM reads of round-robin tiles)
     g->shape[FAM_FLUX2].occ = 1;
-    g->shape[FAM_FLUX2].remap = mixed ? 1u : 0u;
-    g->shape[FAM_FLUX2].order = mixed ? 0u : 1u;
+    g->shape[FAM_FLUX2].remap = 1;
     g->shape[FAM_EMF].order = mixed ? 1u : 0u;
     g->shape[FAM_CONS].fused = 1;
     g->shape[FAM_CONS].remap = 0;
