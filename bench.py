@@ -330,12 +330,14 @@ def timed_run(args, g, p0, precision):
 
 def precision_report(shape, steps=(1, 10, 100)):
     """Mixed-precision variant vs the FP64 step on the same torus: per-primitive
-    relative L1 (interior zones) after each of `steps`."""
+    relative L1 (interior zones) after each of `steps`, and each precision's
+    ms/step over the last 80 of the steps (replayed from a graph, default
+    launch shapes; the 10 steps before them capture the graph)."""
     import numpy as np
     from coolharm3d_amd import Harm3D, torus_grid, torus
     g = torus_grid(*shape)
     p0 = torus(g)
-    states = {}
+    states, ms = {}, {}
     for prec in ("fp64", "mixed"):
         H = Harm3D(g, track_nfail=0, precision=prec)
         H.set_prims(p0)
@@ -343,7 +345,16 @@ def precision_report(shape, steps=(1, 10, 100)):
         H.set_time(0.0, 1.0e-5)
         done, out = 0, {}
         for s in steps:
-            H.step_ch(s - done)
+            if s == steps[-1] and s - done > 80:
+                H.set_graph(True)
+                H.step_ch(s - done - 80)
+                H.sync()
+                t0 = time.perf_counter()
+                H.step_ch(80)
+                H.sync()
+                ms[prec] = (time.perf_counter() - t0) / 80 * 1e3
+            else:
+                H.step_ch(s - done)
             done = s
             out[s] = H.prims()[3:-3, 3:-3, 3:-3].reshape(-1, 8)
         states[prec] = out
@@ -355,7 +366,12 @@ def precision_report(shape, steps=(1, 10, 100)):
         num = np.abs(b - a).sum(axis=0)
         den = np.abs(a).sum(axis=0)
         rep[str(s)] = {names[v]: float(num[v] / den[v]) if den[v] > 0 else float(num[v]) for v in range(8)}
-    return {"grid": list(shape), "steps": list(steps), "rel_l1_vs_fp64": rep,
+    timing = None
+    if len(ms) == 2:
+        ncell = shape[0] * shape[1] * shape[2]
+        timing = {"ms_per_step": ms, "zone_cycles_per_s": {k: ncell / (v * 1e-3) for k, v in ms.items()},
+                  "speedup_mixed_vs_fp64": ms["fp64"] / ms["mixed"]}
+    return {"grid": list(shape), "steps": list(steps), "rel_l1_vs_fp64": rep, "timing": timing,
             "reconstruction_flux_emf": "fp32", "update_con2prim_metric_cooling": "fp64"}
 
 
