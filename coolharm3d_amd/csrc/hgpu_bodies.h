@@ -359,10 +359,19 @@ HD double face_flux(const Params& P, const Ptrs& X, int d, int l, int i, int j, 
 // MET: -1 decides between the static and the binary metric at run time;
 // 0 / 1 compile only the static / binary branch (kernels built per metric, so
 // the branch not taken costs no registers).
+// MET: -1 from P; 0 static metric; 1 binary metric, derivatives from the
+// stored level X.conn; 2 binary metric, the terms themselves precomputed in
+// X.conn ([4][nconn], UU..U3, by k_source_otf, which counts their failures);
+// 3 binary metric, derivatives passed in dgl[28].
 template <int MET = -1>
 HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, int i, int j, int k,
-              double* dU) {
+              double* dU, const double* dgl = nullptr) {
   for (int l = 0; l < NP; l++) dU[l] = 0.;
+  if constexpr (MET == 2) {
+    const long long cid = conn_id(P, i, j, k);
+    for (int m = 0; m < 4; m++) dU[UU + m] = X.conn[m*P.nconn + cid];
+    return 0;
+  }
   int gfail = 0;
   State q;
   bool have_q = false;
@@ -382,8 +391,9 @@ HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, i
           return mhd[a][0]*g->gcon[0][b] + mhd[a][1]*g->gcon[1][b] + mhd[a][2]*g->gcon[2][b]
                + mhd[a][3]*g->gcon[3][b];
         };
-        const double* dg = X.conn + cid;
-        const long long cs = P.nconn;
+        const double* dg;
+        long long cs;
+        if constexpr (MET == 3) { dg = dgl; cs = 1; } else { dg = X.conn + cid; cs = P.nconn; }
         const double w[PN_NG] = {Tu(0, 0), Tu(0, 1) + Tu(1, 0), Tu(0, 2) + Tu(2, 0), Tu(0, 3) + Tu(3, 0),
                                  Tu(1, 1), Tu(2, 2), Tu(3, 3)};
         for (int nu = 0; nu < 4; nu++) {
@@ -603,7 +613,9 @@ HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int
     for (int v = 0; v < NP; v++) U[v] = X.Un[v*NT + c];
   }
   double dU[NP];
-  {
+  if constexpr (MET == 2) {
+    source<2>(P, X, nullptr, g, i, j, k, dU);
+  } else {
     double phz[NP];
     const double* src = (stage == 0) ? X.p : X.ph;
 #pragma unroll

@@ -472,18 +472,22 @@ struct Api {
     if (!c->P.dyn || !c->P.otf) return 0;
     return BE::fill_geometry(c, const_cast<Geom*>(c->X.geom), nullptr, c->X.cent_lvl, 1u << CENT, 0u);
   }
+  // Metric derivatives of the last stage's level (the evaluated mode keeps the
+  // source terms there instead): for hgpu_download_geometry.
+  static int materialize_stage_derivatives(hgpu_ctx* c) {
+    if (!c->P.dyn || !c->P.otf) return 0;
+    const int lvl = (c->X.conn == c->connlev[1]) ? 1 : 0;
+    return BE::fill_conn(c, const_cast<double*>(c->X.conn), lvl);
+  }
 
   static int fill_levels(hgpu_ctx* c) {
     const unsigned faces = (1u << FACE1) | (1u << FACE2) | (1u << FACE3), cent = 1u << CENT;
     const unsigned m64 = c->P.mixed ? cent : (faces | cent), m32 = c->P.mixed ? faces : 0u;
     int rc = BE::orbit(c);   // the holes at t, t + dt/2, t + dt, read by every fill
-    if (c->P.otf) {
-      // every kernel evaluates the face and zone records it needs from the
-      // orbit; only the metric derivatives of the source terms are stored
-      rc |= BE::fill_conn(c, c->connlev[0], 0);
-      rc |= BE::fill_conn(c, c->connlev[1], 1);
-      return rc;
-    }
+    // every kernel evaluates the face and zone records it needs from the
+    // orbit; each stage's update forms its source terms first (the stage's
+    // connection buffer holds the 4 terms instead of 28 derivatives)
+    if (c->P.otf) return rc;
     rc |= BE::fill_geometry(c, c->lev[0], c->levf[0], 0, m64, m32);
     rc |= BE::fill_conn(c, c->connlev[0], 0);
     rc |= BE::fill_geometry(c, c->lev[1], c->levf[1], 1, m64, m32);

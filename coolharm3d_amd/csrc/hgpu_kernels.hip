@@ -218,8 +218,10 @@ __global__ __launch_bounds__(BLOCK, W) void k_flux_mixed(Params P, Ptrs X, const
   commit_fail(P, X, fr, active, c, MASK_GAMMA, D + 1);
 }
 
-// Orbit record of a level: the same address in every lane, read through the
-// scalar cache (constant address space) so the 12 values sit in SGPRs.
+// Positions and velocities of a level's orbit record (the 12 values the
+// metric records read; not the time derivatives, which only the metric
+// derivatives need): the same address in every lane, read through the
+// scalar cache (constant address space) so they sit in SGPRs.
 __device__ __forceinline__ PnOrbit orbit_load(const PnOrbit* p) {
   const __attribute__((address_space(4))) double* q = (const __attribute__((address_space(4))) double*)p;
   PnOrbit o;
@@ -459,14 +461,15 @@ __global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage
     double U[NP];
     bool done;
     if constexpr (OTF) {
-      // the stage's record for U^{n+1} (the metric derivatives of the source
-      // term come from the stage's stored level: evaluating them here, by
-      // forward-mode differentiation, needs ~200 VGPRs on top of the
-      // update's), then the result's record for the inversion
+      // the stage's record for U^n in the predictor (the corrector reads U^n,
+      // and the source terms come precomputed by k_source_otf: evaluating the
+      // metric derivatives here, by forward-mode differentiation, needs ~200
+      // VGPRs on top of the update's), then the result's record for the
+      // inversion
       {
         Geom gs;
-        pn_zone_geom(P, G, orbit_load(X.orbit + lvl), i, j, k, CENT, &gs);
-        zone_conserved<MIXED, 1>(P, X, stage, (stage == 0) ? 0.5*dt : dt, i, j, k, &gs, &gs, fr, U);
+        if (stage == 0) pn_zone_geom(P, G, orbit_load(X.orbit + lvl), i, j, k, CENT, &gs);
+        zone_conserved<MIXED, 2>(P, X, stage, (stage == 0) ? 0.5*dt : dt, i, j, k, &gs, &gs, fr, U);
       }
       Geom gnew;
       pn_zone_geom(P, G, orbit_load(X.orbit + lvl + 1), i, j, k, CENT, &gnew);
@@ -679,6 +682,42 @@ __global__ __launch_bounds__(BLOCK) void k_fill_conn(Params P, Ptrs X, PnGrid G,
   pn_metric_dg(G.B, X.orbit[tsel], Xc[1], Xc[2], Xc[3], dg);
   const long long cid = conn_id(P, i, j, k);
   for (int l = 0; l < PN_NDG; l++) dst[l * P.nconn + cid] = dg[l];   // structure-of-arrays levels
+}
+
+// Source terms of a stage on the binary metric evaluated in the kernels: the
+// metric derivatives at the zone and the stage's level (as k_fill_conn) and
+// the stage's zone record (as the update evaluates it), contracted with the
+// stage's state at once by source() (src/phys.c:121-187); the update reads
+// the 4 terms (UU..U3, [4][nconn] in dst) instead of the 28 derivatives.
+// Failures of get_state() are counted here (the update counts none for the
+// source).
+__global__ __launch_bounds__(BLOCK, 2) void k_source_otf(Params P, Ptrs X, PnGrid G, int stage, double* dst) {
+  const long long z = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = z < (long long)P.n1 * P.n2 * P.n3;
+  FailRec fr; fr.clear();
+  long long c = 0;
+  if (active) {
+    const int k = NG + (int)(z % P.n3);
+    const int j = NG + (int)((z / P.n3) % P.n2);
+    const int i = NG + (int)(z / ((long long)P.n3 * P.n2));
+    const PnOrbit& o = X.orbit[stage];   // all of it: the derivatives read the time derivatives too
+    double Xc[4];
+    zone_X(P, G, i, j, k, CENT, Xc);
+    double dg[PN_NDG];
+    pn_metric_dg(G.B, o, Xc[1], Xc[2], Xc[3], dg);
+    Geom gs;
+    pn_zone_geom(P, G, o, i, j, k, CENT, &gs);
+    c = cidx(P, i, j, k);
+    const double* src = (stage == 0) ? X.p : X.ph;
+    double phz[NP], dU[NP];
+#pragma unroll
+    for (int v = 0; v < NP; v++) phz[v] = rt_val(P, X, v, i, src[v*P.nt + c]);
+    fr.nf[NFAIL_GAMMA_CALC] += source<3>(P, X, phz, &gs, i, j, k, dU, dg);
+    const long long cid = conn_id(P, i, j, k);
+#pragma unroll
+    for (int m = 0; m < 4; m++) dst[m*P.nconn + cid] = dU[UU + m];
+  }
+  commit_fail(P, X, fr, active, c, MASK_GAMMA);
 }
 
 // Batched Utoprim_2d_fast() on independent zones (SoA buffers).
@@ -1086,6 +1125,14 @@ struct GpuBE {
     return hip_check(hipGetLastError(), "k_emf");
   }
   static int update(hgpu_ctx* c, int stage) {
+    if (c->P.otf) {   // the stage's source terms, read by the update
+      Scope sm(c, "metric");
+      const long long n = (long long)c->P.n1 * c->P.n2 * c->P.n3;
+      hipLaunchKernelGGL(k_source_otf, dim3(nblocks(n)), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, c->pn, stage,
+                         c->connlev[stage]);
+      int rc = hip_check(hipGetLastError(), "k_source_otf");
+      if (rc) return rc;
+    }
     Box b = interior_box(c->P);
     Scope s(c, "update");
     const Tiling T = make_tiling(b, priv(c)->shape[FAM_CONS]);
