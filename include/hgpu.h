@@ -124,14 +124,14 @@ int hgpu_autotune(hgpu_ctx *ctx, int reps, double *report);
 int hgpu_set_launch_shape(hgpu_ctx *ctx, int family, int bs, int kc, int flags);
 int hgpu_get_launch_shape(hgpu_ctx *ctx, int family, int *bs, int *kc, int *flags);
 
-/* Binary metric (metric = 3): in_kernels = 1 evaluates the face records and
- * the conserved update's zone records inside the flux and update kernels from
- * the holes' orbit at each level (computed once per step); 0 fills and reads
- * stored levels for them too.  Both give the same results.  Default 1 where
- * possible: the GPU build with PPM reconstruction and no degenerate direction
- * (an error is returned otherwise).  The centre levels used by the inversion
- * tree, fixups and boundaries and the connection levels stay stored.  No
- * counterpart in the reference (whose binary metric bodies are absent). */
+/* Binary metric (metric = 3): in_kernels = 1 evaluates every geometry record
+ * inside the kernels that read it, from the holes' orbit at each level
+ * (computed once per step), and forms each stage's source terms in a kernel
+ * of their own from the metric derivatives; 0 fills and reads stored levels
+ * (records and metric derivatives).  Both give the same results bit for bit.
+ * Default 1 where possible: the GPU build with PPM reconstruction and no
+ * degenerate direction (an error is returned otherwise).  No counterpart in
+ * the reference (whose binary metric bodies are absent). */
 int hgpu_set_metric_eval(hgpu_ctx *ctx, int in_kernels);
 int hgpu_get_metric_eval(hgpu_ctx *ctx, int *in_kernels);
 
