@@ -359,6 +359,67 @@ HD double face_flux(const Params& P, const Ptrs& X, int d, int l, int i, int j, 
 // MET: -1 decides between the static and the binary metric at run time;
 // 0 / 1 compile only the static / binary branch (kernels built per metric, so
 // the branch not taken costs no registers).
+// Binary-metric source terms dU_nu = 1/2 T^{mu lambda} d_nu g_{mu lambda}
+// (equal to T^mu_kappa Gamma^kappa_{nu mu} of source(), src/phys.c:143-153,
+// for the metric connection) plus the cooling term, in two parts so a caller
+// can form the state's part before evaluating the metric derivatives: the
+// weights of the 7 metric components' derivatives (T^tt, T^t1 + T^1t,
+// T^t2 + T^2t, T^t3 + T^3t, T^11, T^22, T^33 with T^{mu lambda} =
+// T^mu_kappa g^{kappa lambda}) and the cooling rate (src/phys.c:541-582),
+// then the contraction, the cooling term and sqrt(-g).
+struct SrcDyn {
+  double w[PN_NG];
+  double f_cool, ucov[4];
+  int gfail;
+};
+HD void source_dyn_state(const Params& P, const Ptrs& X, const double* ph, const Geom* g, int i, SrcDyn& s) {
+  State q;
+  s.gfail = get_state(ph, g, P.gam, &q);
+  double mhd[4][4];
+  mhd_calc(&q, 0, mhd[0]); mhd_calc(&q, 1, mhd[1]);
+  mhd_calc(&q, 2, mhd[2]); mhd_calc(&q, 3, mhd[3]);
+  auto Tu = [&](int a, int b) {
+    return mhd[a][0]*g->gcon[0][b] + mhd[a][1]*g->gcon[1][b] + mhd[a][2]*g->gcon[2][b]
+         + mhd[a][3]*g->gcon[3][b];
+  };
+  s.w[0] = Tu(0, 0); s.w[1] = Tu(0, 1) + Tu(1, 0); s.w[2] = Tu(0, 2) + Tu(2, 0); s.w[3] = Tu(0, 3) + Tu(3, 0);
+  s.w[4] = Tu(1, 1); s.w[5] = Tu(2, 2); s.w[6] = Tu(3, 3);
+  for (int a = 0; a < 4; a++) s.ucov[a] = q.ucov[a];
+  s.f_cool = 0.;
+  if (P.cooling) {
+    int bound = ((q.ucov[0]*(q.p + ph[UU] + ph[RHO] + q.bsq)) > (-ph[RHO])) ? 1 : 0;
+    if (bound) {
+      double T_r = X.tr[i];
+      double Omega_r = X.omega[i];
+      double E_ratio = (P.gam - 1.)*ph[UU] / (ph[RHO] * T_r);
+      double dE = E_ratio - 1.;
+      double base = dE + fabs(dE);
+      double pw = (P.q_exp == 0.5) ? sqrt(base) : pow(base, P.q_exp);
+      s.f_cool = P.s_norm * ph[UU] * Omega_r * pw;
+    }
+  }
+}
+// dg: the derivatives d_nu g_m at dg[(nu*7 + m)*cs]; gdet: sqrt(-g).
+HD void source_dyn_finish(const Params& P, const SrcDyn& s, const double* dg, long long cs, double gdet,
+                          double* dU) {
+  for (int l = 0; l < NP; l++) dU[l] = 0.;
+  for (int nu = 0; nu < 4; nu++) {
+    double sum = 0.;
+    for (int m = 0; m < PN_NG; m++) sum += s.w[m]*dg[(nu*PN_NG + m)*cs];
+    dU[UU + nu] += 0.5*sum;
+  }
+  if (P.cooling) {
+    dU[UU] -= s.f_cool * s.ucov[0];
+    dU[U1] -= s.f_cool * s.ucov[1];
+    dU[U2] -= s.f_cool * s.ucov[2];
+    dU[U3] -= s.f_cool * s.ucov[3];
+  }
+  dU[UU] *= gdet;
+  dU[U1] *= gdet;
+  dU[U2] *= gdet;
+  dU[U3] *= gdet;
+}
+
 // MET: -1 from P; 0 static metric; 1 binary metric, derivatives from the
 // stored level X.conn; 2 binary metric, the terms themselves precomputed in
 // X.conn ([4][nconn], UU..U3, by k_source_otf, which counts their failures);
@@ -372,6 +433,17 @@ HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, i
     for (int m = 0; m < 4; m++) dU[UU + m] = X.conn[m*P.nconn + cid];
     return 0;
   }
+  const bool dyn = (MET < 0) ? (P.dyn != 0) : (MET >= 1);
+  if (dyn) {   // the binary metric (3D, not flat)
+    SrcDyn s;
+    source_dyn_state(P, X, ph, g, i, s);
+    if constexpr (MET == 3) {
+      source_dyn_finish(P, s, dgl, 1, g->g, dU);
+    } else {
+      source_dyn_finish(P, s, X.conn + conn_id(P, i, j, k), P.nconn, g->g, dU);
+    }
+    return s.gfail;
+  }
   int gfail = 0;
   State q;
   bool have_q = false;
@@ -383,25 +455,7 @@ HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, i
       mhd_calc(&q, 0, mhd[0]); mhd_calc(&q, 1, mhd[1]);
       mhd_calc(&q, 2, mhd[2]); mhd_calc(&q, 3, mhd[3]);
       const long long cid = conn_id(P, i, j, k);
-      const bool dyn = (MET < 0) ? (P.dyn != 0) : (MET >= 1);
-      if (dyn) {
-        // T^{mu lambda} = T^mu_kappa g^{kappa lambda}, formed per component
-        // for the pairs the derivative records hold
-        auto Tu = [&](int a, int b) {
-          return mhd[a][0]*g->gcon[0][b] + mhd[a][1]*g->gcon[1][b] + mhd[a][2]*g->gcon[2][b]
-               + mhd[a][3]*g->gcon[3][b];
-        };
-        const double* dg;
-        long long cs;
-        if constexpr (MET == 3) { dg = dgl; cs = 1; } else { dg = X.conn + cid; cs = P.nconn; }
-        const double w[PN_NG] = {Tu(0, 0), Tu(0, 1) + Tu(1, 0), Tu(0, 2) + Tu(2, 0), Tu(0, 3) + Tu(3, 0),
-                                 Tu(1, 1), Tu(2, 2), Tu(3, 3)};
-        for (int nu = 0; nu < 4; nu++) {
-          double sum = 0.;
-          for (int m = 0; m < PN_NG; m++) sum += w[m]*dg[(nu*PN_NG + m)*cs];
-          dU[UU + nu] += 0.5*sum;
-        }
-      } else {
+      {
         // static metric: records [zone][64] of Gamma^b_{nu a}; each sum runs
         // over (a, b) in the same order, the 3D test outside the loop so the
         // record's loads issue together

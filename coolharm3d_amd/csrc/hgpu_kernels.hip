@@ -687,7 +687,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill_conn(Params P, Ptrs X, PnGrid G,
 // Source terms of a stage on the binary metric evaluated in the kernels: the
 // metric derivatives at the zone and the stage's level (as k_fill_conn) and
 // the stage's zone record (as the update evaluates it), contracted with the
-// stage's state at once by source() (src/phys.c:121-187); the update reads
+// stage's state at once as source() forms them (src/phys.c:121-187); the update reads
 // the 4 terms (UU..U3, [4][nconn] in dst) instead of the 28 derivatives.
 // Failures of get_state() are counted here (the update counts none for the
 // source).
@@ -701,18 +701,27 @@ __global__ __launch_bounds__(BLOCK, 2) void k_source_otf(Params P, Ptrs X, PnGri
     const int j = NG + (int)((z / P.n3) % P.n2);
     const int i = NG + (int)(z / ((long long)P.n3 * P.n2));
     const PnOrbit& o = X.orbit[stage];   // all of it: the derivatives read the time derivatives too
+    c = cidx(P, i, j, k);
+    // the state's part first (the zone record is then dead but for sqrt(-g)),
+    // the metric derivatives last, so their evaluation holds few live values
+    SrcDyn sd;
+    double gdet;
+    {
+      Geom gs;
+      pn_zone_geom(P, G, o, i, j, k, CENT, &gs);
+      const double* src = (stage == 0) ? X.p : X.ph;
+      double phz[NP];
+#pragma unroll
+      for (int v = 0; v < NP; v++) phz[v] = rt_val(P, X, v, i, src[v*P.nt + c]);
+      source_dyn_state(P, X, phz, &gs, i, sd);
+      gdet = gs.g;
+    }
     double Xc[4];
     zone_X(P, G, i, j, k, CENT, Xc);
-    double dg[PN_NDG];
+    double dg[PN_NDG], dU[NP];
     pn_metric_dg(G.B, o, Xc[1], Xc[2], Xc[3], dg);
-    Geom gs;
-    pn_zone_geom(P, G, o, i, j, k, CENT, &gs);
-    c = cidx(P, i, j, k);
-    const double* src = (stage == 0) ? X.p : X.ph;
-    double phz[NP], dU[NP];
-#pragma unroll
-    for (int v = 0; v < NP; v++) phz[v] = rt_val(P, X, v, i, src[v*P.nt + c]);
-    fr.nf[NFAIL_GAMMA_CALC] += source<3>(P, X, phz, &gs, i, j, k, dU, dg);
+    source_dyn_finish(P, sd, dg, 1, gdet, dU);
+    fr.nf[NFAIL_GAMMA_CALC] += sd.gfail;
     const long long cid = conn_id(P, i, j, k);
 #pragma unroll
     for (int m = 0; m < 4; m++) dst[m*P.nconn + cid] = dU[UU + m];
