@@ -291,6 +291,7 @@ def bbh_report(args):
     ncell = shape[0] * shape[1] * shape[2]
     return {"workload": f"bbh_disk_{shape[0]}x{shape[1]}x{shape[2]}_mixed", "value": ncell * args.bbh_steps / el,
             "unit": "zone-cycles/s", "ms_per_step": el / args.bbh_steps * 1e3, "steps": args.bbh_steps,
+            "steps_total": 5 + args.warmup + args.bbh_steps,
             "kernel_ms_per_step_eager": kern, "launch_shapes": shapes, "graph": args.graph == "on", "finite": finite,
             "divb_max": divb, "metric": "1PN near-zone, quasi-circular inspiral (Peters), separation 20 M"}
 
@@ -403,7 +404,7 @@ def main():
     ap.add_argument("--bbh", choices=["auto", "off"], default="auto",
                     help="circumbinary-disk report on the binary metric")
     ap.add_argument("--bbh-grid", default="300x160x400")
-    ap.add_argument("--bbh-steps", type=int, default=20)
+    ap.add_argument("--bbh-steps", type=int, default=100)
     ap.add_argument("--c2p", choices=["auto", "off"], default="auto",
                     help="batched con-to-prim report (2^c2p-log2 zones)")
     ap.add_argument("--c2p-log2", type=int, default=24)
