@@ -1134,11 +1134,11 @@ struct GpuBE {
     return hip_check(hipGetLastError(), "k_emf");
   }
   static int update(hgpu_ctx* c, int stage) {
-    if (c->P.otf) {   // the stage's source terms, read by the update
+    if (c->P.otf) {   // the stage's source terms, into the buffer the update reads (the stage's level)
       Scope sm(c, "metric");
       const long long n = (long long)c->P.n1 * c->P.n2 * c->P.n3;
       hipLaunchKernelGGL(k_source_otf, dim3(nblocks(n)), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, c->pn, stage,
-                         c->connlev[stage]);
+                         const_cast<double*>(c->X.conn));
       int rc = hip_check(hipGetLastError(), "k_source_otf");
       if (rc) return rc;
     }
