@@ -35,6 +35,7 @@ class HgpuConfig(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("bbh_m1", ctypes.c_double), ("bbh_m2", ctypes.c_double), ("bbh_sep", ctypes.c_double),
         ("bbh_phi0", ctypes.c_double), ("bbh_inspiral", ctypes.c_int),
+        ("bc_nbr", ctypes.c_int), ("gpos", ctypes.c_int * 3),
     ]
 
 
@@ -64,6 +65,13 @@ SIGNATURES = {
     "hgpu_fixup_bounds": (ctypes.c_int, [ctypes.c_void_p]),
     "hgpu_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "hgpu_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "hgpu_advance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "hgpu_get_boundary_flags": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p, c_int_p]),
+    "hgpu_fixup_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "hgpu_end_step": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
+    "hgpu_set_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "hgpu_download_box": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p, c_int_p, c_double_p, c_int_p]),
+    "hgpu_upload_box": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p, c_int_p, c_double_p, c_int_p]),
     "hgpu_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "hgpu_set_metric_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "hgpu_get_metric_eval": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),

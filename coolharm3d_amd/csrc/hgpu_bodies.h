@@ -344,8 +344,8 @@ HD double face_flux(const Params& P, const Ptrs& X, int d, int l, int i, int j, 
   }
   double f = X.F[(d*NFLUX + l)*NT + c];
   if (l == RHO && d == 0 && P.bc == BC_SPHERICAL_OUTFLOW3) {
-    if (i == NG && f > 0) f = 0.;
-    if (i == NG + P.n1 && f < 0) f = 0.;
+    if (i == NG && f > 0 && !(P.bc_nbr & 1)) f = 0.;
+    if (i == NG + P.n1 && f < 0 && !(P.bc_nbr & 2)) f = 0.;
   }
   return f;
 }
@@ -845,6 +845,8 @@ HD void inflow_check(const Params& P, const Ptrs& X, double* pr, int ii, int jj,
 // lower (side 0) and upper (side 1) faces.
 HD void bound_zone(const Params& P, const Ptrs& X, double* S, int dir, int side, int g, int a, int b,
                    int do_pflag, FailRec& fr) {
+  // a face shared with another block gets its ghosts from that block
+  if ((P.bc_nbr >> (2*dir + side)) & 1) return;
   const long long NT = P.nt;
   const int n1e = P.n1 + NG - 1, n2e = P.n2 + NG - 1, n3e = P.n3 + NG - 1;
   int di, dj, dk, si, sj, sk;

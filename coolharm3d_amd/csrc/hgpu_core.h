@@ -72,6 +72,13 @@ inline int make_params(const hgpu_config& c, Params& P) {
   if (c.cooling && c.metric == 0) return set_error(-2, "the h/r cooling function needs a black-hole metric");
   if (c.bc == BC_SPHERICAL_OUTFLOW3 && (c.n2 < NG || c.n3 < NG))
     return set_error(-2, "reflecting/periodic boundaries need at least NG interior zones in x2 and x3");
+  if (c.bc_nbr < 0 || c.bc_nbr > 63) return set_error(-2, "bc_nbr holds 6 face bits");
+  {
+    const int n[3] = {c.n1, c.n2, c.n3};
+    for (int d = 0; d < 3; d++)
+      if (((c.bc_nbr >> (2*d)) & 3) && n[d] < NG)
+        return set_error(-2, "a face shared with another block needs at least NG interior zones across it");
+  }
   std::memset(&P, 0, sizeof(P));
   P.n1 = c.n1; P.n2 = c.n2; P.n3 = c.n3;
   P.n1t = c.n1 + 2*NG; P.n2t = c.n2 + 2*NG; P.n3t = c.n3 + 2*NG;
@@ -87,6 +94,7 @@ inline int make_params(const hgpu_config& c, Params& P) {
   P.degen[0] = (c.n1 == 1); P.degen[1] = (c.n2 == 1); P.degen[2] = (c.n3 == 1);
   P.rescale_r = c.rescale_r && !P.degen[0];
   P.bc = c.bc;
+  P.bc_nbr = c.bc_nbr;
   P.floor_dim = (c.coords == 1) ? 1 : 3;
   P.n_spatial_dims = (c.n1 > 1) + (c.n2 > 1) + (c.n3 > 1);
   P.track_nfail = c.track_nfail;
@@ -113,6 +121,7 @@ inline CoordParams coord_params(const hgpu_config& c) {
   C.a = (c.metric == 3) ? 0. : c.a;
   C.asq = C.a*C.a; C.M = 1.; C.R0 = c.R0;
   C.h_slope = c.h_slope; C.th_beg = c.th_beg; C.th_length = c.th_length;
+  for (int d = 0; d < 3; d++) C.gpos[d] = c.gpos[d];
   return C;
 }
 
@@ -234,6 +243,7 @@ struct Api {
       B.inspiral = cfg->bbh_inspiral;
       B.R0 = cfg->R0; B.th_beg = cfg->th_beg; B.th_length = cfg->th_length; B.h_slope = cfg->h_slope;
       for (int l = 0; l < 4; l++) c->pn.x0[l] = cfg->startx[l];
+      for (int d = 0; d < 3; d++) c->pn.gpos[d] = cfg->gpos[d];
     }
     rc = BE::init(c);
     if (rc) { delete c; return rc; }
@@ -435,7 +445,15 @@ struct Api {
     if (c->P.dyn) { X.geom = X.geom_new; X.cent_lvl = stage + 1; }
     rc |= BE::recover_queued(c, stage);
     rc |= bounds(c, dst, 0);
-    rc |= BE::fixup_interp(c, dst, PFLAG_INTERP_V);
+    return rc;
+  }
+
+  // The rest of the stage after bounds(prim,0) (src/step_ch.c:105-114,
+  // :171-180): the interpolating fixups, bounds(prim,1) where the boundary
+  // flags ask for it, the flags cleared.
+  static int stage_fixup(hgpu_ctx* c, int stage) {
+    double* dst = (stage == 0) ? c->X.ph : c->X.p;
+    int rc = BE::fixup_interp(c, dst, PFLAG_INTERP_V);
     rc |= BE::fixup_interp(c, dst, PFLAG_INTERP_PRIM);
     rc |= bounds(c, dst, 1);
     rc |= BE::clear_boundary_flag(c);
@@ -496,13 +514,58 @@ struct Api {
     return rc;
   }
 
-  // One advance() stage through the public entry point.
-  static int substep_api(hgpu_ctx* c, int stage) {
+  // One advance() stage and bounds(prim,0)'s physical part, through the
+  // public entry point: the step's geometry levels first (stage 0, binary
+  // metric), the flag map and failure words cleared as step_ch() does before
+  // each stage (src/step_ch.c:65-69, :117-121).
+  static int advance_api(hgpu_ctx* c, int stage) {
     int rc = 0;
     if (c->P.dyn && stage == 0) rc |= fill_levels(c);
+    rc |= BE::clear_failures(c);
     rc |= substep(c, stage);
+    return rc;
+  }
+  // The stage's fixups and bounds(prim,1); set: the boundary flags take the
+  // caller's (globally reduced) values first.
+  static int fixup_api(hgpu_ctx* c, int stage, bool set, int phys, int mpi) {
+    int rc = 0;
+    if (set) rc |= BE::set_boundary_flags(c, phys, mpi);
+    rc |= stage_fixup(c, stage);
     if (c->P.dyn && stage == 1) { c->X.geom = c->X.geom_n = c->X.geom_new = c->lev[2]; c->X.cent_lvl = 2; }
     return rc;
+  }
+  static int substep_api(hgpu_ctx* c, int stage) {
+    int rc = advance_api(c, stage);
+    rc |= fixup_api(c, stage, false, 0, 0);
+    return rc;
+  }
+  // t += dt and the next dt (src/step_ch.c:213-223) from the given speeds
+  // max_char_speed[0..3] or, without them, the block's own.
+  static int end_step(hgpu_ctx* c, const double* vmax4) {
+    int rc = 0;
+    if (vmax4) rc |= BE::set_vmax(c, vmax4);
+    rc |= BE::timestep(c);
+    return rc;
+  }
+  static int set_dt(hgpu_ctx* c, double dt) {
+    double sc[SC_WORDS];
+    int rc = BE::d2h(c, sc, c->X.scal, sizeof(sc));
+    if (rc) return rc;
+    sc[SC_DT] = dt;
+    rc = BE::h2d(c, c->X.scal, sc, sizeof(sc));
+    return rc ? rc : BE::sync(c);
+  }
+  static int box(hgpu_ctx* c, int which, const int* lo, const int* hi, double* prims, int* pflag, int to_host) {
+    if (!c || !lo || !hi) return set_error(-1, "null argument");
+    if (which != 0 && which != 1) return set_error(-2, "which must be 0 (p) or 1 (ph)");
+    const int nt[3] = {c->P.n1t, c->P.n2t, c->P.n3t};
+    Box b;
+    for (int d = 0; d < 3; d++) {
+      if (lo[d] < 0 || hi[d] >= nt[d] || lo[d] > hi[d]) return set_error(-2, "box outside the grid or empty");
+      b.lo[d] = lo[d];
+      b.hi[d] = hi[d];
+    }
+    return BE::box_copy(c, which ? c->X.ph : c->X.p, b, prims, pflag, to_host);
   }
 
   // One step_ch(): predictor, corrector, new time step (src/step_ch.c:50-234).
@@ -511,8 +574,10 @@ struct Api {
     if (c->P.dyn) rc |= fill_levels(c);
     rc |= BE::clear_failures(c);      // src/step_ch.c:65-69
     rc |= substep(c, 0);
+    rc |= stage_fixup(c, 0);
     rc |= BE::clear_failures(c);      // src/step_ch.c:117-121
     rc |= substep(c, 1);
+    rc |= stage_fixup(c, 1);
     rc |= BE::timestep(c);            // src/step_ch.c:218-223
     if (c->P.dyn) {
       // the state is now at t + dt: level 2 (centres) describes it

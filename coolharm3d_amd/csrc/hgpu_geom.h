@@ -21,6 +21,7 @@ struct CoordParams {
   int coords;        // 0 identity, 1 modified Kerr-Schild (COORD_DIAGONAL)
   double startx[4], dx[4];
   double a, asq, M, R0, h_slope, th_beg, th_length;
+  int gpos[3];       // global index of the block's zone 0 minus the local one (globalpos[1..3])
 };
 
 // seesaw(x, 0, 1), src/coord.c:4316-4324 with PERIODIC(), src/macros.h:489-490
@@ -32,9 +33,11 @@ inline double seesaw01(double x) {
   return (x1 < xn) ? (x1 - x0) : (xn2 - x1);
 }
 
-// coord(), src/coord.c:182-228 (single domain: global index = local index)
+// coord(), src/coord.c:182-228, from the global index as get_global_ijk()
+// gives it (src/coord.c:188): the block's offset gpos plus the local index
 inline void coord_xp(const CoordParams& C, int i, int j, int k, int pos, double* xp) {
   xp[0] = 0.;
+  i += C.gpos[0]; j += C.gpos[1]; k += C.gpos[2];
   switch (pos) {
     case FACE1: xp[1] = C.startx[1] + i*C.dx[1];         xp[2] = C.startx[2] + (j + 0.5)*C.dx[2]; xp[3] = C.startx[3] + (k + 0.5)*C.dx[3]; break;
     case FACE2: xp[1] = C.startx[1] + (i + 0.5)*C.dx[1]; xp[2] = C.startx[2] + j*C.dx[2];         xp[3] = C.startx[3] + (k + 0.5)*C.dx[3]; break;

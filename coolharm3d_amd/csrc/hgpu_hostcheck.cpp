@@ -19,7 +19,8 @@ static void host_commit(const Params& P, const Ptrs& X, const FailRec& fr, long 
   if (fr.pflag) {
     X.status[ST_FAILURE_EXISTS] = 1;
     X.status[(fr.pflag == PFLAG_INTERP_PRIM) ? ST_FAILTYPE_PRIM : ST_FAILTYPE_V] = 1;
-    if (fr.boundary) X.status[ST_BOUNDARY_PHYS] = 1;
+    if (fr.boundary & 1) X.status[ST_BOUNDARY_PHYS] = 1;
+    if (fr.boundary & 2) X.status[ST_BOUNDARY_MPI] = 1;
   }
 }
 
@@ -145,7 +146,39 @@ struct HostBE {
     }
     return 0;
   }
-  static int clear_boundary_flag(hgpu_ctx* c) { c->X.status[ST_BOUNDARY_PHYS] = 0; return 0; }
+  static int clear_boundary_flag(hgpu_ctx* c) {
+    c->X.status[ST_BOUNDARY_PHYS] = 0;
+    c->X.status[ST_BOUNDARY_MPI] = 0;
+    return 0;
+  }
+  static int set_boundary_flags(hgpu_ctx* c, int phys, int mpi) {
+    c->X.status[ST_BOUNDARY_PHYS] = phys ? 1 : 0;
+    c->X.status[ST_BOUNDARY_MPI] = mpi ? 1 : 0;
+    return 0;
+  }
+  static int set_vmax(hgpu_ctx* c, const double* vmax4) {
+    for (int l = 0; l < 4; l++) c->X.status[ST_VMAX + l] = dbl_bits(vmax4[l]);
+    return 0;
+  }
+  // a box of zones of S (and pflag) to (to_host) or from contiguous host
+  // arrays [i][j][k][8] / [i][j][k]
+  static int box_copy(hgpu_ctx* c, double* S, const Box& b, double* prims, int* pflag, int to_host) {
+    const long long NT = c->P.nt;
+    for (long long t = 0; t < b.count(); t++) {
+      int i, j, k;
+      b.at(t, i, j, k);
+      const long long cc = cidx(c->P, i, j, k);
+      if (prims) for (int v = 0; v < NP; v++) {
+        if (to_host) prims[t*NP + v] = S[v*NT + cc];
+        else S[v*NT + cc] = prims[t*NP + v];
+      }
+      if (pflag) {
+        if (to_host) pflag[t] = c->X.pflag[cc];
+        else c->X.pflag[cc] = pflag[t];
+      }
+    }
+    return 0;
+  }
   static int clear_failures(hgpu_ctx* c) {
     if (c->X.status[ST_FAILURE_EXISTS]) {
       for (long long t = 0; t < c->P.nt; t++) c->X.pflag[t] = 0;

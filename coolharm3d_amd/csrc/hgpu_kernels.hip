@@ -38,6 +38,8 @@ struct GpuPriv {
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, long long>> acc;
   std::map<std::string, hipEvent_t> open;   // mark() begin events
+  void* box_stage = nullptr;           // staging of hgpu_download_box / hgpu_upload_box
+  size_t box_bytes = 0;
 };
 
 static inline GpuPriv* priv(hgpu_ctx* c) { return static_cast<GpuPriv*>(c->be); }
@@ -86,7 +88,8 @@ __device__ __forceinline__ void commit_fail(const Params& P, const Ptrs& X, cons
       }
     }
     if (fr.pflag) {
-      unsigned f = 1u | ((fr.pflag == PFLAG_INTERP_PRIM) ? 2u : 4u) | (fr.boundary ? 8u : 0u);
+      unsigned f = 1u | ((fr.pflag == PFLAG_INTERP_PRIM) ? 2u : 4u) | ((fr.boundary & 1) ? 8u : 0u) |
+                   ((fr.boundary & 2) ? 16u : 0u);
       atomicOr(&s_flags, f);
     }
   }
@@ -103,6 +106,7 @@ __device__ __forceinline__ void commit_fail(const Params& P, const Ptrs& X, cons
       if (f & 2u) atomicOr(&X.status[ST_FAILTYPE_PRIM], 1ull);
       if (f & 4u) atomicOr(&X.status[ST_FAILTYPE_V], 1ull);
       if (f & 8u) atomicOr(&X.status[ST_BOUNDARY_PHYS], 1ull);
+      if (f & 16u) atomicOr(&X.status[ST_BOUNDARY_MPI], 1ull);
     }
     if (vmax_slot >= 0 && s_vmax) atomicMax(&X.status[ST_VMAX + vmax_slot], s_vmax);
   }
@@ -608,6 +612,35 @@ __global__ void k_reset_words(Ptrs X, int first, int n) {
   if ((int)threadIdx.x < n) X.status[first + threadIdx.x] = 0;
 }
 
+// boundary_phys_pflag / boundary_mpi_pflag: set to given values (set = 1) or cleared
+__global__ void k_boundary_words(Ptrs X, int set, int phys, int mpi) {
+  if (threadIdx.x == 0) {
+    X.status[ST_BOUNDARY_PHYS] = set ? (unsigned long long)(phys != 0) : 0ull;
+    X.status[ST_BOUNDARY_MPI] = set ? (unsigned long long)(mpi != 0) : 0ull;
+  }
+}
+
+// A box of zones of S (and pflag) to (to_box) or from a contiguous staging
+// buffer: prims [t][8] then pflag [t], t the zone's position in the box.
+__global__ __launch_bounds__(BLOCK) void k_box_copy(Params P, Ptrs X, double* S, Box b, double* stage,
+                                                    int* pstage, int to_box) {
+  const long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= b.count()) return;
+  int i, j, k;
+  b.at(t, i, j, k);
+  const long long c = cidx(P, i, j, k);
+  if (stage) {
+    for (int v = 0; v < NP; v++) {
+      if (to_box) stage[t*NP + v] = S[v*P.nt + c];
+      else S[v*P.nt + c] = stage[t*NP + v];
+    }
+  }
+  if (pstage) {
+    if (to_box) pstage[t] = X.pflag[c];
+    else X.pflag[c] = pstage[t];
+  }
+}
+
 __global__ void k_timestep(Params P, Ptrs X) {
   if (threadIdx.x == 0) timestep_update(P, X);
 }
@@ -780,6 +813,7 @@ struct GpuBE {
     if (!g) return;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->step_exec) (void)hipGraphExecDestroy(g->step_exec);
+    if (g->box_stage) (void)hipFree(g->box_stage);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     for (auto& pe : g->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
     for (auto ev : g->pool) (void)hipEventDestroy(ev);
@@ -1181,8 +1215,49 @@ struct GpuBE {
     return hip_check(hipGetLastError(), "k_fixup_interp");
   }
   static int clear_boundary_flag(hgpu_ctx* c) {
-    hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_BOUNDARY_PHYS, 1);
-    return hip_check(hipGetLastError(), "k_reset_words");
+    hipLaunchKernelGGL(k_boundary_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, 0, 0, 0);
+    return hip_check(hipGetLastError(), "k_boundary_words");
+  }
+  static int set_boundary_flags(hgpu_ctx* c, int phys, int mpi) {
+    hipLaunchKernelGGL(k_boundary_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, 1, phys, mpi);
+    return hip_check(hipGetLastError(), "k_boundary_words");
+  }
+  static int set_vmax(hgpu_ctx* c, const double* vmax4) {
+    unsigned long long w[4];
+    for (int l = 0; l < 4; l++) w[l] = dbl_bits(vmax4[l]);
+    return h2d(c, c->X.status + ST_VMAX, w, sizeof(w));
+  }
+  // a box of zones of S (and pflag) to (to_host) or from contiguous host
+  // arrays [i][j][k][8] / [i][j][k], through a device staging buffer kept
+  // between calls (grown when a larger box comes)
+  static int box_copy(hgpu_ctx* c, double* S, const Box& b, double* prims, int* pflag, int to_host) {
+    GpuPriv* g = priv(c);
+    const long long n = b.count();
+    const size_t need = (size_t)n * (NP * sizeof(double) + sizeof(int));
+    int rc = 0;
+    if (need > g->box_bytes) {
+      rc |= sync(c);
+      if (g->box_stage) (void)hipFree(g->box_stage);
+      g->box_stage = nullptr;
+      g->box_bytes = 0;
+      rc |= hip_check(hipMalloc(&g->box_stage, need), "hipMalloc");
+      if (rc) return rc;
+      g->box_bytes = need;
+    }
+    double* ds = static_cast<double*>(g->box_stage);
+    int* ps = reinterpret_cast<int*>(ds + n * NP);
+    if (!to_host) {
+      if (prims) rc |= h2d(c, ds, prims, sizeof(double) * NP * n);
+      if (pflag) rc |= h2d(c, ps, pflag, sizeof(int) * n);
+    }
+    hipLaunchKernelGGL(k_box_copy, dim3(nblocks(n)), dim3(BLOCK), 0, g->stream, c->P, c->X, S, b,
+                       prims ? ds : nullptr, pflag ? ps : nullptr, to_host);
+    rc |= hip_check(hipGetLastError(), "k_box_copy");
+    if (to_host) {
+      if (prims) rc |= d2h(c, prims, ds, sizeof(double) * NP * n);
+      if (pflag) rc |= d2h(c, pflag, ps, sizeof(int) * n);
+    }
+    return rc ? rc : sync(c);
   }
   static int clear_failures(hgpu_ctx* c) {
     hipLaunchKernelGGL(k_clear_pflag, dim3(nblocks(c->P.nt)), dim3(BLOCK), 0, priv(c)->stream, c->X, c->P.nt);

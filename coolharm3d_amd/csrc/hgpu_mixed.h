@@ -486,8 +486,8 @@ HD double face_flux_mixed(const Params& P, const Ptrs& X, int d, int l, int i, i
   }
   double f = (double)X.Ff[(d*NFLUX + l)*NT + c];
   if (l == RHO && d == 0 && P.bc == BC_SPHERICAL_OUTFLOW3) {
-    if (i == NG && f > 0) f = 0.;
-    if (i == NG + P.n1 && f < 0) f = 0.;
+    if (i == NG && f > 0 && !(P.bc_nbr & 1)) f = 0.;
+    if (i == NG + P.n1 && f < 0 && !(P.bc_nbr & 2)) f = 0.;
   }
   return f;
 }

@@ -965,7 +965,7 @@ HD double rt_val(const Params& P, const Ptrs& X, int v, int i, double x) {
 struct FailRec {
   int nf[N_NFAIL];
   int pflag;        // last pflag response (0: none)
-  int boundary;     // check_boundary_pflag() hit a physical boundary
+  int boundary;     // check_boundary_pflag(): bit 0 a physical boundary, bit 1 one shared with another block
   double vmax;      // fastest characteristic speed of an interior face (0: none)
   HD void clear() {
     for (int l = 0; l < N_NFAIL; l++) nf[l] = 0;
@@ -981,9 +981,15 @@ HD void fail_set(const Params& P, FailRec& r, int nfail_id, int state, int i, in
   r.nf[nfail_id]++;
   if (state) {
     r.pflag = (state == 1) ? PFLAG_INTERP_PRIM : PFLAG_INTERP_V;
-    int n1e = P.n1 + NG - 1, n2e = P.n2 + NG - 1, n3e = P.n3 + NG - 1;
-    if (i < NG + NG || i > n1e - NG || j < NG + NG || j > n2e - NG || k < NG + NG || k > n3e - NG)
-      r.boundary = 1;
+    // within NG zones of a face: boundary_phys_pflag, or boundary_mpi_pflag
+    // when the face borders another block
+    const int idx[3] = {i, j, k}, n[3] = {P.n1, P.n2, P.n3};
+    for (int d = 0; d < 3; d++) {
+      int side = -1;
+      if (idx[d] < NG + NG) side = 0;
+      else if (idx[d] > n[d] - 1) side = 1;   // > N_dE - NG with N_dE = n + NG - 1
+      if (side >= 0) r.boundary |= ((P.bc_nbr >> (2*d + side)) & 1) ? 2 : 1;
+    }
   }
 }
 

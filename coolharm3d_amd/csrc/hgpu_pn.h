@@ -387,9 +387,9 @@ HD void zone_X(const Params& P, const PnGrid& G, int i, int j, int k, int pos, d
   const double h1 = (pos == FACE1 || pos == CORN) ? 0. : 0.5;
   const double h2 = (pos == FACE2 || pos == CORN) ? 0. : 0.5;
   const double h3 = (pos == FACE3 || pos == CORN) ? 0. : 0.5;
-  X[1] = G.x0[1] + (i + h1) * P.dx[1];
-  X[2] = G.x0[2] + (j + h2) * P.dx[2];
-  X[3] = G.x0[3] + (k + h3) * P.dx[3];
+  X[1] = G.x0[1] + (i + G.gpos[0] + h1) * P.dx[1];
+  X[2] = G.x0[2] + (j + G.gpos[1] + h2) * P.dx[2];
+  X[3] = G.x0[3] + (k + G.gpos[2] + h3) * P.dx[3];
 }
 // Coordinate functions of zone (i,j,k) at position pos from the tables (the
 // values pn_gcov_num() forms from the coordinates).
@@ -408,21 +408,21 @@ HD void pn_zone_coords(const Params& P, const PnGrid& G, int i, int j, int k, in
 HD void pn_tab_entry(const Params& P, const PnGrid& G, double* tab, long long e) {
   if (e < 2LL * P.n1t) {
     const int h = (int)(e / P.n1t), i = (int)(e % P.n1t);
-    pn_coord1(G.B, G.x0[1] + (i + 0.5 * h) * P.dx[1], &tab[2 * e], &tab[2 * e + 1]);
+    pn_coord1(G.B, G.x0[1] + (i + G.gpos[0] + 0.5 * h) * P.dx[1], &tab[2 * e], &tab[2 * e + 1]);
     return;
   }
   e -= 2LL * P.n1t;
   if (e < 2LL * P.n2t) {
     const int h = (int)(e / P.n2t), j = (int)(e % P.n2t);
     double* t = tab + 4 * P.n1t + 3 * e;
-    pn_coord2(G.B, G.x0[2] + (j + 0.5 * h) * P.dx[2], &t[0], &t[1], &t[2]);
+    pn_coord2(G.B, G.x0[2] + (j + G.gpos[1] + 0.5 * h) * P.dx[2], &t[0], &t[1], &t[2]);
     return;
   }
   e -= 2LL * P.n2t;
   if (e < 2LL * P.n3t) {
     const int h = (int)(e / P.n3t), k = (int)(e % P.n3t);
     double* t = tab + 4 * P.n1t + 6 * P.n2t + 2 * e;
-    pn_coord3(G.x0[3] + (k + 0.5 * h) * P.dx[3], &t[0], &t[1]);
+    pn_coord3(G.x0[3] + (k + G.gpos[2] + 0.5 * h) * P.dx[3], &t[0], &t[1]);
   }
 }
 HD long long pn_tab_entries(const Params& P) { return 2LL * (P.n1t + P.n2t + P.n3t); }

@@ -74,6 +74,7 @@ enum {
   ST_VMAX = 4,               // 4..7: max_char_speed[0..3] as ordered bits
   ST_NFAILTOT = 8,           // 8..22: nfailtot[15]
   ST_SLOW = 23,              // zones queued for the full con-to-prim tree this stage
+  ST_BOUNDARY_MPI = 24,      // boundary_mpi_pflag
   ST_WORDS = 32
 };
 
@@ -92,6 +93,7 @@ struct Params {
   int cooling;             // USE_COOLING_FUNCTION == 1
   int rescale_r;           // RESCALE_R
   int bc;                  // BC_CARTESIAN_OUTFLOW or BC_SPHERICAL_OUTFLOW3
+  int bc_nbr;              // faces bordering another block: bit 2*dir + side (dir 0..2)
   int floor_dim;           // 1: floors per x1 index, 3: per cell
   int degen[3];            // N_d == 1
   int n_spatial_dims;
@@ -129,6 +131,7 @@ struct PnBinary {
 struct PnGrid {
   PnBinary B;
   double x0[4];
+  int gpos[3];            // global index of the block's zone 0 (a decomposed grid's block offset)
   const double* tab;
 };
 

@@ -25,6 +25,18 @@
  * After the host edits p (restart, re-initialisation), step_ch_hgpu_upload()
  * sends it to the device again.
  *
+ * Domain-decomposed runs (USEMPI): each rank's context holds its block
+ * (hgpu_config.gpos = globalpos, bc_nbr from bc_pid), and step_ch() runs each
+ * stage in the two parts the reference separates by its halo exchange:
+ * hgpu_advance(); the send slabs of every face shared with another rank copied
+ * to the host arrays, the reference's own bounds_mpi() and bounds_pflag_mpi()
+ * (src/harm_mpi.c:1631-1744), the received ghost slabs copied back;
+ * mpi_global_imax2() of the boundary flags; hgpu_fixup_stage(); a second
+ * exchange when boundary_mpi_pflag is set (bounds(prim,1), src/step_ch.c:110-114);
+ * then mpi_global_vmax() of the speeds, hgpu_end_step(), mpi_global_min() of
+ * the new dt (src/step_ch.c:561, :618).  HGPU_STAGES=1 in the environment
+ * selects this path in a single-rank run too (no exchange then).
+ *
  * Compile-time inputs the reference keeps inside function bodies:
  *   HGPU_H_O_R   cooling target H/R of cooling_func_hr_disk() (src/phys.c:655).
  */
@@ -42,6 +54,7 @@ void step_ch_hgpu_sync_host(void);
 void step_ch_hgpu_upload(void);
 
 static hgpu_ctx *hg = NULL;
+static int hg_stages = 0;              /* run the stages through the halo hooks */
 static double *hg_stage = NULL;        /* [N1TOT][N2TOT][N3TOT][8] */
 static int *hg_istage = NULL;          /* [15][N1TOT][N2TOT][N3TOT] */
 static int *hg_nfail_base = NULL;      /* nfail counts recorded before the first step */
@@ -93,6 +106,13 @@ static void hg_setup(void) {
   cfg.r_isco = r_isco;
   cfg.h_o_r = HGPU_H_O_R;
   cfg.rhomin = RHOMIN; cfg.uumin = UUMIN;
+  /* the block of this rank: faces shared with another rank, global offset */
+  for (l = 1; l < NDIM; l++) {
+    if (bc_pid[l][BCDN] != BC_PHYS) cfg.bc_nbr |= 1 << (2 * (l - 1));
+    if (bc_pid[l][BCUP] != BC_PHYS) cfg.bc_nbr |= 1 << (2 * (l - 1) + 1);
+    cfg.gpos[l - 1] = globalpos[l];
+  }
+  hg_stages = (cfg.bc_nbr != 0) || (getenv("HGPU_STAGES") && atoi(getenv("HGPU_STAGES")));
 #if( FLOOR_DIM == 0 )
   cfg.rhopower = 0.; cfg.uupower = 0.;
 #else
@@ -164,12 +184,81 @@ void step_ch_hgpu_sync_host(void) {
   }
 }
 
+#if( USEMPI )
+/* bc_range_send/recv[dir][side][index dir][0 beg, 1 end]: the slabs
+   bounds_mpi() sends and receives (src/harm_mpi.c:470-560) */
+extern int bc_range_send[NDIM][2][NDIM][2];
+extern int bc_range_recv[NDIM][2][NDIM][2];
+
+/* one box of zones between the device state `which` and the host arrays */
+static void hg_box(double ****prim, int which, int r[NDIM][2], int to_host) {
+  int i, j, k, l, lo[3], hi[3];
+  long c = 0;
+  for (l = 0; l < 3; l++) { lo[l] = r[l + 1][0]; hi[l] = r[l + 1][1]; }
+  if (to_host) {
+    hg_check(hgpu_download_box(hg, which, lo, hi, hg_stage, hg_istage), "hgpu_download_box");
+    for (i = lo[0]; i <= hi[0]; i++) for (j = lo[1]; j <= hi[1]; j++) for (k = lo[2]; k <= hi[2]; k++) {
+      PLOOP prim[i][j][k][l] = hg_stage[NP * c + l];
+      pflag[i][j][k] = hg_istage[c];
+      c++;
+    }
+  } else {
+    for (i = lo[0]; i <= hi[0]; i++) for (j = lo[1]; j <= hi[1]; j++) for (k = lo[2]; k <= hi[2]; k++) {
+      PLOOP hg_stage[NP * c + l] = prim[i][j][k][l];
+      hg_istage[c] = pflag[i][j][k];
+      c++;
+    }
+    hg_check(hgpu_upload_box(hg, which, lo, hi, hg_stage, hg_istage), "hgpu_upload_box");
+  }
+}
+#endif
+
+/* the MPI part of bounds(prim, .) on the device state `which` (0 p, 1 ph) */
+static void hg_exchange(double ****prim, int which) {
+#if( USEMPI )
+  int d, side;
+  for (d = 1; d < NDIM; d++) for (side = 0; side < 2; side++)
+    if (bc_pid[d][side] != BC_PHYS) hg_box(prim, which, bc_range_send[d][side], 1);
+  bounds_mpi(prim);
+  bounds_pflag_mpi(pflag);
+  for (d = 1; d < NDIM; d++) for (side = 0; side < 2; side++)
+    if (bc_pid[d][side] != BC_PHYS) hg_box(prim, which, bc_range_recv[d][side], 0);
+#else
+  (void)prim; (void)which;
+#endif
+}
+
+/* one advance() stage with bounds and fixups (src/step_ch.c:92-121, :159-180) */
+static void hg_stage_run(int stage) {
+  int fe, phys, mpi;
+  double ****prim = stage ? p : ph;
+  int which = stage ? 0 : 1;
+  hg_check(hgpu_advance(hg, stage), "hgpu_advance");
+  hg_exchange(prim, which);
+  hg_check(hgpu_get_boundary_flags(hg, &fe, &phys, &mpi), "hgpu_get_boundary_flags");
+  mpi_global_imax2(&mpi, &phys);
+  hg_check(hgpu_fixup_stage(hg, stage, phys, mpi), "hgpu_fixup_stage");
+  if (mpi) hg_exchange(prim, which);
+}
+
 void step_ch(void) {
   double tt, dtt, dto, vmax[4];
   long long nft[N_NFAIL];
   int fe, l;
   if (!hg) hg_setup();
-  hg_check(hgpu_step(hg, 1), "hgpu_step");
+  if (hg_stages) {
+    hg_stage_run(0);
+    hg_stage_run(1);
+    hg_check(hgpu_get_status(hg, &fe, vmax), "hgpu_get_status");
+    mpi_global_vmax(NDIM, vmax);
+    hg_check(hgpu_end_step(hg, vmax), "hgpu_end_step");
+    hg_check(hgpu_get_time(hg, &tt, &dtt, &dto), "hgpu_get_time");
+    dto = dtt;
+    mpi_global_min(&dtt);
+    if (dtt != dto) hg_check(hgpu_set_dt(hg, dtt), "hgpu_set_dt");
+  } else {
+    hg_check(hgpu_step(hg, 1), "hgpu_step");
+  }
   hg_check(hgpu_get_time(hg, &tt, &dtt, &dto), "hgpu_get_time");
   t_old = t;
   t = tt;

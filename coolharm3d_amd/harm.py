@@ -75,6 +75,9 @@ def config_for(grid: GridSpec, **kw) -> _lib.HgpuConfig:
         c.rhomin, c.uumin = kw.get("rhomin", 2.0e-10), kw.get("uumin", 2.0e-12)
     c.rhopower, c.uupower = kw.get("rhopower", -1.5), kw.get("uupower", -2.5)
     c.precision = {"fp64": 0, "mixed": 1}[kw.get("precision", "fp64")]
+    c.bc_nbr = kw.get("bc_nbr", 0)
+    for d, g in enumerate(kw.get("gpos", (0, 0, 0))):
+        c.gpos[d] = g
     return c
 
 
@@ -186,7 +189,56 @@ class Harm3D:
         return tuple(int(x) for x in v)
 
     def advance(self, stage):
+        """One advance() stage with its boundaries and fixups (hgpu_substep)."""
         self._chk(self.lib.hgpu_substep(self.h, stage), "hgpu_substep")
+
+    # -- a stage in the parts a decomposed run separates by its halo exchange
+    def advance_only(self, stage):
+        """advance() and the physical boundary conditions of bounds(prim,0)
+        (hgpu_advance)."""
+        self._chk(self.lib.hgpu_advance(self.h, stage), "hgpu_advance")
+
+    def boundary_flags(self):
+        """(failure_exists, boundary_phys_pflag, boundary_mpi_pflag) of the stage."""
+        v = np.zeros(3, np.int32)
+        self._chk(self.lib.hgpu_get_boundary_flags(self.h, _ip(v[0:1]), _ip(v[1:2]), _ip(v[2:3])),
+                  "hgpu_get_boundary_flags")
+        return int(v[0]), int(v[1]), int(v[2])
+
+    def fixup_stage(self, stage, phys, mpi):
+        """The stage's interpolating fixups and bounds(prim,1) with the given
+        (globally reduced) boundary flags (hgpu_fixup_stage)."""
+        self._chk(self.lib.hgpu_fixup_stage(self.h, stage, int(phys), int(mpi)), "hgpu_fixup_stage")
+
+    def end_step(self, vmax4=None):
+        """t += dt and the next dt from max_char_speed (given, or the block's own)."""
+        if vmax4 is None:
+            self._chk(self.lib.hgpu_end_step(self.h, None), "hgpu_end_step")
+        else:
+            v = np.ascontiguousarray(vmax4, np.float64)
+            assert v.shape == (4,)
+            self._chk(self.lib.hgpu_end_step(self.h, _dp(v)), "hgpu_end_step")
+        self.nstep += 1
+
+    def set_dt(self, dt):
+        self._chk(self.lib.hgpu_set_dt(self.h, float(dt)), "hgpu_set_dt")
+
+    def download_box(self, which, lo, hi):
+        """(prims [ni][nj][nk][8], pflag [ni][nj][nk]) of the zones lo <= idx <= hi."""
+        lo_a, hi_a = np.array(lo, np.int32), np.array(hi, np.int32)
+        n = tuple(int(h - l + 1) for l, h in zip(lo, hi))
+        pr = np.empty(n + (8,))
+        pf = np.empty(n, np.int32)
+        self._chk(self.lib.hgpu_download_box(self.h, which, _ip(lo_a), _ip(hi_a), _dp(pr), _ip(pf)),
+                  "hgpu_download_box")
+        return pr, pf
+
+    def upload_box(self, which, lo, hi, prims, pflag):
+        lo_a, hi_a = np.array(lo, np.int32), np.array(hi, np.int32)
+        pr = np.ascontiguousarray(prims, np.float64)
+        pf = np.ascontiguousarray(pflag, np.int32)
+        self._chk(self.lib.hgpu_upload_box(self.h, which, _ip(lo_a), _ip(hi_a), _dp(pr), _ip(pf)),
+                  "hgpu_upload_box")
 
     def sync(self):
         self._chk(self.lib.hgpu_sync(self.h), "hgpu_sync")

@@ -56,6 +56,17 @@ typedef struct hgpu_config {
    * orbit shrinks by radiation reaction (1) or stays circular (0)                      */
   double bbh_m1, bbh_m2, bbh_sep, bbh_phi0;
   int bbh_inspiral;
+  /* Domain decomposition (zero for a single block).  bc_nbr: bit 2*(d-1) + side is set when
+   * face side (0 lower, 1 upper) of direction x_d borders another block, i.e. bc_pid[d][side] !=
+   * BC_PHYS (src/harm_mpi.c:1560-1600): that face gets no physical boundary condition, no
+   * fix_flux() (src/bounds.c:2259-2270) and no inflow_check(), and a flagged zone near it raises
+   * boundary_mpi_pflag instead of boundary_phys_pflag (check_boundary_pflag(),
+   * src/harm_mpi.c:1388-1430); its ghosts are filled by the caller (hgpu_upload_box()).
+   * gpos: the block's globalpos[1..3] (src/harm_mpi.c:1445-1480), the global index of its first
+   * interior zone, so zone coordinates are startx + (gpos + i - NG + 1/2) dx as coord() forms them
+   * from the global index (src/coord.c:182-228); startx[] stays the global grid's.             */
+  int bc_nbr;
+  int gpos[3];
 } hgpu_config;
 
 typedef struct hgpu_ctx hgpu_ctx;
@@ -136,8 +147,47 @@ int hgpu_set_metric_eval(hgpu_ctx *ctx, int in_kernels);
 int hgpu_get_metric_eval(hgpu_ctx *ctx, int *in_kernels);
 
 /* One advance() stage (0: predictor into ph, 1: corrector into p) followed by
- * bounds and fixups, as step_ch() sequences them (src/step_ch.c:92-180). */
+ * bounds and fixups, as step_ch() sequences them (src/step_ch.c:63-121,
+ * :159-180), the flag map cleared first.  hgpu_substep(ctx, 0), hgpu_substep(ctx,
+ * 1), hgpu_end_step(ctx, NULL) is one hgpu_step(ctx, 1). */
 int hgpu_substep(hgpu_ctx *ctx, int stage);
+
+/* The same stage in the two parts a decomposed run separates with its halo
+ * exchange (bounds(prim,0) then bounds(prim,1) around the fixups,
+ * src/step_ch.c:103-121):
+ *   hgpu_advance():     failures cleared (src/step_ch.c:65-69, :117-121),
+ *                       advance() and the physical boundary conditions of
+ *                       bounds(prim,0) (faces not in bc_nbr; src/bounds.c:31-579);
+ *   -- the caller exchanges the ghosts of the bc_nbr faces (hgpu_download_box(),
+ *      bounds_mpi(), bounds_pflag_mpi(), hgpu_upload_box(), src/harm_mpi.c:1631-1744)
+ *      and reduces the boundary flags (mpi_global_imax2(), src/bounds.c:2164-2166) --
+ *   hgpu_fixup_stage(): with the boundary flags set to phys / mpi (the
+ *                       globally reduced values): fixup_interp_v(),
+ *                       fixup_interp_prim() (src/fixup.c:499-761), then the
+ *                       physical part of bounds(prim,1) if phys; the flags are
+ *                       then cleared.  The caller exchanges ghosts again if mpi.
+ * hgpu_get_boundary_flags() reads failure_exists, boundary_phys_pflag and
+ * boundary_mpi_pflag of the stage (waits for the stream). */
+int hgpu_advance(hgpu_ctx *ctx, int stage);
+int hgpu_get_boundary_flags(hgpu_ctx *ctx, int *failure_exists, int *phys, int *mpi);
+int hgpu_fixup_stage(hgpu_ctx *ctx, int stage, int phys, int mpi);
+
+/* End of step_ch() after the corrector (src/step_ch.c:213-223): t_old = t,
+ * t += dt, dt_old = dt, dt = timestep() from max_char_speed[1..3] (vmax4 !=
+ * NULL: these values, e.g. after mpi_global_vmax(), src/step_ch.c:561; NULL:
+ * the block's own).  hgpu_set_dt() replaces dt (after mpi_global_min(),
+ * src/step_ch.c:618). */
+int hgpu_end_step(hgpu_ctx *ctx, const double *vmax4);
+int hgpu_set_dt(hgpu_ctx *ctx, double dt);
+
+/* Copies of a box of zones lo[d] <= i_d <= hi[d] (indices including ghosts,
+ * 0 <= lo <= hi < N_dTOT) of p (which = 0) or ph (1): primitives [i][j][k][8]
+ * and pflag [i][j][k] over the box, i slowest (either pointer may be NULL).
+ * The halo slabs bounds_mpi() sends and receives (src/harm_mpi.c:470-560:
+ * NG layers along the face, every zone across it) are such boxes. */
+int hgpu_download_box(hgpu_ctx *ctx, int which, const int *lo, const int *hi, double *prims, int *pflag);
+int hgpu_upload_box(hgpu_ctx *ctx, int which, const int *lo, const int *hi, const double *prims,
+                    const int *pflag);
 
 /* Failure bookkeeping: pflag[NT] (src/macros.h:256-259), p_gamma[NT],
  * nfail[15][NT] (src/macros.h:236-251), nfailtot[15]. */

@@ -266,3 +266,76 @@ void ref_c2p_batch(long n, double a_spin, double gam_in, const double *U, const 
     for (l = 0; l < NP; l++) prim[l * n + z] = pz[l];
   }
 }
+
+/* ------------------------------------------------------------------------
+ * Branch coverage of the recovery tree.  The library is linked with
+ * -Wl,--wrap for the solvers recover_primitives() and fixup_entropy_eq() call
+ * (src/step_ch.c:832, :842; src/fixup.c:401, :409) and for the two
+ * interpolating fixups (src/step_ch.c:106-107, :172-173), so every call from
+ * the reference's own objects passes through a counter here on its way to the
+ * unmodified function.  ref_branch[s][0] counts calls of solver s,
+ * ref_branch[s][1] the calls that returned 0; for s = 0..3 the solvers are
+ * Utoprim_2d_fast, Utoprim_1d, Utoprim_1d_ee, Utoprim_1d_ee2; s = 4, 5 are
+ * fixup_interp_v and fixup_interp_prim (calls only).  The return codes of each
+ * solver are tallied in ref_codes[s][0..REF_NCODES-1] as (code, count) pairs.
+ * ------------------------------------------------------------------------ */
+#define REF_NBRANCH 6
+#define REF_NCODES 32
+long ref_branch[REF_NBRANCH][2];
+long ref_codes[REF_NBRANCH][REF_NCODES][2];
+
+static void ref_tally(int s, int rc)
+{
+  int c;
+  ref_branch[s][0]++;
+  if (rc == 0) ref_branch[s][1]++;
+  for (c = 0; c < REF_NCODES; c++) {
+    if (ref_codes[s][c][1] == 0) { ref_codes[s][c][0] = rc; ref_codes[s][c][1] = 1; return; }
+    if (ref_codes[s][c][0] == rc) { ref_codes[s][c][1]++; return; }
+  }
+}
+
+void ref_reset_branch(void)
+{
+  memset(ref_branch, 0, sizeof(ref_branch));
+  memset(ref_codes, 0, sizeof(ref_codes));
+}
+
+int __real_Utoprim_2d_fast(double *U, struct of_geom *geom, double *prim, double *gamma, double *bsq);
+int __wrap_Utoprim_2d_fast(double *U, struct of_geom *geom, double *prim, double *gamma, double *bsq)
+{
+  int rc = __real_Utoprim_2d_fast(U, geom, prim, gamma, bsq);
+  ref_tally(0, rc);
+  return rc;
+}
+
+int __real_Utoprim_1d(double U[NP], double gcov[NDIM][NDIM], double gcon[NDIM][NDIM], double gdet,
+                      double prim[NP], double *gamma, double *bsq);
+int __wrap_Utoprim_1d(double U[NP], double gcov[NDIM][NDIM], double gcon[NDIM][NDIM], double gdet,
+                      double prim[NP], double *gamma, double *bsq)
+{
+  int rc = __real_Utoprim_1d(U, gcov, gcon, gdet, prim, gamma, bsq);
+  ref_tally(1, rc);
+  return rc;
+}
+
+int __real_Utoprim_1d_ee(double U[NP], double S, struct of_geom *geom, double prim[NP], double *gamma);
+int __wrap_Utoprim_1d_ee(double U[NP], double S, struct of_geom *geom, double prim[NP], double *gamma)
+{
+  int rc = __real_Utoprim_1d_ee(U, S, geom, prim, gamma);
+  ref_tally(2, rc);
+  return rc;
+}
+
+int __real_Utoprim_1d_ee2(double U[NP], double S, struct of_geom *geom, double prim[NP], double *gamma);
+int __wrap_Utoprim_1d_ee2(double U[NP], double S, struct of_geom *geom, double prim[NP], double *gamma)
+{
+  int rc = __real_Utoprim_1d_ee2(U, S, geom, prim, gamma);
+  ref_tally(3, rc);
+  return rc;
+}
+
+void __real_fixup_interp_v(double ****pv);
+void __wrap_fixup_interp_v(double ****pv) { ref_tally(4, 0); __real_fixup_interp_v(pv); }
+void __real_fixup_interp_prim(double ****pv);
+void __wrap_fixup_interp_prim(double ****pv) { ref_tally(5, 0); __real_fixup_interp_prim(pv); }

@@ -154,6 +154,28 @@ class RefRun:
     def step(self, n=1):
         self.lib.ref_step(n)
 
+    BRANCH_NAMES = ("Utoprim_2d_fast", "Utoprim_1d", "Utoprim_1d_ee", "Utoprim_1d_ee2",
+                    "fixup_interp_v", "fixup_interp_prim")
+
+    def reset_branch(self):
+        self.lib.ref_reset_branch()
+
+    def branch(self):
+        """Calls and successful returns of the recovery tree's solvers and
+        fixups since the last reset_branch() (ref_harness.c, linked with
+        --wrap): {name: (calls, returned 0, {code: count})}."""
+        calls = (ctypes.c_long * 12).in_dll(self.lib, "ref_branch")
+        codes = (ctypes.c_long * (6 * 32 * 2)).in_dll(self.lib, "ref_codes")
+        out = {}
+        for s, name in enumerate(self.BRANCH_NAMES):
+            hist = {}
+            for c in range(32):
+                code, count = codes[(s * 32 + c) * 2], codes[(s * 32 + c) * 2 + 1]
+                if count:
+                    hist[int(code)] = int(count)
+            out[name] = (int(calls[2 * s]), int(calls[2 * s + 1]), hist)
+        return out
+
     def diag(self):
         a = np.zeros(3)
         self.lib.ref_get_diag(_p(a))

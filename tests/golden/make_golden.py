@@ -11,6 +11,17 @@ the reference's outputs on the configurations the parity tests use:
 * c1_torus24x24x16.npz -- Fishbone-Moncrief torus, Kerr a = 0.9 in MKS
   coordinates, PPM, HLL, PPM EMFs, h/r cooling, floors 1e-7 r^-1.5 and
   1e-9 r^-2.5 (coolharm3d_amd.problems.torus, seed 1234); after 1, 10, 100 steps.
+* c0f_balsara1024_failures.npz, c1f_torus24x24x16_failures.npz -- the same two
+  set-ups with a few zones of the initial state overwritten (FAILURES_C0,
+  FAILURES_C1: negative densities and internal energies, Lorentz factors of
+  ~200 and ~1e4, singly and in 3x3x3 clusters) so that step one runs every
+  branch of recover_primitives() (src/step_ch.c:801-1024), the entropy fix
+  with both of its solvers (src/fixup.c:313-454) and both interpolating fixups
+  with and without enough good neighbours (src/fixup.c:499-761).  Besides the
+  state, pflag and nfail after 1, 2, 10 steps, these record the reference's
+  branch counters of step one (ref_harness.c: calls and successful returns of
+  Utoprim_2d_fast, Utoprim_1d, Utoprim_1d_ee, Utoprim_1d_ee2, the fixups'
+  calls, and each solver's return codes).
 
 Usage:  python tests/golden/make_golden.py   (needs oracle/_ref built: make -C oracle ref)
 Each configuration runs in its own process (the reference keeps its grid in
@@ -44,6 +55,103 @@ def record(R, out, tag):
     s = R.scalars()
     out[f"{tag}_t"] = np.float64(s["t"])
     out[f"{tag}_dt"] = np.float64(s["dt"])
+
+
+# Overwritten zones (i, j, k including ghosts), applied to the reference's
+# initial state after its init-time fixup and boundaries.  Kinds:
+#   "rho" / "u": set the density / internal energy to the value;
+#   "ut": scale the velocity u~^i (coordinate components) to Euclidean length
+#         value, keeping its direction (x1 where it is zero);
+# a kind ending in "3" applies to the 3x3x3 block centred on the zone.  Every
+# zone is at least NG zones from the grid edges, so the ghosts stay valid.
+FAILURES_C0 = [("ut", 200, 3, 3, 200.0), ("ut", 300, 3, 3, 1.0e5), ("rho", 400, 3, 3, -1.0e-3),
+               ("u", 500, 3, 3, -1.0e-3), ("ut", 699, 3, 3, 1.0e5), ("ut", 700, 3, 3, 1.0e5),
+               ("ut", 701, 3, 3, 1.0e5), ("rho", 799, 3, 3, -1.0e-3), ("rho", 800, 3, 3, -1.0e-3),
+               ("rho", 801, 3, 3, -1.0e-3), ("ut", 899, 3, 3, 200.0), ("ut", 900, 3, 3, 200.0),
+               ("ut", 901, 3, 3, 200.0)]
+FAILURES_C1 = [("ut3", 14, 14, 8, 1.0e4), ("rho3", 20, 20, 12, -1.0e-3), ("ut", 12, 12, 15, 20.0),
+               ("ut", 18, 10, 14, 1.0e4), ("u", 8, 8, 12, -1.0e-3), ("u", 6, 14, 8, 1.0),
+               ("rho", 22, 12, 6, -1.0e-3)]
+FAIL_STEPS = (1, 2, 10)
+
+
+def corrupt(p, spec):
+    """Copy of the primitive state p[i][j][k][8] with the zones of spec overwritten."""
+    p = np.array(p, dtype=np.float64, copy=True)
+    zones = []
+    for kind, i, j, k, val in spec:
+        if kind.endswith("3"):
+            zones += [(kind[:-1], i + a, j + b, k + c, val) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)]
+        else:
+            zones.append((kind, i, j, k, val))
+    for kind, i, j, k, val in zones:
+        for idx, n in zip((i, j, k), p.shape[:3]):
+            assert n == 7 or 3 <= idx < n - 3, "overwritten zones must be interior"
+        z = p[i, j, k]
+        if kind == "rho":
+            z[0] = val
+        elif kind == "u":
+            z[1] = val
+        elif kind == "ut":
+            v = z[2:5].copy()
+            n = np.sqrt((v * v).sum())
+            z[2:5] = (v / n if n > 0 else np.array([1.0, 0.0, 0.0])) * val
+        else:
+            raise ValueError(kind)
+    return p
+
+
+def record_branch(R, out, tag):
+    """The reference's branch counters since the last reset, flattened:
+    {tag}_branch[6][2] (calls, returned 0) and {tag}_codes[6][32][2]."""
+    b = R.branch()
+    calls = np.zeros((6, 2), np.int64)
+    codes = np.zeros((6, 32, 2), np.int64)
+    for s, name in enumerate(R.BRANCH_NAMES):
+        n, ok, hist = b[name]
+        calls[s] = (n, ok)
+        for c, (code, count) in enumerate(sorted(hist.items())):
+            codes[s, c] = (code, count)
+    out[f"{tag}_branch"] = calls
+    out[f"{tag}_codes"] = codes
+
+
+def run_failures(R, out, p0):
+    R.set_prims(p0)
+    R.reset_nfail()
+    out["p0"] = R.prims()
+    done = 0
+    for s in FAIL_STEPS:
+        R.reset_branch()
+        R.step(s - done)
+        if done == 0:
+            record_branch(R, out, "s1")
+        done = s
+        record(R, out, f"s{s}")
+
+
+def make_c0f():
+    from refwrap import RefRun
+    G = dict(np.load(os.path.join(HERE, "c0_balsara1024.npz"), allow_pickle=False))
+    R = RefRun("c0", 1024, 1, 1)
+    R.set_time(0.0, 1.0e-5, 0.4)
+    R.set_end_time(0.4)
+    out = {"n": np.array([1024, 1, 1]), "cour": np.float64(0.4), "dt0": np.float64(1.0e-5)}
+    run_failures(R, out, corrupt(G["p0"], FAILURES_C0))
+    np.savez_compressed(os.path.join(HERE, "c0f_balsara1024_failures.npz"), **out)
+
+
+def make_c1f():
+    from refwrap import RefRun, torus_params
+    from coolharm3d_amd.problems import torus_grid
+    G = dict(np.load(os.path.join(HERE, "c1_torus24x24x16.npz"), allow_pickle=False))
+    n = tuple(int(x) for x in G["n"])
+    g = torus_grid(*n)
+    R = RefRun("c1", *n, params=torus_params(g))
+    R.set_time(0.0, 1.0e-5, g.cour)
+    out = {"n": np.array(n), "cour": np.float64(g.cour), "dt0": np.float64(1.0e-5), "r_isco": G["r_isco"]}
+    run_failures(R, out, corrupt(G["p0"], FAILURES_C1))
+    np.savez_compressed(os.path.join(HERE, "c1f_torus24x24x16_failures.npz"), **out)
 
 
 def make_c0():
@@ -83,8 +191,8 @@ def make_c1():
 
 if __name__ == "__main__":
     if len(sys.argv) > 1:
-        {"c0": make_c0, "c1": make_c1}[sys.argv[1]]()
+        {"c0": make_c0, "c1": make_c1, "c0f": make_c0f, "c1f": make_c1f}[sys.argv[1]]()
     else:
-        for cfg in ("c0", "c1"):
+        for cfg in ("c0", "c1", "c0f", "c1f"):
             subprocess.run([sys.executable, __file__, cfg], check=True, stdout=subprocess.DEVNULL)
             print("wrote", cfg)

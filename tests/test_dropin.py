@@ -54,11 +54,14 @@ def _built(cfg):
     return os.path.exists(os.path.join(ROOT, "oracle", "_ref", cfg, "libref.so"))
 
 
-def _run(cfg, tmp_path):
+def _run(cfg, tmp_path, stages=False):
+    """The reference driver with the drop-in; stages: step_ch() through the
+    per-stage hooks a decomposed run uses (HGPU_STAGES=1)."""
     path = str(tmp_path / f"{cfg}.npz")
     code = SCRIPT.format(root=ROOT, oracle=os.path.join(ROOT, "oracle"), cfg=cfg, path=path)
-    print(f"[drop-in] running the reference driver with {cfg}", flush=True)
-    subprocess.run([sys.executable, "-c", code], check=True, stdout=subprocess.DEVNULL, timeout=600)
+    env = dict(os.environ, HGPU_STAGES="1" if stages else "0")
+    print(f"[drop-in] running the reference driver with {cfg} (stages {stages})", flush=True)
+    subprocess.run([sys.executable, "-c", code], check=True, stdout=subprocess.DEVNULL, timeout=600, env=env)
     return dict(np.load(path))
 
 
@@ -88,6 +91,14 @@ def test_dropin_host_balsara_bit_identical(tmp_path):
     _check_c0(_run("dropin_c0_host", tmp_path))
 
 
+@pytest.mark.skipif(not _built("dropin_c0_host"), reason="drop-in build not present (make -C oracle dropin)")
+def test_dropin_host_balsara_staged_bit_identical(tmp_path):
+    """The per-stage path of the shim (hgpu_advance, boundary flags through
+    mpi_global_imax2(), hgpu_fixup_stage, hgpu_end_step after
+    mpi_global_vmax(), mpi_global_min()) in a single-rank run."""
+    _check_c0(_run("dropin_c0_host", tmp_path, stages=True))
+
+
 @pytest.mark.slow
 @pytest.mark.skipif(not _built("dropin_c1_host"), reason="drop-in build not present (make -C oracle dropin)")
 def test_dropin_host_torus(tmp_path):
@@ -104,3 +115,9 @@ def test_dropin_gpu_balsara_bit_identical(tmp_path):
 @pytest.mark.skipif(not _built("dropin_c1_gpu"), reason="drop-in build not present (make -C oracle dropin)")
 def test_dropin_gpu_torus(tmp_path):
     _check_c1(_run("dropin_c1_gpu", tmp_path), 1e-9)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not _built("dropin_c1_gpu"), reason="drop-in build not present (make -C oracle dropin)")
+def test_dropin_gpu_torus_staged(tmp_path):
+    _check_c1(_run("dropin_c1_gpu", tmp_path, stages=True), 1e-9)

@@ -77,3 +77,19 @@ def test_gpu_equals_host_build_of_the_same_kernels():
     assert e.max() <= 1e-13, e
     assert np.array_equal(Hd.pflag(), Hh.pflag())
     assert np.array_equal(Hd.nfail(), Hh.nfail())
+
+
+@pytest.mark.parametrize("which", ["c0f", "c1f"])
+def test_gpu_matches_reference_on_failures(which):
+    """Every branch of the recovery tree and both interpolating fixups on
+    step one (tests/test_failure_fixtures.py shows the reference's branch
+    counters on these inputs): state per zone, pflag and nfail after 1, 2
+    and 10 steps."""
+    from coolharm3d_amd import Harm3D
+    from helpers import failure_grid, check_failure_run
+    grid, kw, G = failure_grid(which)
+    print(f"[gpu parity] failure fixture {which}", flush=True)
+    H = Harm3D(grid, **kw)
+    H.set_prims(G["p0"])
+    H.set_time(0.0, float(G["dt0"]))
+    check_failure_run(H, G, (1, 2, 10), H.step_ch, grid.n1 + 6)
