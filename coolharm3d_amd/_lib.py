@@ -35,7 +35,7 @@ class HgpuConfig(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("bbh_m1", ctypes.c_double), ("bbh_m2", ctypes.c_double), ("bbh_sep", ctypes.c_double),
         ("bbh_phi0", ctypes.c_double), ("bbh_inspiral", ctypes.c_int),
-        ("bc_nbr", ctypes.c_int), ("gpos", ctypes.c_int * 3),
+        ("bc_nbr", ctypes.c_int), ("gpos", ctypes.c_int * 3), ("no_confirm_iters", ctypes.c_int),
     ]
 
 

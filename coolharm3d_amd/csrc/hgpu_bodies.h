@@ -722,8 +722,7 @@ HD bool recover_fast(const Params& P, const Ptrs& X, int i, int j, int k, double
 #pragma unroll
   for (int l = 0; l < NP; l++) pt[l] = pf[l];
   double gamma = 1., bsq = 0.;
-  int ret = utoprim_2d_fast(U, g, P.gam, P.gam_m1_o_gam, pt, &gamma, &bsq, nullptr,
-                            P.mixed ? MIXED_EXTRA_NEWT_ITER : EXTRA_NEWT_ITER);
+  int ret = utoprim_2d_fast(U, g, P.gam, P.gam_m1_o_gam, pt, &gamma, &bsq, nullptr, P.n_extra_newt);
   if (ret || check_gamma(gamma) == 2) return false;
   if ((pt[UU] < BETA_MIN * bsq) || (pt[RHO] < BETA_MIN * bsq)) return false;
 #pragma unroll

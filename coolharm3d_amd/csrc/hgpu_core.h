@@ -95,6 +95,7 @@ inline int make_params(const hgpu_config& c, Params& P) {
   P.rescale_r = c.rescale_r && !P.degen[0];
   P.bc = c.bc;
   P.bc_nbr = c.bc_nbr;
+  P.n_extra_newt = c.no_confirm_iters ? 0 : EXTRA_NEWT_ITER;
   P.floor_dim = (c.coords == 1) ? 1 : 3;
   P.n_spatial_dims = (c.n1 > 1) + (c.n2 > 1) + (c.n3 > 1);
   P.track_nfail = c.track_nfail;

@@ -1043,14 +1043,20 @@ struct GpuBE {
     const size_t st_bytes = sizeof(unsigned long long) * ST_WORDS;
     const size_t nf_bytes = c->P.track_nfail ? sizeof(int) * N_NFAIL * NT : 0;
     const size_t pf_bytes = sizeof(int) * NT;
+    // the update candidates run the predictor, which writes ph and p_gamma:
+    // both are saved and restored with the flags and counters
+    const size_t gm_bytes = sizeof(double) * NT, ph_bytes = sizeof(double) * NP * NT;
+    const size_t o_nf = st_bytes, o_pf = o_nf + nf_bytes, o_gm = o_pf + pf_bytes, o_ph = o_gm + gm_bytes;
     void* save = nullptr;
-    rc |= alloc(c, &save, st_bytes + nf_bytes + pf_bytes);
+    rc |= alloc(c, &save, o_ph + ph_bytes);
     if (rc) return rc;
     char* sv = static_cast<char*>(save);
     hipStream_t st = g->stream;
     rc |= hip_check(hipMemcpyAsync(sv, c->X.status, st_bytes, hipMemcpyDeviceToDevice, st), "save");
-    if (nf_bytes) rc |= hip_check(hipMemcpyAsync(sv + st_bytes, c->X.nfail, nf_bytes, hipMemcpyDeviceToDevice, st), "save");
-    rc |= hip_check(hipMemcpyAsync(sv + st_bytes + nf_bytes, c->X.pflag, pf_bytes, hipMemcpyDeviceToDevice, st), "save");
+    if (nf_bytes) rc |= hip_check(hipMemcpyAsync(sv + o_nf, c->X.nfail, nf_bytes, hipMemcpyDeviceToDevice, st), "save");
+    rc |= hip_check(hipMemcpyAsync(sv + o_pf, c->X.pflag, pf_bytes, hipMemcpyDeviceToDevice, st), "save");
+    rc |= hip_check(hipMemcpyAsync(sv + o_gm, c->X.gamma, gm_bytes, hipMemcpyDeviceToDevice, st), "save");
+    rc |= hip_check(hipMemcpyAsync(sv + o_ph, c->X.ph, ph_bytes, hipMemcpyDeviceToDevice, st), "save");
     const bool prof = g->prof;
     g->prof = false;
     hipEvent_t e0 = ev(g), e1 = ev(g);
@@ -1121,8 +1127,10 @@ struct GpuBE {
     g->pool.push_back(e1);
     g->prof = prof;
     rc |= hip_check(hipMemcpyAsync(c->X.status, sv, st_bytes, hipMemcpyDeviceToDevice, st), "restore");
-    if (nf_bytes) rc |= hip_check(hipMemcpyAsync(c->X.nfail, sv + st_bytes, nf_bytes, hipMemcpyDeviceToDevice, st), "restore");
-    rc |= hip_check(hipMemcpyAsync(c->X.pflag, sv + st_bytes + nf_bytes, pf_bytes, hipMemcpyDeviceToDevice, st), "restore");
+    if (nf_bytes) rc |= hip_check(hipMemcpyAsync(c->X.nfail, sv + o_nf, nf_bytes, hipMemcpyDeviceToDevice, st), "restore");
+    rc |= hip_check(hipMemcpyAsync(c->X.pflag, sv + o_pf, pf_bytes, hipMemcpyDeviceToDevice, st), "restore");
+    rc |= hip_check(hipMemcpyAsync(c->X.gamma, sv + o_gm, gm_bytes, hipMemcpyDeviceToDevice, st), "restore");
+    rc |= hip_check(hipMemcpyAsync(c->X.ph, sv + o_ph, ph_bytes, hipMemcpyDeviceToDevice, st), "restore");
     rc |= sync(c);
     release(c, save);
     return rc;

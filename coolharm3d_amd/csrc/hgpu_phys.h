@@ -20,11 +20,6 @@ constexpr int MAX_NEWT_ITER = 30;         // src/u2p_defs.h:76-79
 constexpr double NEWT_TOL = 1.0e-10;
 constexpr double MIN_NEWT_TOL = 1.0e-10;
 constexpr int EXTRA_NEWT_ITER = 2;
-// Extra iterations after convergence in the mixed-precision step's inversion:
-// the test |dW/W| <= NEWT_TOL is on the step just taken, so with quadratic
-// convergence the accepted iterate is already at round-off and the reference's
-// two confirming iterations change nothing at the mixed step's precision.
-constexpr int MIXED_EXTRA_NEWT_ITER = 0;
 constexpr double NEWT_TOL2 = 1.0e-15;
 constexpr double W_TOO_BIG = 1.e20;
 constexpr double UTSQ_TOO_BIG = 1.e20;

@@ -94,6 +94,7 @@ struct Params {
   int rescale_r;           // RESCALE_R
   int bc;                  // BC_CARTESIAN_OUTFLOW or BC_SPHERICAL_OUTFLOW3
   int bc_nbr;              // faces bordering another block: bit 2*dir + side (dir 0..2)
+  int n_extra_newt;        // confirming Newton iterations of the update's inversion (2, or 0 on request)
   int floor_dim;           // 1: floors per x1 index, 3: per cell
   int degen[3];            // N_d == 1
   int n_spatial_dims;

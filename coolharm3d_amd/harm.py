@@ -76,6 +76,7 @@ def config_for(grid: GridSpec, **kw) -> _lib.HgpuConfig:
     c.rhopower, c.uupower = kw.get("rhopower", -1.5), kw.get("uupower", -2.5)
     c.precision = {"fp64": 0, "mixed": 1}[kw.get("precision", "fp64")]
     c.bc_nbr = kw.get("bc_nbr", 0)
+    c.no_confirm_iters = 1 if kw.get("no_confirm_iters", False) else 0
     for d, g in enumerate(kw.get("gpos", (0, 0, 0))):
         c.gpos[d] = g
     return c

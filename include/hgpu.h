@@ -67,6 +67,11 @@ typedef struct hgpu_config {
    * from the global index (src/coord.c:182-228); startx[] stays the global grid's.             */
   int bc_nbr;
   int gpos[3];
+  /* 1: the conserved update's inversion stops at convergence, without the two confirming
+   * Newton iterations Utoprim_2d_fast() runs after it (EXTRA_NEWT_ITER, src/u2p_defs.h:79);
+   * zones the update hands to the full recovery tree keep them.  0 (default): the reference's
+   * iteration count.  No counterpart in the reference (an algorithm change, in either precision). */
+  int no_confirm_iters;
 } hgpu_config;
 
 typedef struct hgpu_ctx hgpu_ctx;
@@ -129,8 +134,10 @@ int hgpu_set_graph(hgpu_ctx *ctx, int on);
  * hgpu_autotune() times every candidate of families 0-4 on the current state
  * (reps launches each; family 4 as the whole predictor update, split or fused)
  * and keeps the fastest; report (NULL or [5][4]) gets
- * bs, kc, flags, ms per launch.  The state, flags and counters are unchanged;
- * results do not depend on the shapes.  No counterpart in the reference. */
+ * bs, kc, flags, ms per launch.  The state (p, ph), p_gamma, the flags and
+ * the counters are unchanged; the face fluxes hgpu_download_fluxes() returns
+ * are then those of the tuning pass.  Results do not depend on the shapes.
+ * No counterpart in the reference. */
 int hgpu_autotune(hgpu_ctx *ctx, int reps, double *report);
 int hgpu_set_launch_shape(hgpu_ctx *ctx, int family, int bs, int kc, int flags);
 int hgpu_get_launch_shape(hgpu_ctx *ctx, int family, int *bs, int *kc, int *flags);

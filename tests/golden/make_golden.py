@@ -11,6 +11,10 @@ the reference's outputs on the configurations the parity tests use:
 * c1_torus24x24x16.npz -- Fishbone-Moncrief torus, Kerr a = 0.9 in MKS
   coordinates, PPM, HLL, PPM EMFs, h/r cooling, floors 1e-7 r^-1.5 and
   1e-9 r^-2.5 (coolharm3d_amd.problems.torus, seed 1234); after 1, 10, 100 steps.
+* c0_balsara1024_t04.npz -- the shock tube run as the reference's driver runs
+  it, step_ch() while t < GridLength[0] = 0.4 (src/main.c:62-81,
+  src/init.balsara1.c:56): the number of steps, the state, pflag, nfail, t
+  and dt at the end.
 * c0f_balsara1024_failures.npz, c1f_torus24x24x16_failures.npz -- the same two
   set-ups with a few zones of the initial state overwritten (FAILURES_C0,
   FAILURES_C1: negative densities and internal energies, Lorentz factors of
@@ -154,6 +158,23 @@ def make_c1f():
     np.savez_compressed(os.path.join(HERE, "c1f_torus24x24x16_failures.npz"), **out)
 
 
+def make_c0_full():
+    from refwrap import RefRun
+    G = dict(np.load(os.path.join(HERE, "c0_balsara1024.npz"), allow_pickle=False))
+    R = RefRun("c0", 1024, 1, 1)
+    R.set_time(0.0, 1.0e-5, 0.4)
+    R.set_prims(G["p0"])
+    R.reset_nfail()
+    nstep = 0
+    while R.scalars()["t"] < 0.4:
+        R.step(1)
+        nstep += 1
+    out = {"p0": G["p0"], "cour": np.float64(0.4), "dt0": np.float64(1.0e-5), "t_end": np.float64(0.4),
+           "nstep": np.int64(nstep)}
+    record(R, out, "end")
+    np.savez_compressed(os.path.join(HERE, "c0_balsara1024_t04.npz"), **out)
+
+
 def make_c0():
     from refwrap import RefRun
     R = RefRun("c0", 1024, 1, 1)
@@ -191,8 +212,8 @@ def make_c1():
 
 if __name__ == "__main__":
     if len(sys.argv) > 1:
-        {"c0": make_c0, "c1": make_c1, "c0f": make_c0f, "c1f": make_c1f}[sys.argv[1]]()
+        {"c0": make_c0, "c1": make_c1, "c0f": make_c0f, "c1f": make_c1f, "c0t": make_c0_full}[sys.argv[1]]()
     else:
-        for cfg in ("c0", "c1", "c0f", "c1f"):
+        for cfg in ("c0", "c1", "c0f", "c1f", "c0t"):
             subprocess.run([sys.executable, __file__, cfg], check=True, stdout=subprocess.DEVNULL)
             print("wrote", cfg)

@@ -88,3 +88,37 @@ def test_zone_check_catches_what_the_l1_sum_hides():
     assert rel_l1(ref, got).max() <= 1e-12
     bad = zone_errors(ref, got, 1e-9, 1e-13)
     assert bad.tolist() == [[np.ravel_multi_index((i, j, k), ref.shape[:3]), 1]]
+
+
+def test_oracle_balsara_to_t04_bit_identical():
+    from pyoracle import OracleRun
+    from coolharm3d_amd import balsara1_grid
+    G = golden("c0_balsara1024_t04.npz")
+    O = OracleRun(balsara1_grid(1024, cour=float(G["cour"])))
+    O.set_prims(G["p0"])
+    O.set_time(0.0, float(G["dt0"]))
+    n = 0
+    while O.time()["t"] < float(G["t_end"]):
+        O.step(1)
+        n += 1
+    assert n == int(G["nstep"])
+    assert np.array_equal(interior(O.prims()), G["end_prims"])
+    assert np.array_equal(O.nfail(), G["end_nfail"])
+    assert O.time()["t"] == float(G["end_t"])
+
+
+@pytest.mark.skipif(not os.path.exists(HOSTLIB), reason="host check build missing")
+def test_host_build_balsara_to_t04_bit_identical():
+    from coolharm3d_amd import Harm3D, balsara1_grid
+    G = golden("c0_balsara1024_t04.npz")
+    H = Harm3D(balsara1_grid(1024, cour=float(G["cour"])), hostcheck=True)
+    H.set_prims(G["p0"])
+    H.set_time(0.0, float(G["dt0"]))
+    n = 0
+    while H.time()[0] < float(G["t_end"]):
+        H.step_ch(1)
+        n += 1
+    assert n == int(G["nstep"])
+    assert np.array_equal(interior(H.prims()), G["end_prims"])
+    assert np.array_equal(H.nfail(), G["end_nfail"])
+    assert H.time()[0] == float(G["end_t"])

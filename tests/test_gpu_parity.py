@@ -93,3 +93,24 @@ def test_gpu_matches_reference_on_failures(which):
     H.set_prims(G["p0"])
     H.set_time(0.0, float(G["dt0"]))
     check_failure_run(H, G, (1, 2, 10), H.step_ch, grid.n1 + 6)
+
+
+def test_gpu_balsara_to_t04_matches_reference():
+    """The whole configuration, t = 0 to 0.4 (1716 steps), bit for bit; the
+    second half replayed from the captured step graph."""
+    from coolharm3d_amd import Harm3D, balsara1_grid
+    G = golden("c0_balsara1024_t04.npz")
+    H = Harm3D(balsara1_grid(1024, cour=float(G["cour"])))
+    H.set_prims(G["p0"])
+    H.set_time(0.0, float(G["dt0"]))
+    n = 0
+    while H.time()[0] < float(G["t_end"]):
+        if n == 800:
+            H.set_graph(True)
+        H.step_ch(1)
+        n += 1
+    assert n == int(G["nstep"])
+    assert np.array_equal(interior(H.prims()), G["end_prims"]), \
+        f"max |diff| {np.abs(interior(H.prims()) - G['end_prims']).max():.3e}"
+    assert np.array_equal(H.nfail(), G["end_nfail"])
+    assert H.time()[0] == float(G["end_t"])

@@ -35,30 +35,6 @@ def test_torus_full_size_invariants(shape):
     assert np.array_equal(nf, H.nfailtot())
 
 
-def test_balsara_to_final_time_shock_structure():
-    """Balsara test 1 to t = 0.4 (the 1D Riemann workload): the run reaches the end
-    time exactly (timestep clamp, src/step_ch.c:615) and keeps the known
-    shock-tube bounds (1 >= rho >= 0.125 outside the compound wave region is
-    not asserted; the solution stays within [0.1, 1.1] and B1 stays 0.5)."""
-    from coolharm3d_amd import Harm3D, balsara1_grid, balsara1
-    g = balsara1_grid(1024)
-    H = Harm3D(g)
-    H.set_prims(balsara1(g))
-    H.fixup_bounds()
-    H.set_time(0.0, 1.0e-5)
-    for _ in range(4000):
-        H.step_ch(10)
-        t, dt, _ = H.time()
-        if t >= 0.4 - 1e-15 or dt <= 0:
-            break
-    t, dt, _ = H.time()
-    assert t == pytest.approx(0.4, abs=1e-12)
-    p = H.prims()[3:-3, 3, 3]
-    assert np.isfinite(p).all()
-    assert 0.1 < p[:, 0].min() and p[:, 0].max() < 1.1
-    assert np.allclose(p[:, 5], 0.5, atol=1e-10)
-
-
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_graph_replay_matches_eager(precision):
     """Steps replayed from one captured HIP graph equal eager steps bit for bit,
@@ -151,11 +127,11 @@ def test_launch_shapes_and_autotune_keep_results(precision):
                 assert H.launch_shape(fam) == (bs, kc, flags)
         if mode == "autotune":
             fe0, v0 = H.status()
-            before = (H.prims(), H.pflag(), H.nfail(), H.nfailtot(), v0)
+            before = (H.prims(), H.prims(1), H.gamma(), H.pflag(), H.nfail(), H.nfailtot(), v0)
             rep = H.autotune(reps=1)
             print(f"[shapes] autotune picked {rep}", flush=True)
             fe1, v1 = H.status()
-            after = (H.prims(), H.pflag(), H.nfail(), H.nfailtot(), v1)
+            after = (H.prims(), H.prims(1), H.gamma(), H.pflag(), H.nfail(), H.nfailtot(), v1)
             assert fe0 == fe1
             for x, y in zip(before, after):
                 assert np.array_equal(x, y)
