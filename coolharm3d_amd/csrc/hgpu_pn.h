@@ -7,8 +7,9 @@
 //
 // * Metric: the 1PN near-zone metric of two point masses in harmonic
 //   coordinates (Blanchet, Faye & Ponsot 1998, Phys. Rev. D 58, 124002), G = c = 1:
-//     g00 = -1 + 2V - 2V^2,  g0i = -4 V_i,  gij = delta_ij (1 + 2V),
-//   with V the 1PN retarded potential of the two bodies,
+//     g00 = -1 + 2V - 2U^2,  g0i = -4 V_i,  gij = delta_ij (1 + 2U),
+//   with U = sum_a m_a/r_a the Newtonian potential (U^2 and V^2 differ beyond
+//   1PN order) and V the 1PN retarded potential of the two bodies,
 //     V = sum_a m_a/r_a [1 + 2 v_a^2 - (n_a.v_a)^2/2] - m_a m_b/(r_a r_ab)
 //         + m_a m_b (r_b^2 - r_a^2 - r_ab^2)/(4 r_a r_ab^3)     (b != a),
 //   and V_i = sum_a m_a v_a^i / r_a  (y_b at distance r_b from the field point).

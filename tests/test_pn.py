@@ -284,3 +284,161 @@ def test_gpu_bbh_graph_replay():
     G, _ = _bbh_run(False, "mixed", steps=4, graph=True)
     assert np.array_equal(E.prims(), G.prims())
     assert E.time() == G.time()
+
+
+# ---------------------------------------------------------------------------
+# The binary's own terms, pinned by routes independent of the closed form the
+# library evaluates.
+
+def _orbit(t, m1, m2, b, phi0):
+    """Hole positions of the fixed circular orbit (Newtonian Kepler frequency,
+    centre of mass at the origin)."""
+    M = m1 + m2
+    ph = phi0 + math.sqrt(M / b ** 3) * t
+    n = np.array([math.cos(ph), math.sin(ph), 0.0])
+    return (m2 / M) * b * n, -(m1 / M) * b * n
+
+
+def _numerical(x):
+    """(X1, X2, X3) of Cartesian x for R0 = 0, th_beg = 0, th_length = pi,
+    h_slope = 0: X1 = log r, X2 = theta / pi, X3 = phi."""
+    r = float(np.linalg.norm(x))
+    return (math.log(r), math.acos(x[2] / r) / math.pi, math.atan2(x[1], x[0]) % (2 * math.pi))
+
+
+def _UV_library(g, r, sth):
+    """(U, V) of metric records g[..., 42]: g_phiphi = (1 + 2U) (r sin theta)^2
+    gives the Newtonian U, g_tt = -1 + 2V - 2U^2 then V (the time coordinate is
+    the harmonic one, so g_tt is the same in numerical coordinates)."""
+    U = 0.5 * (g[..., 31] / (r * sth) ** 2 - 1.0)
+    return U, 0.5 * (g[..., 16] + 1.0) + U * U
+
+
+def _V_library(x, t, **kw):
+    """V of the library's metric at Cartesian x."""
+    g = geom(_numerical(x), t, h_slope=0.0, **kw)
+    r = float(np.linalg.norm(x))
+    return float(_UV_library(g, r, math.sqrt(x[0] ** 2 + x[1] ** 2) / r)[1])
+
+
+def _V_retarded(x, t, m1, m2, b, phi0):
+    """The 1PN potential of two point masses from its definition as a retarded
+    integral expanded in 1/c (Blanchet & Damour 1989; Blanchet, Faye & Ponsot
+    1998): V = sum_a sigma_a / r_a + (1/2) d^2/dt^2 sum_a m_a r_a, with the
+    point-mass source sigma_a = m_a (1 + 3/2 v_a^2 - m_b / r_ab).  The second
+    time derivative is taken numerically along the orbit, so this route shares
+    no algebra with the expanded closed form."""
+    m = (m1, m2)
+    h = 0.5
+
+    def mr(s):
+        y = _orbit(s, m1, m2, b, phi0)
+        return sum(m[a] * np.linalg.norm(x - y[a]) for a in range(2))
+
+    d2 = (-mr(t - 2 * h) + 16 * mr(t - h) - 30 * mr(t) + 16 * mr(t + h) - mr(t + 2 * h)) / (12 * h * h)
+    y = _orbit(t, m1, m2, b, phi0)
+    ya, yb = _orbit(t - 1e-3, m1, m2, b, phi0), _orbit(t + 1e-3, m1, m2, b, phi0)
+    V = 0.5 * d2
+    for a in range(2):
+        v2 = float(np.sum(((yb[a] - ya[a]) / 2e-3) ** 2))
+        V += m[a] * (1.0 + 1.5 * v2 - m[1 - a] / b) / np.linalg.norm(x - y[a])
+    return V
+
+
+BIN = dict(m1=0.3, m2=0.7, sep=20.0, phi0=0.4)
+
+
+def _pts(t):
+    y1, y2 = _orbit(t, BIN["m1"], BIN["m2"], BIN["sep"], BIN["phi0"])
+    return [y1 + np.array([3.0, 1.0, 0.5]), y1 + np.array([-2.0, 0.0, 2.5]), y2 + np.array([0.5, -4.0, -1.0]),
+            y2 + np.array([0.0, 2.0, 6.0]), 0.5 * (y1 + y2) + np.array([0.0, 0.0, 3.0]),
+            np.array([35.0, -12.0, 20.0]), np.array([-80.0, 50.0, -30.0])]
+
+
+def test_potential_equals_retarded_integral_form():
+    """Every coefficient of V -- the velocity terms and both interaction
+    terms, including the one from the holes' accelerations -- against the
+    retarded-integral definition, for unequal masses, near each hole, between
+    them and farther out: agreement to 1e-8 relative, the finite-difference
+    error of the time derivative near a hole (a 1 % change of the m1 m2
+    acceleration term alone moves V by ~1e-5 relative there)."""
+    kw = dict(m1=BIN["m1"], m2=BIN["m2"], sep=BIN["sep"], phi0=BIN["phi0"], inspiral=False)
+    for t in (0.0, 77.0):
+        for x in _pts(t):
+            vl = _V_library(x, t, **kw)
+            vr = _V_retarded(x, t, BIN["m1"], BIN["m2"], BIN["sep"], BIN["phi0"])
+            assert abs(vl - vr) <= 1e-8 * abs(vr), (t, x, vl, vr)
+
+
+def test_potential_check_catches_a_wrong_interaction_coefficient():
+    """The check above fails for a plausible mistake: the acceleration
+    interaction term m_a m_b (r_b^2 - r_a^2 - r_ab^2)/(4 r_a r_ab^3) with the
+    wrong sign -- it is the part of V that the retarded form gets from the
+    second time derivative, so dropping that derivative must show up."""
+    t = 0.0
+    m1, m2, b, phi0 = BIN["m1"], BIN["m2"], BIN["sep"], BIN["phi0"]
+    y = _orbit(t, m1, m2, b, phi0)
+    for x in _pts(t)[:4]:
+        vr = _V_retarded(x, t, m1, m2, b, phi0)
+        r = [np.linalg.norm(x - y[a]) for a in range(2)]
+        m = (m1, m2)
+        acc = sum(m[a] * m[1 - a] * (r[1 - a] ** 2 - r[a] ** 2 - b * b) / (4 * r[a] * b ** 3) for a in range(2))
+        assert abs(acc) > 1e-6
+        vl = _V_library(x, t, m1=m1, m2=m2, sep=b, phi0=phi0, inspiral=False)
+        assert abs((vl - 2 * acc) - vr) > 1e3 * 1e-8 * abs(vr)
+
+
+def test_far_field_monopole_is_the_binding_energy():
+    """The sphere average of V at radius R >> b is M_1PN / R with M_1PN the
+    binary's total mass-energy, M + (1/2) mu v^2 - m1 m2 / b = M - m1 m2/(2b)
+    on a circular orbit (up to (b/R)^2 corrections of the non-harmonic 1PN
+    terms).  This fixes the remaining coefficient, the -m_a m_b/(r_a r_ab)
+    term, which no vacuum field equation sees."""
+    m1, m2, b = BIN["m1"], BIN["m2"], BIN["sep"]
+    R = 4000.0
+    n2, n3 = 96, 96
+    c = bbh_block(1, n2, n3, ghosts=0, m1=m1, m2=m2, sep=b, phi0=BIN["phi0"], inspiral=False, h_slope=0.0)
+    c.startx[1] = math.log(R) - 0.5 * c.dx[1]
+    g = PnMetric(c, hostcheck=True).geometry(0.0, CENT).reshape(n2, n3, 42)
+    th = math.pi * (np.arange(n2) + 0.5) / n2
+    V = _UV_library(g, R, np.sin(th)[:, None])[1]
+    w = np.sin(th)[:, None] * np.ones((1, n3))
+    mono = R * float((V * w).sum() / w.sum())
+    expect = m1 + m2 - m1 * m2 / (2 * b)
+    assert abs(mono - expect) <= 2e-6, (mono, expect)
+    # the binding term itself is ~5e-3: a 10 % change of the -m_a m_b/(r_a r_ab)
+    # coefficient (2 m1 m2/b x 0.1 = 2e-3) is far outside the tolerance
+    assert m1 * m2 / (2 * b) > 1e3 * 2e-6
+
+
+def test_harmonic_gauge_condition():
+    """The metric is in harmonic gauge to the order it is built: the time
+    coordinate obeys box t = d_mu(sqrt(-g) g^{mu t}) / sqrt(-g) = 0 up to
+    relative remainders of order U and v^2 (the terms the 1PN metric leaves
+    out).  At leading order this is dV/dt + d_i V_i = 0, so it fixes the shift
+    g_ti = -4 V_i against the time dependence of V: with -3 V_i in place of
+    -4 V_i the residual would be a quarter of d_t(sqrt(-g) g^tt), against a
+    bar of 1.5 U + 0.01 (< 0.09 at these points).  Finite differences in t and the numerical coordinates (in which
+    sqrt(-g) carries the coordinate Jacobian, so the divergence is the same
+    invariant)."""
+    kw = dict(m1=0.4, m2=0.6, sep=100.0, phi0=0.2, inspiral=False, h_slope=0.0)
+    y1, y2 = _orbit(30.0, 0.4, 0.6, 100.0, 0.2)
+    for x in (y1 + np.array([12.0, 5.0, 3.0]), y2 + np.array([-8.0, 10.0, -6.0]), np.array([20.0, 130.0, 40.0])):
+        X = _numerical(x)
+        h = [1.0, 1e-4, 1e-4, 1e-4]
+
+        def A(dt=0.0, d=None, s=0.0):
+            Xs = list(X)
+            if d is not None:
+                Xs[d] += s
+            g = geom(tuple(Xs), 30.0 + dt, **kw)
+            return g[32] * gcon(g)[:, 0]          # sqrt(-g) g^{mu t}
+
+        dA0 = (A(h[0])[0] - A(-h[0])[0]) / (2 * h[0])
+        terms = [dA0] + [(A(0.0, i, h[i + 1])[i + 1] - A(0.0, i, -h[i + 1])[i + 1]) / (2 * h[i + 1])
+                         for i in range(3)]
+        res = sum(terms)
+        scale = abs(dA0)
+        U = 0.4 / np.linalg.norm(x - y1) + 0.6 / np.linalg.norm(x - y2)
+        assert scale > 0 and 1.5 * U + 0.01 < 0.1
+        assert abs(res) <= (1.5 * U + 0.01) * scale, (terms, res)
