@@ -19,11 +19,19 @@
  * (geom_arr[0] and conn, so the device uses the reference's own metric and
  * connection) and the primitive state.  After every step it copies back the
  * clock, failure_exists and nfailtot (check_for_crash() reads them,
- * src/step_ch.c:1244) and, with HGPU_HOST_SYNC = 1 (default), the state and
- * maps.  With HGPU_HOST_SYNC = 0 the host arrays are refreshed only by
- * step_ch_hgpu_sync_host(), which the caller invokes before diag()/dump().
- * After the host edits p (restart, re-initialisation), step_ch_hgpu_upload()
- * sends it to the device again.
+ * src/step_ch.c:1244) and, depending on HGPU_HOST_SYNC (compile-time default,
+ * overridden by the environment variable of the same name), the state and
+ * maps p, p_gamma, pflag, nfail:
+ *   1  after every step;
+ *   2  only after the steps whose host arrays the driver reads next: the
+ *      every-10th-step diag(OUT_EVERY) energy dump, a diag(i) whose output
+ *      time T_out[i] is reached, restart_write() every n_restart steps, and
+ *      the end of the run (src/main.c:62-100, src/diag.c:49-145,
+ *      src/restart.c:22-75);
+ *   0  never -- the caller invokes step_ch_hgpu_sync_host() itself.
+ * After the host edits p (restart_init() reading a checkpoint on the first
+ * step is picked up by the set-up; later edits), step_ch_hgpu_upload() sends
+ * it to the device again.
  *
  * Domain-decomposed runs (USEMPI): each rank's context holds its block
  * (hgpu_config.gpos = globalpos, bc_nbr from bc_pid), and step_ch() runs each
@@ -52,6 +60,9 @@
 
 void step_ch_hgpu_sync_host(void);
 void step_ch_hgpu_upload(void);
+extern const int mask[];     /* outputs this build writes (src/diag.c:31) */
+
+static int hg_sync_mode = -1;          /* HGPU_HOST_SYNC, resolved on the first step */
 
 static hgpu_ctx *hg = NULL;
 static int hg_stages = 0;              /* run the stages through the halo hooks */
@@ -228,6 +239,20 @@ static void hg_exchange(double ****prim, int which) {
 #endif
 }
 
+/* Whether the driver reads the host state after this step: main() increments
+   nstep after step_ch() and then calls diag(OUT_EVERY) (dump_ener() every 10th
+   step), diag(i) for outputs due at t, restart_write() every n_restart steps,
+   and leaves the loop at GridLength[0] (src/main.c:62-100). */
+static int hg_host_due(void) {
+  int i;
+  const int n = nstep + 1;
+  if (n % 10 == 0) return 1;
+  if (n_restart > 0 && n % n_restart == 0) return 1;
+  if (t >= GridLength[0]) return 1;
+  OUT_LOOP if (mask[i] && t >= T_out[i]) return 1;
+  return 0;
+}
+
 /* one advance() stage with bounds and fixups (src/step_ch.c:92-121, :159-180) */
 static void hg_stage_run(int stage) {
   int fe, phys, mpi;
@@ -268,7 +293,7 @@ void step_ch(void) {
   failure_exists = fe;
   hg_check(hgpu_get_nfailtot(hg, nft), "hgpu_get_nfailtot");
   for (l = 0; l < N_NFAIL; l++) nfailtot[l] = hg_nfailtot_base[l] + (ulint)nft[l];
-#if( HGPU_HOST_SYNC )
-  step_ch_hgpu_sync_host();
-#endif
+  if (hg_sync_mode < 0)
+    hg_sync_mode = getenv("HGPU_HOST_SYNC") ? atoi(getenv("HGPU_HOST_SYNC")) : HGPU_HOST_SYNC;
+  if (hg_sync_mode == 1 || (hg_sync_mode == 2 && hg_host_due())) step_ch_hgpu_sync_host();
 }

@@ -339,3 +339,18 @@ void __real_fixup_interp_v(double ****pv);
 void __wrap_fixup_interp_v(double ****pv) { ref_tally(4, 0); __real_fixup_interp_v(pv); }
 void __real_fixup_interp_prim(double ****pv);
 void __wrap_fixup_interp_prim(double ****pv) { ref_tally(5, 0); __real_fixup_interp_prim(pv); }
+
+/* Checkpoints through the reference's own ASCII restart files (rdump0/1.<pid>
+   in the working directory; src/restart.c:22-75 write, :80-220 read): */
+extern void restart_write(void);
+extern int restart_init(void);
+void ref_restart_write(void) { restart_write(); }
+/* restart_init() instead of init() (src/main.c:49-59); -1 when no file exists */
+int ref_restart_init(void)
+{
+  if (!restart_init()) return -1;
+  global_init();
+  return 0;
+}
+void ref_set_n_restart(int n) { n_restart = n; }
+int ref_get_nstep(void) { return nstep; }

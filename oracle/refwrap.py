@@ -53,7 +53,7 @@ class RefRun:
     """One reference process state (load the .so under a private name so two
     configurations can coexist in one interpreter)."""
 
-    def __init__(self, cfg: str, n1: int, n2: int, n3: int, params=None):
+    def __init__(self, cfg: str, n1: int, n2: int, n3: int, params=None, restart=False):
         path = ref_lib_path(cfg)
         if not os.path.exists(path):
             raise FileNotFoundError(path)
@@ -68,7 +68,12 @@ class RefRun:
                 arr[i] = float(v)
         if L.ref_setup(n1, n2, n3) != 0:
             raise RuntimeError("reference already set up in this process")
-        L.ref_init()
+        if restart:
+            # the reference's restart_init() reads the checkpoint in the working directory
+            if L.ref_restart_init() != 0:
+                raise FileNotFoundError("no restart file in the working directory")
+        else:
+            L.ref_init()
         d = np.zeros(7, np.int32)
         L.ref_dims(_p(d))
         self.N = tuple(int(x) for x in d[:3])
@@ -147,6 +152,15 @@ class RefRun:
         a = np.zeros(11)
         self.lib.ref_get_coord(i, j, k, pos, _p(a))
         return a
+
+    def restart_write(self):
+        self.lib.ref_restart_write()
+
+    def set_n_restart(self, n):
+        self.lib.ref_set_n_restart(int(n))
+
+    def nstep(self):
+        return int(self.lib.ref_get_nstep())
 
     def fixup_bounds(self):
         self.lib.ref_fixup_bounds()

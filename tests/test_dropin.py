@@ -121,3 +121,88 @@ def test_dropin_gpu_torus(tmp_path):
 @pytest.mark.skipif(not _built("dropin_c1_gpu"), reason="drop-in build not present (make -C oracle dropin)")
 def test_dropin_gpu_torus_staged(tmp_path):
     _check_c1(_run("dropin_c1_gpu", tmp_path, stages=True), 1e-9)
+
+
+SYNC_SCRIPT = r"""
+import sys, numpy as np
+sys.path[:0] = [{root!r}, {oracle!r}]
+from refwrap import RefRun
+R = RefRun({cfg!r}, 1024, 1, 1)
+R.set_time(0.0, 1.0e-5, 0.4)
+R.set_n_restart(7)
+R.reset_nfail()
+out = {{"p0": R.prims()}}
+for s in range(1, 11):
+    R.step(1)
+    out[f"s{{s}}"] = R.prims()
+np.savez({path!r}, **out)
+"""
+
+
+@pytest.mark.skipif(not _built("dropin_c0_host"), reason="drop-in build not present (make -C oracle dropin)")
+def test_dropin_host_sync_when_due(tmp_path):
+    """HGPU_HOST_SYNC=2: the host arrays are refreshed exactly after the steps
+    whose state the driver reads next (restart_write() every n_restart = 7
+    steps, diag(OUT_EVERY)'s energy dump every 10th step), and are stale
+    otherwise."""
+    path = str(tmp_path / "sync.npz")
+    code = SYNC_SCRIPT.format(root=ROOT, oracle=os.path.join(ROOT, "oracle"), cfg="dropin_c0_host", path=path)
+    subprocess.run([sys.executable, "-c", code], check=True, stdout=subprocess.DEVNULL, timeout=600,
+                   env=dict(os.environ, HGPU_HOST_SYNC="2"), cwd=str(tmp_path))
+    R = dict(np.load(path))
+    G = golden("c0_balsara1024.npz")
+    assert np.array_equal(R["s6"], R["p0"])          # steps 1-6: nothing due, host untouched
+    assert not np.array_equal(R["s7"], R["p0"])      # step 7: restart due
+    assert np.array_equal(R["s9"], R["s7"])          # steps 8-9: stale again
+    assert np.array_equal(interior(R["s10"]), G["s10_prims"])   # step 10: energy dump due
+
+
+RESTART_SCRIPT = r"""
+import sys, numpy as np
+sys.path[:0] = [{root!r}, {oracle!r}]
+from refwrap import RefRun
+phase = {phase!r}
+if phase == "write":
+    R = RefRun({cfg!r}, 1024, 1, 1)
+    R.set_time(0.0, 1.0e-5, 0.4)
+    R.step(5)
+    R.restart_write()
+    R.step(5)
+else:
+    R = RefRun({cfg!r}, 1024, 1, 1, restart=True)
+    R.step(5)
+np.savez({path!r}, p=R.prims(), t=R.scalars()["t"], dt=R.scalars()["dt"], nstep=R.nstep())
+"""
+
+
+def _restart_pair(cfg, tmp_path):
+    d = tmp_path / cfg
+    d.mkdir()
+    res = {}
+    for phase in ("write", "read"):
+        path = str(d / f"{phase}.npz")
+        code = RESTART_SCRIPT.format(root=ROOT, oracle=os.path.join(ROOT, "oracle"), cfg=cfg, path=path, phase=phase)
+        subprocess.run([sys.executable, "-c", code], check=True, stdout=subprocess.DEVNULL, timeout=600,
+                       cwd=str(d), env=dict(os.environ, HGPU_HOST_SYNC="1"))
+        res[phase] = dict(np.load(path))
+    assert (d / "rdump0.0000").exists()
+    return res
+
+
+@pytest.mark.skipif(not (_built("dropin_c0_host") and _built("c0")), reason="reference/drop-in builds not present")
+def test_dropin_host_ascii_restart_round_trip(tmp_path):
+    """The reference's ASCII checkpoint (restart_write() / restart_init(),
+    src/restart.c) through the drop-in: 5 steps, checkpoint, 5 more steps; a
+    fresh process reads the checkpoint (restart_init() instead of init(), the
+    shim uploads what it read on its first step) and runs 5 steps.  The
+    restarted run equals the pure reference build doing the same round trip,
+    bit for bit, and continues where the uninterrupted run continues (to the
+    precision the checkpoint format keeps)."""
+    ref = _restart_pair("c0", tmp_path)
+    drop = _restart_pair("dropin_c0_host", tmp_path)
+    assert np.array_equal(drop["write"]["p"], ref["write"]["p"])
+    assert np.array_equal(drop["read"]["p"], ref["read"]["p"])
+    assert float(drop["read"]["t"]) == float(ref["read"]["t"]) and int(drop["read"]["nstep"]) == 10
+    assert float(drop["read"]["dt"]) == float(ref["read"]["dt"])
+    e = np.abs(drop["read"]["p"] - drop["write"]["p"]).max()
+    assert e <= 1e-12 * np.abs(drop["write"]["p"]).max()
