@@ -117,6 +117,34 @@ def cpu_worker(args):
     print(json.dumps({"zc": n1 * n2 * n3 * args.slab_steps, "s": el, "rss_mb": rss}), flush=True)
 
 
+CPU_SHARE = 16   # host cores one GPU's job may use on the benchmark machine
+
+
+def host_cpu():
+    """The host's CPU model, logical CPU count and the cores the baselines use:
+    every core of the job's share (at most CPU_SHARE processes, whatever the
+    machine's total)."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None and os.path.exists("/proc/cpuinfo"):
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "affinity": avail, "used": min(CPU_SHARE, avail)}
+
+
 def cpu_baseline(args):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from refwrap import have_ref
@@ -125,7 +153,8 @@ def cpu_baseline(args):
     if kind == "port" and not have_oracle():
         return {"value": None, "unit": "zone-cycles/s", "cores": 0, "kind": kind,
                 "sample": "neither oracle/_ref/c1 nor oracle/_build/liboracle.so is built on this machine"}
-    ncores = min(16, os.cpu_count() or 1)
+    host = host_cpu()
+    ncores = host["used"]
     slab = 4
     procs = []
     for w in range(ncores):
@@ -143,9 +172,10 @@ def cpu_baseline(args):
                 r = json.loads(line)
                 rate += r["zc"] / r["s"]
                 n_ok += 1
-    what = "reference step_ch() (gcc -O2)" if kind == "reference" else "oracle restatement of step_ch() (gcc -O2)"
+    what = "reference step_ch() (gcc -O3)" if kind == "reference" else "oracle restatement of step_ch() (gcc -O3)"
     return {"value": rate if n_ok else None, "unit": "zone-cycles/s", "cores": n_ok, "kind": kind,
-            "sample": (f"{what}, {n_ok} independent processes, each a "
+            "cpu": host,
+            "sample": (f"{what}, {n_ok} independent processes (the job's host-core share), each a "
                        f"{args.n1}x{args.n2}x{slab} x3-slab of the {args.n1}x{args.n2}x{args.n3} torus, "
                        f"{args.cpu_steps} timed steps after 1 untimed; rates summed (no halo exchange: "
                        f"an upper bound on an MPI run)")}
@@ -188,7 +218,8 @@ def c2p_cpu_baseline(args):
     if not have_ref("c1"):
         return {"value": None, "unit": "solves/s", "cores": 0, "kind": "reference",
                 "sample": "oracle/_ref/c1 is not built on this machine"}
-    ncores = min(16, os.cpu_count() or 1)
+    host = host_cpu()
+    ncores = host["used"]
     procs = []
     for w in range(ncores):
         cmd = [sys.executable, os.path.abspath(__file__), "--_c2p-cpu-worker", "--worker", str(w),
@@ -204,7 +235,8 @@ def c2p_cpu_baseline(args):
                 rate += r["n"] / r["s"]
                 n_ok += 1
     return {"value": rate if n_ok else None, "unit": "solves/s", "cores": n_ok, "kind": "reference",
-            "sample": (f"reference Utoprim_2d_fast() (gcc -O2, oracle/_ref/c1) from the cold guess, {n_ok} "
+            "cpu": host,
+            "sample": (f"reference Utoprim_2d_fast() (gcc -O3, oracle/_ref/c1) from the cold guess, {n_ok} "
                        f"independent processes, each {args.c2p_cpu_zones} zones of the same distribution "
                        f"(different seeds); rates summed")}
 
