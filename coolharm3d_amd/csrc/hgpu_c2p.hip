@@ -18,6 +18,8 @@
 #include "hgpu_c2p_api.h"
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace hgpu {
 namespace {
@@ -321,6 +323,54 @@ int grad_launch(hipStream_t stream, long long n, const float* w, const float* fe
   return hipc(hipGetLastError(), "k_mlp_grad");
 }
 
+// Launches of a compacted solve: the persistent first pass of budget_first
+// iterations, then passes of budget_next over the queue of zones still
+// iterating (the last pass runs to completion).  The queue sizes are cleared
+// by a kernel (no memset node, so the sequence can be captured in a graph).
+int c2p_compacted(hipStream_t s, const C2pArgs& A, const float* mlp, int budget_first, int budget_next,
+                  const C2pWork& W, unsigned G) {
+  const int passes = c2p_npasses(budget_first, budget_next);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, W.count, 64);
+  if (mlp) hipLaunchKernelGGL(k_c2p_first<true>, dim3(G), dim3(CB), 0, s, A, mlp, budget_first, W.q[0], W.count);
+  else hipLaunchKernelGGL(k_c2p_first<false>, dim3(G), dim3(CB), 0, s, A, mlp, budget_first, W.q[0], W.count);
+  for (int p = 1; p <= passes; p++) {
+    const int budget = (p == passes) ? C2P_MAX_ITERS : budget_next;
+    hipLaunchKernelGGL(k_c2p_next, dim3(G), dim3(CB), 0, s, A, W.q[(p - 1) & 1], W.count + (p - 1), budget,
+                       W.q[p & 1], W.count + p);
+  }
+  return hipc(hipGetLastError(), "k_c2p_next");
+}
+
+// Static scratch of hgpu_c2p_solve_static(): the queue sizes and both
+// compaction queues for C2P_STATIC_MAX zones (256 + 2 x 160 B per zone,
+// 80 MiB).  Every GPU the library runs on holds its own copy until the process
+// ends.  Host code never uses the array's name as an address: each device's
+// address comes from hipGetSymbolAddress, once per device, with that device
+// current, and is kept in scratch_ptr.
+constexpr long long C2P_STATIC_MAX = 1LL << 18;
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) char scratch[256 + 2 * (C2P_QD * sizeof(double) + 2 * sizeof(int)) * C2P_STATIC_MAX];
+static_assert(sizeof(scratch) == (size_t)(256 + 2 * (C2P_QD * sizeof(double) + 2 * sizeof(int)) * C2P_STATIC_MAX),
+              "scratch must hold the compaction layout of C2P_STATIC_MAX zones");
+char* scratch_ptr[MAX_DEVICES];
+std::once_flag scratch_once[MAX_DEVICES];
+int scratch_rc[MAX_DEVICES];
+
+int static_scratch(char** out) {
+  int dev = 0;
+  if (int rc = hipc(hipGetDevice(&dev), "hipGetDevice")) return rc;
+  if (dev < 0 || dev >= MAX_DEVICES) return set_error(-4, "device index beyond the static scratch table");
+  std::call_once(scratch_once[dev], [dev] {
+    void* p = nullptr;
+    scratch_rc[dev] = hipc(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    scratch_ptr[dev] = static_cast<char*>(p);
+  });
+  if (scratch_rc[dev]) return scratch_rc[dev];
+  if (!scratch_ptr[dev]) return set_error(-4, "hipGetSymbolAddress(scratch) returned no address");
+  *out = scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace
 }  // namespace hgpu
 
@@ -408,17 +458,26 @@ int hgpu_c2p_solve(void* stream, long long n, double a, double gam, const double
     else hipLaunchKernelGGL(k_c2p_first<false>, dim3(G), dim3(CB), 0, s, A, mlp, C2P_MAX_ITERS, none, nullptr);
     return hipc(hipGetLastError(), "k_c2p_first");
   }
-  C2pWork W = c2p_work_layout(work, n);
-  const int passes = c2p_npasses(budget_first, budget_next);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, W.count, 64);
-  if (mlp) hipLaunchKernelGGL(k_c2p_first<true>, dim3(G), dim3(CB), 0, s, A, mlp, budget_first, W.q[0], W.count);
-  else hipLaunchKernelGGL(k_c2p_first<false>, dim3(G), dim3(CB), 0, s, A, mlp, budget_first, W.q[0], W.count);
-  for (int p = 1; p <= passes; p++) {
-    const int budget = (p == passes) ? C2P_MAX_ITERS : budget_next;
-    hipLaunchKernelGGL(k_c2p_next, dim3(G), dim3(CB), 0, s, A, W.q[(p - 1) & 1], W.count + (p - 1), budget,
-                       W.q[p & 1], W.count + p);
-  }
-  return hipc(hipGetLastError(), "k_c2p_next");
+  return c2p_compacted(s, A, mlp, budget_first, budget_next, c2p_work_layout(work, n), G);
+}
+
+/* Compacted solve with its queues in the library's static scratch array
+ * (no allocation; at most C2P_STATIC_MAX zones per call). */
+int hgpu_c2p_solve_static(void* stream, long long n, double a, double gam, const double* U, const double* rth,
+                          const float* mlp, int plain_guess, int budget_first, int budget_next, double* prim,
+                          int* code, int* iters) {
+  if (n > C2P_STATIC_MAX)
+    return set_error(-3, "n exceeds the static scratch capacity (" + std::to_string(C2P_STATIC_MAX) +
+                             " zones per call); use hgpu_c2p_solve() with a workspace");
+  char* ws = nullptr;
+  if (n > 0)
+    if (int rc = static_scratch(&ws)) return rc;
+  if (int rc = c2p_check_solve(n, a, gam, U, rth, plain_guess, 1, budget_first, budget_next, ws,
+                               (long long)sizeof(scratch), prim, code, iters))
+    return rc;
+  if (n == 0) return 0;
+  C2pArgs A{n, a, gam, (gam - 1.)/gam, U, rth, plain_guess, prim, code, iters};
+  return c2p_compacted(st(stream), A, mlp, budget_first, budget_next, c2p_work_layout(ws, n), persistent_blocks(n));
 }
 
 int hgpu_mlp_sizes(int* out) {

@@ -287,6 +287,19 @@ int hgpu_c2p_solve(void *stream, long long n, double a, double gam, const double
                    const float *mlp, int plain_guess, int compact, int budget_first, int budget_next,
                    void *work, long long work_bytes, double *prim, int *code, int *iters);
 
+/* hgpu_c2p_solve() with compact = 1, allocating nothing: its compaction
+ * queues live in a static device array of the library (80 MiB on each GPU it
+ * runs on, held until the process ends), so n is at most 262144 zones per
+ * call; larger n fails with status -3 and a message.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * caller passes inputs and outputs as for hgpu_c2p_solve(); results are
+ * identical to it.  Every call on one GPU shares that GPU's array, so calls
+ * that overlap on one GPU (concurrent streams or host threads) race. */
+int hgpu_c2p_solve_static(void *stream, long long n, double a, double gam, const double *U, const double *rth,
+                          const float *mlp, int plain_guess, int budget_first, int budget_next, double *prim,
+                          int *code, int *iters);
+
 /* The network (FP32): 8 inputs -> 64 GELU -> 64 GELU -> 2 outputs (log(W/D),
  * log gamma).  out[0..5] = parameter count, trainable count, offset of the
  * trainable part, inputs, hidden width, outputs. */
