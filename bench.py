@@ -11,10 +11,13 @@ CFL time step.  value = interior zones x steps / wall time of the timed steps
 resident in HBM.
 
 Also reported:
-  roofline          -- the dominant kernel's algorithmic HBM bytes per launch
-                       over its average launch time (HIP events on the library
-                       stream, during the timed steps), against 8 TB/s, and the
-                       measured HBM bytes per launch (counter summary in profiles/);
+  roofline          -- the dominant kernel, the conserved update + inversion
+                       (k_update): the state's algorithmic bytes, 160 B per zone
+                       and launch (320 B per zone-cycle over the step's two
+                       update launches), over its average launch time (HIP
+                       events on the library stream), against 8 TB/s; the
+                       measured HBM bytes per launch (counter summary in
+                       profiles/); each kernel family's own byte model beside it;
   mixed_precision   -- the same grid and steps with FP32 reconstruction, face
                        fluxes and EMFs (FP64 update, con-to-prim, metric, cooling),
                        and its speed-up over the FP64 step;
@@ -56,7 +59,15 @@ sys.path.insert(0, ROOT)
 METRIC = "zone-cycles/s per RK2 step, 1 MI355X, mixed vs FP64; con2prim solves/s"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per unit of work of each kernel (FP64 state):
+# Algorithmic bytes of the roofline figure: the step's state traffic, 320 B per
+# zone-cycle (the primitive state read three times and written twice, 8 FP64
+# values per pass), carried by the two conserved-update launches of a step:
+# 160 B per interior zone and update launch.  Faces, EMFs, U^n, flags, halos
+# and geometry are intermediates and excluded.
+STATE_BYTES_PER_ZONE_CYCLE = 320.0
+UPDATE_STATE_BYTES = STATE_BYTES_PER_ZONE_CYCLE / 2
+
+# Per-kernel byte models (FP64 state), reported beside it:
 #   flux   : per face   -- 8 primitives of the zone read once (64 B) + 8 face fluxes written (64 B)
 #   emf    : per edge   -- 6 B-field face fluxes read (48 B) + 3 EMFs written (24 B)
 #   update : per zone   -- p read (64 B); 15 non-magnetic face fluxes + 3 EMFs read (144 B); result
@@ -482,20 +493,32 @@ def main():
 
     tag = "fp64" if args.precision == "fp64" else "mixed"
     units = units_per_launch(n1, n2, n3)
-    dom = max(("flux", "emf", "update"), key=lambda k: kern[k][0])
-    avg_s = kern[dom][0] / max(kern[dom][1], 1) / 1e3
-    algo = ALGO_BYTES[dom] if tag == "fp64" else ALGO_BYTES_MIXED[dom]
-    achieved = algo * units[dom] / avg_s / 1e9
+    model = ALGO_BYTES if tag == "fp64" else ALGO_BYTES_MIXED
+    per_launch_s = {k: kern[k][0] / max(kern[k][1], 1) / 1e3 for k in ("flux", "emf", "update")}
     workload = f"torus{n1}x{n2}x{n3}_{tag}"
     traffic_tab = measured_traffic(workload)
-    traffic = traffic_tab.get(dom) if traffic_tab else None
+    # the dominant kernel: the conserved update + inversion (k_update), the longest
+    # single kernel of the step; its algorithmic bytes are the state's 160 B per zone
+    avg_s = per_launch_s["update"]
+    algo = UPDATE_STATE_BYTES * units["update"]
+    achieved = algo / avg_s / 1e9
+    traffic = traffic_tab.get("update") if traffic_tab else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_update",
+                "algo_bytes_per_unit": UPDATE_STATE_BYTES, "unit_of_work": "interior zone",
+                "units_per_launch": units["update"],
                 "traffic_source": f"profiles/hbm_traffic_{workload}.md" if traffic else None,
-                "avg_launch_ms": avg_s * 1e3, "algo_bytes_per_launch": algo * units[dom],
+                "avg_launch_ms": avg_s * 1e3, "algo_bytes_per_launch": algo,
                 "kernel_ms_per_step": {k: kern[k][0] / args.prof_steps
                                        for k in ("flux", "emf", "update", "recover", "bounds", "fixup")},
-                "state_roofline_frac": value * 320.0 / (HBM_PEAK_GBS * 1e9),
+                "state_roofline_frac": value * STATE_BYTES_PER_ZONE_CYCLE / (HBM_PEAK_GBS * 1e9),
+                "kernel_models": {
+                    k: {"bytes_per_unit": model[k], "units_per_launch": units[k],
+                        "avg_launch_ms": per_launch_s[k] * 1e3,
+                        "achieved_GBs": model[k] * units[k] / per_launch_s[k] / 1e9,
+                        "frac": model[k] * units[k] / per_launch_s[k] / 1e9 / HBM_PEAK_GBS,
+                        "traffic": traffic_tab.get(k) if traffic_tab else None}
+                    for k in ("flux", "emf", "update")},
                 "launch_shapes": shapes, "timed_steps_graph": args.graph == "on"}
 
     mixed = None
