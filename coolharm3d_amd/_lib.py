@@ -36,6 +36,7 @@ class HgpuConfig(ctypes.Structure):
         ("bbh_m1", ctypes.c_double), ("bbh_m2", ctypes.c_double), ("bbh_sep", ctypes.c_double),
         ("bbh_phi0", ctypes.c_double), ("bbh_inspiral", ctypes.c_int),
         ("bc_nbr", ctypes.c_int), ("gpos", ctypes.c_int * 3), ("no_confirm_iters", ctypes.c_int),
+        ("diag3_exponent", ctypes.c_int), ("diag3_factor", ctypes.c_double),
     ]
 
 
@@ -46,6 +47,7 @@ class HgpuPnConfig(ctypes.Structure):
         ("th_length", ctypes.c_double), ("h_slope", ctypes.c_double),
         ("n1", ctypes.c_int), ("n2", ctypes.c_int), ("n3", ctypes.c_int),
         ("startx", ctypes.c_double * 4), ("dx", ctypes.c_double * 4),
+        ("coords", ctypes.c_int), ("diag3_exponent", ctypes.c_int), ("diag3_factor", ctypes.c_double),
     ]
 
 

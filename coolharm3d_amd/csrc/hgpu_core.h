@@ -60,10 +60,15 @@ inline int make_params(const hgpu_config& c, Params& P) {
   if (c.n1 < 1 || c.n2 < 1 || c.n3 < 1) return set_error(-2, "grid sizes must be >= 1");
   if (c.metric != 0 && c.metric != 2 && c.metric != 3)
     return set_error(-2, "metric must be 0 (Minkowski Cartesian), 2 (Kerr-Schild) or 3 (binary black hole)");
-  if (c.metric == 3 && c.coords != 1) return set_error(-2, "the binary-black-hole metric needs coords = 1 (spherical)");
+  if (c.metric == 3 && c.coords != 1 && c.coords != 4)
+    return set_error(-2, "the binary-black-hole metric needs spherical coordinates (coords = 1 or 4)");
   if (c.metric == 3 && !(c.bbh_m1 >= 0. && c.bbh_m2 >= 0. && c.bbh_m1 + c.bbh_m2 > 0. && c.bbh_sep > 0.))
     return set_error(-2, "binary: masses >= 0 with a positive sum and a positive separation required");
-  if (c.coords != 0 && c.coords != 1) return set_error(-2, "coords must be 0 (identity) or 1 (modified Kerr-Schild)");
+  if (c.coords != 0 && c.coords != 1 && c.coords != 4)
+    return set_error(-2, "coords must be 0 (identity), 1 (COORD_DIAGONAL) or 4 (COORD_DIAGONAL3)");
+  // set_special_coord()'s check of the exponent (src/init.nz_pn_bbh_disk.c:360)
+  if (c.coords == 4 && (c.diag3_exponent < 3 || c.diag3_exponent % 2 == 0))
+    return set_error(-2, "COORD_DIAGONAL3 needs an odd diag3_exponent >= 3");
   if (c.recon != RECON_MC && c.recon != RECON_PPM_MC) return set_error(-2, "recon must be 2 (MC) or 5 (PPM+MC)");
   if (c.flux != FLUX_LF && c.flux != FLUX_HLLE) return set_error(-2, "flux must be 0 (LF) or 1 (HLLE)");
   if (c.bc != BC_CARTESIAN_OUTFLOW && c.bc != BC_SPHERICAL_OUTFLOW3) return set_error(-2, "bc must be 0 or 10");
@@ -96,7 +101,7 @@ inline int make_params(const hgpu_config& c, Params& P) {
   P.bc = c.bc;
   P.bc_nbr = c.bc_nbr;
   P.n_extra_newt = c.no_confirm_iters ? 0 : EXTRA_NEWT_ITER;
-  P.floor_dim = (c.coords == 1) ? 1 : 3;
+  P.floor_dim = (c.coords != 0) ? 1 : 3;   // spherical maps: floors depend on r, i.e. on x1 only
   P.n_spatial_dims = (c.n1 > 1) + (c.n2 > 1) + (c.n3 > 1);
   P.track_nfail = c.track_nfail;
   if (c.precision != 0 && c.precision != 1) return set_error(-2, "precision must be 0 (FP64) or 1 (mixed)");
@@ -122,6 +127,7 @@ inline CoordParams coord_params(const hgpu_config& c) {
   C.a = (c.metric == 3) ? 0. : c.a;
   C.asq = C.a*C.a; C.M = 1.; C.R0 = c.R0;
   C.h_slope = c.h_slope; C.th_beg = c.th_beg; C.th_length = c.th_length;
+  C.diag3_exponent = c.diag3_exponent; C.diag3_factor = c.diag3_factor;
   for (int d = 0; d < 3; d++) C.gpos[d] = c.gpos[d];
   return C;
 }
@@ -243,6 +249,7 @@ struct Api {
       B.m1 = cfg->bbh_m1; B.m2 = cfg->bbh_m2; B.b0 = cfg->bbh_sep; B.phi0 = cfg->bbh_phi0;
       B.inspiral = cfg->bbh_inspiral;
       B.R0 = cfg->R0; B.th_beg = cfg->th_beg; B.th_length = cfg->th_length; B.h_slope = cfg->h_slope;
+      B.coords = cfg->coords; B.diag3_exponent = cfg->diag3_exponent; B.diag3_factor = cfg->diag3_factor;
       for (int l = 0; l < 4; l++) c->pn.x0[l] = cfg->startx[l];
       for (int d = 0; d < 3; d++) c->pn.gpos[d] = cfg->gpos[d];
     }

@@ -18,9 +18,11 @@ namespace hgpu {
 
 struct CoordParams {
   int metric;        // 0 Minkowski Cartesian, 2 Kerr-Schild spherical
-  int coords;        // 0 identity, 1 modified Kerr-Schild (COORD_DIAGONAL)
+  int coords;        // 0 identity, 1 modified Kerr-Schild (COORD_DIAGONAL), 4 COORD_DIAGONAL3
   double startx[4], dx[4];
   double a, asq, M, R0, h_slope, th_beg, th_length;
+  int diag3_exponent;
+  double diag3_factor;
   int gpos[3];       // global index of the block's zone 0 minus the local one (globalpos[1..3])
 };
 
@@ -58,6 +60,15 @@ inline void x_of_xp(const CoordParams& C, const double* xp, double* x, double J[
     x[3] = xp[3];
     J[1] = x[1] - C.R0;
     J[2] = C.th_length + 2 * M_PI * C.h_slope * cos(2 * M_PI * xp[2]);
+  } else if (C.coords == 4) {
+    // COORD_DIAGONAL3: theta a polynomial in f = 2 seesaw(x2) - 1 (src/coord.c:270-276, :468-472)
+    const double f = 2 * seesaw01(xp[2]) - 1.;
+    x[0] = xp[0];
+    x[1] = C.R0 + exp(xp[1]);
+    x[2] = 0.5*M_PI*(1. + C.h_slope*f + C.diag3_factor*pow(f, (double)C.diag3_exponent));
+    x[3] = xp[3];
+    J[1] = x[1] - C.R0;
+    J[2] = M_PI*(C.h_slope + C.diag3_exponent*C.diag3_factor*pow(f, (double)(C.diag3_exponent - 1)));
   } else {
     for (int l = 0; l < 4; l++) x[l] = xp[l];
   }
@@ -200,6 +211,12 @@ inline void ks_connection(const CoordParams& C, int i, int j, double* conn) {
   if (C.coords == 1) {
     Jd[1] = r - C.R0;                          // d^2 r/dx1^2 = exp(x1)
     Jd[2] = -4.*M_PI*M_PI*C.h_slope*sin(2.*M_PI*xp[2]);
+  } else if (C.coords == 4) {
+    // dx_dxp_dxp_calc(), src/coord.c:942-947
+    const double f = 2 * seesaw01(xp[2]) - 1.;
+    const int n = C.diag3_exponent;
+    Jd[1] = r - C.R0;
+    Jd[2] = 2*M_PI*n*(n - 1)*C.diag3_factor*pow(f, (double)(n - 2));
   }
   // dg'_{ab}/dx^c in numerical coordinates, c = 1 (x1) and 2 (x2)
   double dg[4][4][4] = {{{0}}};   // dg[c][a][b]
@@ -241,7 +258,7 @@ struct CoolTables {
 
 inline void ks_at_equator(const CoordParams& C, double r, double* xr, double* xth) {
   double xp[4] = {0., 0., 0.5, 0.};
-  xp[1] = (C.coords == 1) ? log(r - C.R0) : r;
+  xp[1] = (C.coords != 0) ? log(r - C.R0) : r;
   double x[4], J[4];
   x_of_xp(C, xp, x, J);
   *xr = x[1];

@@ -216,13 +216,26 @@ HD void pn_coord1(const PnBinary& B, const T& X1, T* r, T* dr) {
   *r = er + B.R0;
   *dr = er;
 }
+// x^p for an integer p >= 1 as pow() forms it, with the derivative
+// p x^(p-1) (defined at x = 0, where the chain rule through pow's log is not)
+HD double pow_int(double x, int p) { return pow(x, (double)p); }
+HD Dual4 pow_int(const Dual4& x, int p) { return ad::chain(x, ::pow(x.v, (double)p), p * ::pow(x.v, (double)(p - 1))); }
+
 template <class T>
 HD void pn_coord2(const PnBinary& B, const T& X2, T* st, T* ct, T* dth) {
-  // theta from the reflected X2, its derivative from X2 itself, as x_of_xp()
-  // and dx_dxp_calc() form them (src/coord.c:240-382)
+  // theta from the reflected X2, as x_of_xp() and dx_dxp_calc() form it
+  // (src/coord.c:240-382): COORD_DIAGONAL's derivative from X2 itself,
+  // COORD_DIAGONAL3's from f = 2 seesaw(X2) - 1 (src/coord.c:270-276, :468-472)
   const T X2s = X2 + (seesaw_unit(value(X2)) - value(X2));
-  const T th = B.th_length * X2s + B.th_beg + B.h_slope * sin(2. * M_PI * X2s);
-  *dth = B.th_length + 2. * M_PI * B.h_slope * cos(2. * M_PI * X2);
+  T th;
+  if (B.coords == 4) {
+    const T f = 2. * X2s - 1.;
+    th = 0.5 * M_PI * (1. + B.h_slope * f + B.diag3_factor * pow_int(f, B.diag3_exponent));
+    *dth = M_PI * (B.h_slope + (B.diag3_exponent * B.diag3_factor) * pow_int(f, B.diag3_exponent - 1));
+  } else {
+    th = B.th_length * X2s + B.th_beg + B.h_slope * sin(2. * M_PI * X2s);
+    *dth = B.th_length + 2. * M_PI * B.h_slope * cos(2. * M_PI * X2);
+  }
   *st = sin(th);
   *ct = cos(th);
   if (fabs(value(*st)) < 1.E-16) set_value(*st, value(*st) >= 0. ? 1.E-16 : -1.E-16);   // trigvalues(), src/metric.c:29-40

@@ -12,6 +12,10 @@ inline int pn_check(const hgpu_pn_config* c) {
   if (!(c->b0 > 0.)) return set_error(-2, "separation b0 must be > 0");
   if (c->n1 < 1 || c->n2 < 1 || c->n3 < 1) return set_error(-2, "block sizes must be >= 1");
   if ((long long)c->n1 * c->n2 * c->n3 > (1LL << 31) - 1) return set_error(-2, "block larger than 2^31 zones");
+  if (c->coords != 0 && c->coords != 1 && c->coords != 4)
+    return set_error(-2, "coords must be 0 or 1 (COORD_DIAGONAL) or 4 (COORD_DIAGONAL3)");
+  if (c->coords == 4 && (c->diag3_exponent < 3 || c->diag3_exponent % 2 == 0))
+    return set_error(-2, "COORD_DIAGONAL3 needs an odd diag3_exponent >= 3");
   return 0;
 }
 
@@ -19,6 +23,8 @@ inline PnBinary pn_binary(const hgpu_pn_config* c) {
   PnBinary B;
   B.m1 = c->m1; B.m2 = c->m2; B.b0 = c->b0; B.phi0 = c->phi0; B.inspiral = c->inspiral;
   B.R0 = c->R0; B.th_beg = c->th_beg; B.th_length = c->th_length; B.h_slope = c->h_slope;
+  B.coords = (c->coords == 4) ? 4 : 1;
+  B.diag3_exponent = c->diag3_exponent; B.diag3_factor = c->diag3_factor;
   return B;
 }
 

@@ -121,8 +121,10 @@ struct PnBinary {
   double b0, phi0;        // separation and phase at t = 0
   int inspiral;           // 1: radiation-reaction decay of b (when both masses are > 0);
                           // 0: circular orbit of fixed b0
-  // numerical coordinates
+  // numerical coordinates: theta map 1 (COORD_DIAGONAL) or 4 (COORD_DIAGONAL3)
   double R0, th_beg, th_length, h_slope;
+  int coords, diag3_exponent;
+  double diag3_factor;
 };
 // Binary and grid of a context whose metric is the binary's; x0: startx.
 // tab: the coordinate functions at every zone's lower face (h = 0) and centre

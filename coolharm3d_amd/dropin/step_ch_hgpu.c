@@ -17,7 +17,15 @@
  * On its first call the shim creates the device context from the run's
  * globals and options, uploads the geometry the reference computed
  * (geom_arr[0] and conn, so the device uses the reference's own metric and
- * connection) and the primitive state.  After every step it copies back the
+ * connection) and the primitive state.  Coordinates: COORD_IDENTITY,
+ * COORD_DIAGONAL and COORD_DIAGONAL3 (with the run's diag3_exponent and
+ * diag3_factor).  METRIC_GENERAL_DYNAMIC with a post-Newtonian binary
+ * (METRIC_DYNAMIC_TYPE_CHOICE one of the dyn_*pn* metrics, whose bodies the
+ * reference tree does not ship) runs on the library's own binary metric: the
+ * 1PN near-zone metric of m_bh1, m_bh2 at initial_bbh_separation and phase
+ * phi_0_bbh, inspiralling by radiation reaction (HGPU_BBH_CIRCULAR=1: a fixed
+ * circular orbit), evaluated on the device at every level; nothing of the
+ * host's geometry is uploaded then.  After every step it copies back the
  * clock, failure_exists and nfailtot (check_for_crash() reads them,
  * src/step_ch.c:1244) and, depending on HGPU_HOST_SYNC (compile-time default,
  * overridden by the environment variable of the same name), the state and
@@ -90,6 +98,20 @@ static void hg_setup(void) {
   cfg.metric = 0;
 #elif( METRIC_TYPE_CHOICE == METRIC_KS_SPHERICAL )
   cfg.metric = 2;
+#elif( METRIC_TYPE_CHOICE == METRIC_GENERAL_DYNAMIC && \
+       METRIC_DYNAMIC_TYPE_CHOICE != METRIC_DYNAMIC_KS_SPHERICAL && \
+       METRIC_DYNAMIC_TYPE_CHOICE != METRIC_DYNAMIC_MINK_CARTESIAN && \
+       METRIC_DYNAMIC_TYPE_CHOICE != METRIC_DYNAMIC_MINK_SPHERICAL )
+  /* A post-Newtonian binary (the dyn_*pn* metrics of src/metric.h:104-171): the
+     library evaluates its own binary metric -- the 1PN near-zone metric of the
+     two holes on a radiation-reaction inspiral -- at the step's three levels on
+     the device, from the binary set up by set_special_coord()
+     (src/init.nz_pn_bbh_disk.c:311-317; src/globals.h:472-487).  The reference's
+     higher-order and matched-zone metrics are not restated. */
+  cfg.metric = 3;
+  cfg.bbh_m1 = m_bh1; cfg.bbh_m2 = m_bh2;
+  cfg.bbh_sep = initial_bbh_separation; cfg.bbh_phi0 = phi_0_bbh;
+  cfg.bbh_inspiral = (getenv("HGPU_BBH_CIRCULAR") && atoi(getenv("HGPU_BBH_CIRCULAR"))) ? 0 : 1;
 #else
   fprintf(stderr, "step_ch_hgpu: METRIC_TYPE_CHOICE %d is not supported on the GPU\n", METRIC_TYPE_CHOICE);
   myexit(1);
@@ -98,6 +120,9 @@ static void hg_setup(void) {
   cfg.coords = 0;
 #elif( COORD_TYPE_CHOICE == COORD_DIAGONAL )
   cfg.coords = 1;
+#elif( COORD_TYPE_CHOICE == COORD_DIAGONAL3 )
+  cfg.coords = 4;
+  cfg.diag3_exponent = diag3_exponent; cfg.diag3_factor = diag3_factor;
 #else
   fprintf(stderr, "step_ch_hgpu: COORD_TYPE_CHOICE %d is not supported on the GPU\n", COORD_TYPE_CHOICE);
   myexit(1);
@@ -145,7 +170,9 @@ static void hg_setup(void) {
   hg_nfail_base = (int *)malloc(sizeof(int) * N_NFAIL * hg_ncell());
   if (!hg_stage || !hg_istage || !hg_nfail_base) { fprintf(stderr, "step_ch_hgpu: out of host memory\n"); myexit(1); }
 
-  /* the reference's geometry records and connection */
+  /* the reference's geometry records and connection (a post-Newtonian binary's
+     are evaluated by the library at every level instead) */
+#if( METRIC_TYPE_CHOICE != METRIC_GENERAL_DYNAMIC )
   {
     long long ng, nc;
     hg_check(hgpu_geometry_counts(hg, &ng, &nc), "hgpu_geometry_counts");
@@ -159,6 +186,7 @@ static void hg_setup(void) {
     hg_check(hgpu_upload_geometry(hg, (const double *)geom_arr[0], cn), "hgpu_upload_geometry");
     free(cn);
   }
+#endif
   for (l = 0; l < N_NFAIL; l++) {
     hg_nfailtot_base[l] = nfailtot[l];
     ALL_LOOP hg_nfail_base[(long)l * hg_ncell() + ((long)i * N2TOT + j) * N3TOT + k] = nfail[l][i][j][k];

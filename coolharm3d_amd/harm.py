@@ -48,7 +48,8 @@ def config_for(grid: GridSpec, **kw) -> _lib.HgpuConfig:
     bbh = grid.metric == "pn_binary"
     ks = grid.metric == "kerr_schild" or bbh
     c.metric = 3 if bbh else (2 if ks else 0)
-    c.coords = 1 if grid.coords == "mks" else 0
+    c.coords = {"identity": 0, "mks": 1, "diag3": 4}[grid.coords]
+    c.diag3_exponent, c.diag3_factor = grid.diag3_exponent, grid.diag3_factor
     c.recon = kw.get("recon", RECON_PPM_MC if ks else RECON_MC)
     c.flux = kw.get("flux", FLUX_HLLE)
     c.ct_para = kw.get("ct_para", 1 if ks else 0)

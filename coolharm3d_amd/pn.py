@@ -22,13 +22,17 @@ FACE1, FACE2, FACE3, CENT, CORN = range(5)
 
 
 def bbh_block(n1=300, n2=160, n3=400, sep=20.0, r_in_over_sep=1.5, r_out_over_sep=13.0, h_slope=0.3,
-              ghosts=3, m1=0.5, m2=0.5, phi0=0.0, inspiral=True):
+              ghosts=3, m1=0.5, m2=0.5, phi0=0.0, inspiral=True, coords="mks", diag3_exponent=9):
     """Configuration of the circumbinary block: r in [1.5, 13] separations,
-    log-radial X1, modified Kerr-Schild theta map, full 2 pi in phi; the block
-    includes `ghosts` ghost layers on every side."""
+    log-radial X1, full 2 pi in phi; the block includes `ghosts` ghost layers
+    on every side.  Theta map: "mks", the modified Kerr-Schild map with sin
+    coefficient h_slope, or "diag3", COORD_DIAGONAL3 with h_slope its linear
+    coefficient and the reference's exponent and factor 1 - h_slope."""
     c = _lib.HgpuPnConfig()
     c.m1, c.m2, c.b0, c.phi0, c.inspiral = m1, m2, sep, phi0, 1 if inspiral else 0
     c.R0, c.th_beg, c.th_length, c.h_slope = 0.0, 0.0, math.pi, h_slope
+    c.coords = 4 if coords == "diag3" else 1
+    c.diag3_exponent, c.diag3_factor = diag3_exponent, (1.0 - h_slope) if coords == "diag3" else 0.0
     dx1 = (math.log(r_out_over_sep * sep) - math.log(r_in_over_sep * sep)) / n1
     dx = [0.0, dx1, 1.0 / n2, 2 * math.pi / n3]
     start = [0.0, math.log(r_in_over_sep * sep), 0.0, 0.0]

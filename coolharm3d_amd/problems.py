@@ -37,7 +37,7 @@ class GridSpec:
     startx: tuple   # numerical-coordinate origin of cell (0,0,0) corner (ghosts included)
     dx: tuple       # (dt, dx1, dx2, dx3)
     metric: str     # "minkowski" or "kerr_schild"
-    coords: str     # "identity" or "mks"
+    coords: str     # "identity", "mks" (COORD_DIAGONAL) or "diag3" (COORD_DIAGONAL3)
     gam: float
     cour: float
     t_end: float
@@ -46,6 +46,8 @@ class GridSpec:
     h_slope: float = 0.0
     th_beg: float = 0.0
     th_length: float = math.pi
+    diag3_exponent: int = 9
+    diag3_factor: float = 0.0
     extra: dict = field(default_factory=dict)
 
     @property
@@ -78,22 +80,33 @@ def balsara1(grid: GridSpec):
 
 
 def torus_grid(n1=192, n2=192, n3=128, a=0.9, Rin=1.3, Rout=240.0, hslope_harm=0.3,
-               gam=5.0 / 3.0, cour=0.8, x3_extent=0.5 * math.pi, dt0=1.0e-5, t_end=1.0e4):
-    """Modified Kerr-Schild grid of ``src/init.torus.c:54-118`` with R0 = 0.
+               gam=5.0 / 3.0, cour=0.8, x3_extent=0.5 * math.pi, dt0=1.0e-5, t_end=1.0e4, coords="mks",
+               diag3_h_slope=0.0729419909586, diag3_exponent=9):
+    """Kerr-Schild grid of ``src/init.torus.c:54-118`` with R0 = 0.
 
-    ``hslope_harm`` is HARM's h; the reference's sin coefficient is (1-h)/2
-    (``src/coord.c:252``), so h = 0.3 gives h_slope = 0.35.
+    coords "mks" (COORD_DIAGONAL): ``hslope_harm`` is HARM's h; the
+    reference's sin coefficient is (1-h)/2 (``src/coord.c:252``), so h = 0.3
+    gives h_slope = 0.35.  coords "diag3" (COORD_DIAGONAL3, ``src/coord.c:270-276``):
+    theta = pi/2 (1 + h_slope f + diag3_factor f^n), f = 2 x2 - 1, with the
+    values of the reference's binary-disk set-up (``src/init.nz_pn_bbh_disk.c:332-360``):
+    h_slope = 0.0729419909586 (the polar-to-equatorial theta spacing ratio),
+    n = 9, diag3_factor = 1 - 2 th_cutout/pi - h_slope.
     """
     th_cutout = 1.0e-15 * math.pi
     th_beg = th_cutout
     th_length = (math.pi - th_cutout) - th_beg
-    h_slope = 0.5 * (1.0 - hslope_harm)
+    if coords == "diag3":
+        h_slope = diag3_h_slope
+        factor = 1.0 - 2 * th_cutout / math.pi - h_slope
+    else:
+        h_slope = 0.5 * (1.0 - hslope_harm)
+        factor = 0.0
     L1 = math.log(Rout / Rin)
     dx1, dx2, dx3 = L1 / n1, 1.0 / n2, x3_extent / n3
     startx = (0.0, math.log(Rin) - NG * dx1, -NG * dx2, -NG * dx3)
-    return GridSpec(n1, n2, n3, startx, (dt0, dx1, dx2, dx3), "kerr_schild", "mks",
+    return GridSpec(n1, n2, n3, startx, (dt0, dx1, dx2, dx3), "kerr_schild", coords,
                     gam=gam, cour=cour, t_end=t_end, a=a, R0=0.0, h_slope=h_slope,
-                    th_beg=th_beg, th_length=th_length,
+                    th_beg=th_beg, th_length=th_length, diag3_exponent=diag3_exponent, diag3_factor=factor,
                     extra=dict(Rin=Rin, Rout=Rout, x3_extent=x3_extent))
 
 
@@ -105,10 +118,25 @@ def _seesaw01(x):
     return np.where(x1 < 1.0, x1, 2.0 - x1)
 
 
+def theta_map(grid: GridSpec, xp2):
+    """theta(x2) and dtheta/dx2 of the grid's spherical map, as x_of_xp() and
+    dx_dxp_calc() form them (``src/coord.c:245-276``, ``:447-472``)."""
+    xp2 = np.asarray(xp2, dtype=np.float64)
+    x2 = _seesaw01(xp2)
+    if grid.coords == "diag3":
+        n, c = grid.diag3_exponent, grid.diag3_factor
+        f = 2.0 * x2 - 1.0
+        th = 0.5 * math.pi * (1.0 + grid.h_slope * f + c * np.power(f, float(n)))
+        dth = math.pi * (grid.h_slope + n * c * np.power(f, float(n - 1)))
+    else:
+        th = grid.th_length * x2 + grid.th_beg + grid.h_slope * np.sin(2.0 * math.pi * x2)
+        dth = grid.th_length + 2.0 * math.pi * grid.h_slope * np.cos(2.0 * math.pi * xp2)
+    return th, dth
+
+
 def mks_r_th(grid: GridSpec, xp1, xp2):
     r = grid.R0 + np.exp(xp1)
-    x2 = _seesaw01(xp2)
-    th = grid.th_length * x2 + grid.th_beg + grid.h_slope * np.sin(2.0 * math.pi * x2)
+    th, _ = theta_map(grid, xp2)
     return r, th
 
 
@@ -200,7 +228,7 @@ def torus(grid: GridSpec, rin=6.0, rmax=12.0, beta=100.0, kappa=1.0e-3, randfact
     ravg = 0.25 * (rho2[1:, 1:] + rho2[:-1, 1:] + rho2[1:, :-1] + rho2[:-1, :-1])
     A[1:-1, 1:-1] = np.maximum(ravg / 1.0 - 0.2, 0.0)
     # gdet of MKS at cell centres: Sigma |sin th| * (r - R0) * dth/dx2
-    dthdx2 = grid.th_length + 2.0 * math.pi * grid.h_slope * np.cos(2.0 * math.pi * xp2)
+    dthdx2 = theta_map(grid, xp2)[1]
     gdet = SS * np.abs(sth) * (r - grid.R0) * dthdx2[None, :]
     B1v = -(A[:-1, :-1] - A[:-1, 1:] + A[1:, :-1] - A[1:, 1:]) / (2.0 * grid.dx[2] * gdet)
     B2v = (A[:-1, :-1] + A[:-1, 1:] - A[1:, :-1] - A[1:, 1:]) / (2.0 * grid.dx[1] * gdet)
@@ -244,7 +272,7 @@ def _ks_gcov_mks(grid, r, th):
     xp2 = grid.startx[2] + (np.arange(N2T) + 0.5) * grid.dx[2]
     J = np.ones(r.shape + (4,))
     J[..., 1] = r - grid.R0
-    J[..., 2] = (grid.th_length + 2.0 * math.pi * grid.h_slope * np.cos(2.0 * math.pi * xp2))[None, :]
+    J[..., 2] = theta_map(grid, xp2)[1][None, :]
     return g * J[..., :, None] * J[..., None, :]
 
 
@@ -264,18 +292,26 @@ def _ucon_from_prims(gcov, r, th, grid, v1, v2, v3):
 # Circumbinary disk around an inspiralling black-hole binary
 
 def bbh_grid(n1=300, n2=160, n3=400, sep=20.0, r_in=1.5, r_out=13.0, hslope_harm=0.3, m1=0.5, m2=0.5,
-             inspiral=True, gam=5.0 / 3.0, cour=0.8, dt0=1.0e-3, t_end=1.0e6):
+             inspiral=True, gam=5.0 / 3.0, cour=0.8, dt0=1.0e-3, t_end=1.0e6, coords="mks",
+             diag3_h_slope=0.0729419909586, diag3_exponent=9):
     """Spherical grid about the binary's centre of mass: r in [r_in, r_out]
-    separations (log-radial X1), the modified Kerr-Schild theta map with HARM's
-    h = hslope_harm, full 2 pi in phi; M = m1 + m2 = 1 and the separation `sep`
-    in units of M."""
-    th_beg, th_length = 1.0e-15 * math.pi, math.pi * (1.0 - 2.0e-15)
-    h_slope = 0.5 * (1.0 - hslope_harm)
+    separations (log-radial X1), full 2 pi in phi; M = m1 + m2 = 1 and the
+    separation `sep` in units of M.  Theta map: coords "mks", the modified
+    Kerr-Schild map with HARM's h = hslope_harm, or "diag3", the reference's
+    binary-disk map (COORD_DIAGONAL3 with its h_slope and exponent,
+    ``src/init.nz_pn_bbh_disk.c:332-360``)."""
+    th_cutout = 1.0e-15 * math.pi
+    th_beg, th_length = th_cutout, math.pi * (1.0 - 2.0e-15)
+    if coords == "diag3":
+        h_slope, factor = diag3_h_slope, 1.0 - 2 * th_cutout / math.pi - diag3_h_slope
+    else:
+        h_slope, factor = 0.5 * (1.0 - hslope_harm), 0.0
     Rin, Rout = r_in * sep, r_out * sep
     dx1, dx2, dx3 = math.log(Rout / Rin) / n1, 1.0 / n2, 2.0 * math.pi / n3
     startx = (0.0, math.log(Rin) - NG * dx1, -NG * dx2, -NG * dx3)
-    return GridSpec(n1, n2, n3, startx, (dt0, dx1, dx2, dx3), "pn_binary", "mks", gam=gam, cour=cour,
+    return GridSpec(n1, n2, n3, startx, (dt0, dx1, dx2, dx3), "pn_binary", coords, gam=gam, cour=cour,
                     t_end=t_end, a=0.0, R0=0.0, h_slope=h_slope, th_beg=th_beg, th_length=th_length,
+                    diag3_exponent=diag3_exponent, diag3_factor=factor,
                     extra=dict(Rin=Rin, Rout=Rout, sep=sep, m1=m1, m2=m2, inspiral=inspiral))
 
 
@@ -291,6 +327,8 @@ def bbh_gdet(grid: GridSpec, metric_cls, hostcheck=False, t=0.0, chunk=16):
         c = bbh_block(m, N2T, N3T, sep=e["sep"], m1=e["m1"], m2=e["m2"], inspiral=e["inspiral"], ghosts=0,
                       h_slope=0.0)
         c.h_slope, c.th_beg, c.th_length = grid.h_slope, grid.th_beg, grid.th_length
+        c.coords = 4 if grid.coords == "diag3" else 1
+        c.diag3_exponent, c.diag3_factor = grid.diag3_exponent, grid.diag3_factor
         for l in range(1, 4):
             c.startx[l] = grid.startx[l] + (i0 * grid.dx[1] if l == 1 else 0.0)
             c.dx[l] = grid.dx[l]
@@ -342,7 +380,7 @@ def bbh_disk(grid: GridSpec, gdet, h_o_r=0.1, beta=100.0, noise=0.01, seed=2718,
     B1v = g1[:, :, None] / gdet
     B2v = g2[:, :, None] / gdet
     # b^2 ~ gamma_ij B^i B^j in the weak field of the grid (r >= 1.5 separations)
-    dthdx2 = grid.th_length + 2.0 * math.pi * grid.h_slope * np.cos(2.0 * math.pi * xp2)
+    dthdx2 = theta_map(grid, xp2)[1]
     bsq = (r[:, :, None] * B1v) ** 2 + (r[:, :, None] * dthdx2[None, :, None] * B2v) ** 2
     bsq_max = bsq[NG:-NG, NG:-NG, NG:-NG].max()
     pmax = (gam - 1.0) * p[NG:-NG, NG:-NG, NG:-NG, UU].max()

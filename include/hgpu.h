@@ -29,7 +29,9 @@ typedef struct hgpu_config {
   int metric;              /* 0: METRIC_MINK_CARTESIAN, 2: METRIC_KS_SPHERICAL (src/options.h:73-83),
                               3: the time-dependent binary-black-hole metric of hgpu_pn_geometry()
                               (METRIC_GENERAL_DYNAMIC with a PN binary, src/metric.h:104-171)     */
-  int coords;              /* 0: COORD_IDENTITY, 1: COORD_DIAGONAL (src/options.h:136-147)           */
+  int coords;              /* 0: COORD_IDENTITY, 1: COORD_DIAGONAL, 4: COORD_DIAGONAL3
+                              (src/options.h:136-147; the last two with h_slope, th_beg,
+                              th_length, and diag3_exponent/diag3_factor below)                  */
   int recon;               /* 2: RECON_MC, 5: RECON_PPM_MC (src/options.h:151-157)                   */
   int flux;                /* 0: FLUX_LF, 1: FLUX_HLLE (src/options.h:169-172)                        */
   int ct_para;             /* USE_FLUX_CT_PARA (src/options.h:165)                                    */
@@ -72,6 +74,12 @@ typedef struct hgpu_config {
    * zones the update hands to the full recovery tree keep them.  0 (default): the reference's
    * iteration count.  No counterpart in the reference (an algorithm change, in either precision). */
   int no_confirm_iters;
+  /* coords = 4 (COORD_DIAGONAL3, src/coord.c:270-276): theta = pi/2 (1 + h_slope f +
+   * diag3_factor f^diag3_exponent), f = 2 seesaw(X2) - 1; diag3_exponent an odd integer
+   * >= 3 (the reference's globals of the same names, set by set_special_coord(),
+   * src/init.nz_pn_bbh_disk.c:357-360: 9 and 1 - 2 th_cutout/pi - h_slope).           */
+  int diag3_exponent;
+  double diag3_factor;
 } hgpu_config;
 
 typedef struct hgpu_ctx hgpu_ctx;
@@ -348,6 +356,9 @@ typedef struct hgpu_pn_config {
   double R0, th_beg, th_length, h_slope;  /* numerical-coordinate map              */
   int n1, n2, n3;            /* zones of the block (ghosts included if wanted)     */
   double startx[4], dx[4];   /* X of the block's first zone corner, zone widths    */
+  int coords;                /* theta map: 0 or 1 COORD_DIAGONAL, 4 COORD_DIAGONAL3 */
+  int diag3_exponent;        /* COORD_DIAGONAL3 parameters (as in hgpu_config)     */
+  double diag3_factor;
 } hgpu_pn_config;
 
 /* Geometry records (struct of_geom, 42 doubles) of every zone of the block at

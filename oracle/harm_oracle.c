@@ -144,6 +144,15 @@ static void x_of_xp(const orc_t* o, const double* xp, double* x, double J[4][4])
     x[3] = xp[3];
     J[1][1] = x[1] - o->c.R0;
     J[2][2] = o->c.th_length + 2 * M_PI * o->c.h_slope * cos(2 * M_PI * xp[2]);
+  } else if (o->c.coords == 4) {
+    /* COORD_DIAGONAL3, src/coord.c:270-276 and :468-472 */
+    double f = 2 * seesaw01(xp[2]) - 1.;
+    x[0] = xp[0];
+    x[1] = o->c.R0 + exp(xp[1]);
+    x[2] = 0.5 * M_PI * (1. + o->c.h_slope * f + o->c.diag3_factor * pow(f, o->c.diag3_exponent));
+    x[3] = xp[3];
+    J[1][1] = x[1] - o->c.R0;
+    J[2][2] = M_PI * (o->c.h_slope + o->c.diag3_exponent * o->c.diag3_factor * pow(f, o->c.diag3_exponent - 1));
   } else {
     for (a = 0; a < 4; a++) x[a] = xp[a];
   }
@@ -256,6 +265,12 @@ static dn dn_scale(dn a, double s) { dn r = {a.v * s, a.d1 * s, a.d2 * s}; retur
 static dn dn_sin(dn a) { double c = cos(a.v); dn r = {sin(a.v), c * a.d1, c * a.d2}; return r; }
 static dn dn_cos(dn a) { double s = sin(a.v); dn r = {cos(a.v), -s * a.d1, -s * a.d2}; return r; }
 static dn dn_exp(dn a) { double e = exp(a.v); dn r = {e, e * a.d1, e * a.d2}; return r; }
+/* a^n for an integer n >= 1, derivative n a^(n-1) (defined at a = 0) */
+static dn dn_powi(dn a, int n) {
+  double f = pow(a.v, n), df = n * pow(a.v, n - 1);
+  dn r = {f, df * a.d1, df * a.d2};
+  return r;
+}
 
 /* Kerr-Schild metric in numerical coordinates, g'_ab = g_ab J_a J_b (the
    modified Kerr-Schild Jacobian is diagonal), as derivative numbers. */
@@ -271,6 +286,16 @@ static void ks_gcov_num_dn(const orc_t* o, double x1, double x2, dn g[4][4]) {
     dn w = dn_scale(X2, 2 * M_PI);
     th = dn_add(dn_add(dn_scale(X2, o->c.th_length), dn_c(o->c.th_beg)), dn_scale(dn_sin(w), o->c.h_slope));
     J2 = dn_add(dn_c(o->c.th_length), dn_scale(dn_cos(w), 2 * M_PI * o->c.h_slope));
+  } else if (o->c.coords == 4) {
+    /* COORD_DIAGONAL3 at interior zone centres (0 < x2 < 1: no reflection) */
+    const int n = o->c.diag3_exponent;
+    const double fc = o->c.diag3_factor;
+    dn e = dn_exp(X1);
+    dn f = dn_sub(dn_scale(X2, 2.), dn_c(1.));
+    r = dn_add(dn_c(o->c.R0), e);
+    J1 = e;
+    th = dn_scale(dn_add(dn_add(dn_c(1.), dn_scale(f, o->c.h_slope)), dn_scale(dn_powi(f, n), fc)), 0.5 * M_PI);
+    J2 = dn_scale(dn_add(dn_c(o->c.h_slope), dn_scale(dn_powi(f, n - 1), n * fc)), M_PI);
   } else {
     r = X1; th = X2; J1 = dn_c(1.); J2 = dn_c(1.);
   }
@@ -1709,7 +1734,7 @@ static void step_ch(orc_t* o) {
    in x coordinates (get_special_geometry(), src/metric.c:4277-4310). */
 static void ks_special(const orc_t* o, double r, geom_t* G) {
   double xp[4] = {0., 0., 0.5, 0.}, x[4], J[4][4];
-  xp[1] = (o->c.coords == 1) ? log(r - o->c.R0) : r;
+  xp[1] = (o->c.coords != 0) ? log(r - o->c.R0) : r;
   x_of_xp(o, xp, x, J);
   ks_gcov(o, x[1], x[2], G->gcov);
   ks_gcon(o, x[1], x[2], G->gcon);
@@ -1773,7 +1798,8 @@ static void cooling_tables(orc_t* o) {
 orc_t* orc_create(const orc_config* cfg) {
   if (!cfg || cfg->n1 < 1 || cfg->n2 < 1 || cfg->n3 < 1) return NULL;
   if (cfg->metric != 0 && cfg->metric != 2) return NULL;
-  if (cfg->coords != 0 && cfg->coords != 1) return NULL;
+  if (cfg->coords != 0 && cfg->coords != 1 && cfg->coords != 4) return NULL;
+  if (cfg->coords == 4 && (cfg->diag3_exponent < 3 || cfg->diag3_exponent % 2 == 0)) return NULL;
   if (cfg->recon != 2 && cfg->recon != 5) return NULL;
   if (cfg->bc != 0 && cfg->bc != 10) return NULL;
   if (cfg->cooling && cfg->metric != 2) return NULL;

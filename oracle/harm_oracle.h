@@ -21,7 +21,7 @@ extern "C" {
 typedef struct {
   int n1, n2, n3;          /* interior zones (N1, N2, N3) */
   int metric;              /* 0: METRIC_MINK_CARTESIAN, 2: METRIC_KS_SPHERICAL */
-  int coords;              /* 0: COORD_IDENTITY, 1: COORD_DIAGONAL (modified Kerr-Schild) */
+  int coords;              /* 0: COORD_IDENTITY, 1: COORD_DIAGONAL (modified Kerr-Schild), 4: COORD_DIAGONAL3 */
   int recon;               /* 2: RECON_MC, 5: RECON_PPM_MC */
   int flux;                /* 0: FLUX_LF, 1: FLUX_HLLE */
   int ct_para;             /* USE_FLUX_CT_PARA */
@@ -32,6 +32,8 @@ typedef struct {
   double gam, cour, t_final;
   double a, R0, h_slope, th_beg, th_length, r_isco, h_o_r;
   double rhomin, uumin, rhopower, uupower;
+  int diag3_exponent;      /* COORD_DIAGONAL3: theta = pi/2 (1 + h_slope f + diag3_factor f^n) */
+  double diag3_factor;
 } orc_config;
 
 typedef struct orc orc_t;

@@ -37,7 +37,8 @@ class OrcConfig(ctypes.Structure):
                 ("a", ctypes.c_double), ("R0", ctypes.c_double), ("h_slope", ctypes.c_double),
                 ("th_beg", ctypes.c_double), ("th_length", ctypes.c_double), ("r_isco", ctypes.c_double),
                 ("h_o_r", ctypes.c_double), ("rhomin", ctypes.c_double), ("uumin", ctypes.c_double),
-                ("rhopower", ctypes.c_double), ("uupower", ctypes.c_double)]
+                ("rhopower", ctypes.c_double), ("uupower", ctypes.c_double),
+                ("diag3_exponent", ctypes.c_int), ("diag3_factor", ctypes.c_double)]
 
 
 _LIB = None
@@ -110,7 +111,8 @@ def config_for(grid, **kw) -> OrcConfig:
     c.n1, c.n2, c.n3 = grid.n1, grid.n2, grid.n3
     ks = grid.metric == "kerr_schild"
     c.metric = 2 if ks else 0
-    c.coords = 1 if grid.coords == "mks" else 0
+    c.coords = {"identity": 0, "mks": 1, "diag3": 4}[grid.coords]
+    c.diag3_exponent, c.diag3_factor = grid.diag3_exponent, grid.diag3_factor
     c.recon = kw.get("recon", 5 if ks else 2)
     c.flux = kw.get("flux", 1)
     c.ct_para = kw.get("ct_para", 1 if ks else 0)

@@ -14,6 +14,8 @@
  *
  * ref_params: [0]=a [1]=Rin [2]=Rout [3]=h_slope [4]=gam [5]=cour
  *             [6]=x3 extent [7]=t_end [8]=initial dt
+ *             [9]=diag3_exponent (COORD_DIAGONAL3 builds; 0 = the reference's 9,
+ *                 src/init.nz_pn_bbh_disk.c:358)
  */
 #include "decs.h"
 
@@ -46,7 +48,9 @@ void init_base(void)
   th_beg    = th_cutout;
   th_end    = M_PI - th_cutout;
   th_length = th_end - th_beg;
-  diag3_exponent = 7;
+  /* COORD_DIAGONAL3's parameters as set_special_coord() sets them,
+     src/init.nz_pn_bbh_disk.c:357-359 (h_slope is its dtheta_min/(pi/2)) */
+  diag3_exponent = (ref_params[9] > 0.) ? (int)ref_params[9] : 9;
   diag3_factor   = 1. - 2*th_cutout/M_PI - h_slope;
 
   t = 0.;

@@ -200,4 +200,4 @@ def torus_params(grid, dt0=1.0e-5):
     """ref_params[] layout of oracle/ref_init_kerr.c for a torus GridSpec."""
     ex = grid.extra
     return [grid.a, ex["Rin"], ex["Rout"], grid.h_slope, grid.gam, grid.cour,
-            ex["x3_extent"], grid.t_end, dt0]
+            ex["x3_extent"], grid.t_end, dt0, grid.diag3_exponent]

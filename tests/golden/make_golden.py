@@ -11,6 +11,10 @@ the reference's outputs on the configurations the parity tests use:
 * c1_torus24x24x16.npz -- Fishbone-Moncrief torus, Kerr a = 0.9 in MKS
   coordinates, PPM, HLL, PPM EMFs, h/r cooling, floors 1e-7 r^-1.5 and
   1e-9 r^-2.5 (coolharm3d_amd.problems.torus, seed 1234); after 1, 10, 100 steps.
+* c1d3_torus24x24x16.npz -- the same torus on the theta map of the
+  reference's binary-disk set-up, COORD_DIAGONAL3 (src/coord.c:270-276;
+  h_slope 0.0729419909586, exponent 9, src/init.nz_pn_bbh_disk.c:332-360);
+  after 1, 10, 100 steps.
 * c0_balsara1024_t04.npz -- the shock tube run as the reference's driver runs
   it, step_ch() while t < GridLength[0] = 0.4 (src/main.c:62-81,
   src/init.balsara1.c:56): the number of steps, the state, pflag, nfail, t
@@ -189,13 +193,14 @@ def make_c0():
     np.savez_compressed(os.path.join(HERE, "c0_balsara1024.npz"), **out)
 
 
-def make_c1():
+def make_c1(coords="mks"):
     from refwrap import RefRun, torus_params
     from coolharm3d_amd.problems import torus, torus_grid
     n = (24, 24, 16)
-    g = torus_grid(*n)
+    g = torus_grid(*n, coords=coords)
     raw = torus(g)
-    R = RefRun("c1", *n, params=torus_params(g))
+    cfg, name = ("c1d3", "c1d3_torus24x24x16.npz") if coords == "diag3" else ("c1", "c1_torus24x24x16.npz")
+    R = RefRun(cfg, *n, params=torus_params(g))
     R.set_prims(raw)
     R.fixup_bounds()
     R.reset_nfail()
@@ -207,13 +212,14 @@ def make_c1():
         R.step(s - done)
         done = s
         record(R, out, f"s{s}")
-    np.savez_compressed(os.path.join(HERE, "c1_torus24x24x16.npz"), **out)
+    np.savez_compressed(os.path.join(HERE, name), **out)
 
 
 if __name__ == "__main__":
     if len(sys.argv) > 1:
-        {"c0": make_c0, "c1": make_c1, "c0f": make_c0f, "c1f": make_c1f, "c0t": make_c0_full}[sys.argv[1]]()
+        {"c0": make_c0, "c1": make_c1, "c0f": make_c0f, "c1f": make_c1f, "c0t": make_c0_full,
+         "c1d3": lambda: make_c1("diag3")}[sys.argv[1]]()
     else:
-        for cfg in ("c0", "c1", "c0f", "c1f", "c0t"):
+        for cfg in ("c0", "c1", "c0f", "c1f", "c0t", "c1d3"):
             subprocess.run([sys.executable, __file__, cfg], check=True, stdout=subprocess.DEVNULL)
             print("wrote", cfg)

@@ -29,7 +29,7 @@ if "c0" in cfg:
     R.set_time(0.0, 1.0e-5, 0.4)
 else:
     from coolharm3d_amd.problems import torus, torus_grid
-    g = torus_grid(24, 24, 16)
+    g = torus_grid(24, 24, 16, coords="diag3" if "c1d3" in cfg else "mks")
     R = RefRun(cfg, 24, 24, 16, params=torus_params(g))
     R.set_prims(torus(g))
     R.fixup_bounds()
@@ -75,8 +75,8 @@ def _check_c0(R):
         assert float(R[f"s{s}_t"]) == float(G[f"s{s}_t"]) and float(R[f"s{s}_dt"]) == float(G[f"s{s}_dt"])
 
 
-def _check_c1(R, tol100):
-    G = golden("c1_torus24x24x16.npz")
+def _check_c1(R, tol100, fixture="c1_torus24x24x16.npz"):
+    G = golden(fixture)
     assert np.array_equal(R["p0"], G["p0"])
     for s, tol in ((1, 1e-12), (10, 1e-12), (100, tol100)):
         assert_rel_l1(G[f"s{s}_prims"], interior(R[f"s{s}_prims"]), tol, f"drop-in torus after {s} steps")
@@ -117,10 +117,42 @@ def test_dropin_gpu_torus(tmp_path):
     _check_c1(_run("dropin_c1_gpu", tmp_path), 1e-9)
 
 
+@pytest.mark.slow
+@pytest.mark.skipif(not _built("dropin_c1d3_host"), reason="drop-in build not present (make -C oracle dropin)")
+def test_dropin_host_torus_diag3(tmp_path):
+    """The reference driver built with COORD_DIAGONAL3 (the binary-disk set-up's
+    theta map) and the drop-in: the shim maps the coordinates and their
+    diag3_exponent/diag3_factor globals."""
+    _check_c1(_run("dropin_c1d3_host", tmp_path), 1e-12, "c1d3_torus24x24x16.npz")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not _built("dropin_c1d3_gpu"), reason="drop-in build not present (make -C oracle dropin)")
+def test_dropin_gpu_torus_diag3(tmp_path):
+    _check_c1(_run("dropin_c1d3_gpu", tmp_path), 1e-9, "c1d3_torus24x24x16.npz")
+
+
 @pytest.mark.gpu
 @pytest.mark.skipif(not _built("dropin_c1_gpu"), reason="drop-in build not present (make -C oracle dropin)")
 def test_dropin_gpu_torus_staged(tmp_path):
     _check_c1(_run("dropin_c1_gpu", tmp_path, stages=True), 1e-9)
+
+
+DYN_OBJ = os.path.join(ROOT, "oracle", "_ref", "dropin_dyn", "step_ch_hgpu.o")
+
+
+@pytest.mark.skipif(not os.path.exists(DYN_OBJ), reason="dynamic-metric shim not compiled (make -C oracle dropin)")
+def test_dropin_maps_the_binary_metric_and_diag3():
+    """The shim compiled against the reference's headers with
+    METRIC_GENERAL_DYNAMIC + METRIC_DYNAMIC_FULLPN_NZ and COORD_DIAGONAL3 (the
+    binary-disk configuration, whose metric bodies the reference does not
+    ship, so it cannot link): it reads the binary and coordinate globals the
+    library's binary metric needs and uploads no host geometry."""
+    out = subprocess.run(["nm", DYN_OBJ], capture_output=True, text=True, check=True).stdout
+    undefined = {l.split()[-1] for l in out.splitlines() if l.split()[:1] == ["U"]}
+    for g in ("m_bh1", "m_bh2", "initial_bbh_separation", "phi_0_bbh", "diag3_exponent", "diag3_factor"):
+        assert g in undefined, g
+    assert "hgpu_upload_geometry" not in undefined and "geom_arr" not in undefined
 
 
 SYNC_SCRIPT = r"""

@@ -51,6 +51,18 @@ def test_gpu_torus_matches_reference():
     _compare(H, G, (1, 10, 100), n[0] + 6)
 
 
+def test_gpu_torus_diag3_matches_reference():
+    """The torus on the reference's binary-disk theta map, COORD_DIAGONAL3
+    (src/coord.c:270-276), against the reference build of that configuration."""
+    from coolharm3d_amd import Harm3D, torus_grid
+    G = golden("c1d3_torus24x24x16.npz")
+    n = tuple(int(x) for x in G["n"])
+    H = Harm3D(torus_grid(*n, coords="diag3"), r_isco=float(G["r_isco"]))
+    H.set_prims(G["p0"])
+    H.set_time(0.0, float(G["dt0"]))
+    _compare(H, G, (1, 10, 100), n[0] + 6)
+
+
 def test_gpu_torus_initial_fixup_matches_reference():
     from coolharm3d_amd import Harm3D, torus_grid, torus
     G = golden("c1_torus24x24x16.npz")

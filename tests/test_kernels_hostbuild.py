@@ -72,6 +72,19 @@ def test_torus_matches_reference():
     _run(H, G, (1, 10, 100), n[0] + 6)
 
 
+@pytest.mark.slow
+def test_torus_diag3_matches_reference():
+    """The torus on the COORD_DIAGONAL3 theta map (the reference's binary-disk
+    set-up's, src/coord.c:270-276) against the reference build of it."""
+    from coolharm3d_amd import Harm3D, torus_grid
+    G = golden("c1d3_torus24x24x16.npz")
+    n = tuple(int(x) for x in G["n"])
+    H = Harm3D(torus_grid(*n, coords="diag3"), hostcheck=True, r_isco=float(G["r_isco"]))
+    H.set_prims(G["p0"])
+    H.set_time(0.0, float(G["dt0"]))
+    _run(H, G, (1, 10, 100), n[0] + 6)
+
+
 def test_torus_initial_fixup_matches_reference():
     """hgpu_fixup_bounds on the generated torus equals the reference's
     fixup(p); bounds(p,0) (src/fixup.c:8, src/bounds.c:492-579)."""

@@ -81,6 +81,26 @@ def test_oracle_torus_matches_reference():
         assert O.time()["dt"] == pytest.approx(float(G[f"s{s}_dt"]), rel=1e-10)
 
 
+@pytest.mark.slow
+def test_oracle_torus_diag3_matches_reference():
+    """The torus on the reference's binary-disk theta map (COORD_DIAGONAL3,
+    src/coord.c:270-276) against the reference build of that configuration."""
+    from coolharm3d_amd.problems import torus_grid
+    G = golden("c1d3_torus24x24x16.npz")
+    n = tuple(int(x) for x in G["n"])
+    O = OracleRun(torus_grid(*n, coords="diag3"), r_isco=float(G["r_isco"]))
+    O.set_prims(G["p0"])
+    O.set_time(0.0, float(G["dt0"]))
+    done = 0
+    for s, tol in ((1, 1e-12), (10, 1e-12), (100, 1e-9)):
+        O.step(s - done)
+        done = s
+        assert_rel_l1(G[f"s{s}_prims"], interior(O.prims()), tol, f"diag3 torus after {s} steps")
+        assert np.array_equal(interior(O.pflag()), G[f"s{s}_pflag"].astype(np.int32)), f"pflag after {s}"
+        ok, info = nfail_equal_modulo_cooling_init(G[f"s{s}_nfail"], O.nfail(), n[0] + 6)
+        assert ok, info
+
+
 def test_oracle_torus_first_step_flags():
     from coolharm3d_amd.problems import torus_grid
     G = golden("c1_torus24x24x16.npz")
@@ -200,13 +220,20 @@ def test_oracle_metric_records(torus_small):
 
 
 def _ks_mks_gcov(grid, x1, x2):
-    """Textbook Kerr-Schild metric (M = 1) in modified Kerr-Schild coordinates
-    r = R0 + e^x1, th = th_length x2 + th_beg + h sin(2 pi x2)."""
+    """Textbook Kerr-Schild metric (M = 1) in the grid's numerical coordinates
+    r = R0 + e^x1 and th(x2): th_length x2 + th_beg + h sin(2 pi x2) (modified
+    Kerr-Schild) or pi/2 (1 + h f + c f^n), f = 2 x2 - 1 (COORD_DIAGONAL3),
+    written out here from those formulas."""
     a = grid.a
     r = grid.R0 + math.exp(x1)
-    th = grid.th_length * x2 + grid.th_beg + grid.h_slope * math.sin(2 * math.pi * x2)
     dr = math.exp(x1)
-    dth = grid.th_length + 2 * math.pi * grid.h_slope * math.cos(2 * math.pi * x2)
+    if grid.coords == "diag3":
+        n, c, f = grid.diag3_exponent, grid.diag3_factor, 2 * x2 - 1
+        th = 0.5 * math.pi * (1 + grid.h_slope * f + c * f ** n)
+        dth = math.pi * (grid.h_slope + n * c * f ** (n - 1))
+    else:
+        th = grid.th_length * x2 + grid.th_beg + grid.h_slope * math.sin(2 * math.pi * x2)
+        dth = grid.th_length + 2 * math.pi * grid.h_slope * math.cos(2 * math.pi * x2)
     s, c = math.sin(th), math.cos(th)
     sig = r * r + a * a * c * c
     z = 2 * r / sig
@@ -222,8 +249,16 @@ def _ks_mks_gcov(grid, x1, x2):
     return J @ g @ J
 
 
-def test_oracle_connection(torus_small):
-    g, O = torus_small
+@pytest.fixture(scope="module")
+def torus_small_diag3():
+    from coolharm3d_amd.problems import torus_grid
+    g = torus_grid(24, 24, 16, coords="diag3")
+    return g, OracleRun(g)
+
+
+@pytest.mark.parametrize("coords", ["mks", "diag3"])
+def test_oracle_connection(coords, torus_small, torus_small_diag3):
+    g, O = torus_small if coords == "mks" else torus_small_diag3
     rng = np.random.default_rng(5)
     for (i, j) in ((3, 3), (10, 12), (26, 26), (15, 4)):
         G = O.conn(i, j)

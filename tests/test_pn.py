@@ -95,14 +95,21 @@ def test_circular_orbit_helical_symmetry():
         assert np.allclose(a, b, rtol=1e-11, atol=1e-12 * np.abs(a).max())
 
 
-@pytest.mark.parametrize("inspiral", [True, False])
-def test_connection_against_finite_differences(inspiral):
+@pytest.mark.parametrize("inspiral,coords", [(True, "mks"), (False, "mks"), (True, "diag3")])
+def test_connection_against_finite_differences(inspiral, coords):
+    """The connection (metric derivatives by forward-mode differentiation)
+    against central differences of the metric in t and X, on the modified
+    Kerr-Schild and the COORD_DIAGONAL3 theta maps (h_slope 0.0729...,
+    exponent 9: the reference's binary-disk values)."""
     X, t = (math.log(40.0), 0.37, 1.3), 50.0
-    conn = PnMetric(one_zone(*X, inspiral=inspiral), hostcheck=True).connection(t).reshape(4, 4, 4)
+    kw = dict(inspiral=inspiral, coords=coords)
+    if coords == "diag3":
+        kw["h_slope"] = 0.0729419909586
+    conn = PnMetric(one_zone(*X, **kw), hostcheck=True).connection(t).reshape(4, 4, 4)
     h = [1e-2, 1e-5, 1e-5, 1e-5]
 
     def g_at(dt=0.0, dX=(0.0, 0.0, 0.0)):
-        return gcov(geom((X[0] + dX[0], X[1] + dX[1], X[2] + dX[2]), t + dt, inspiral=inspiral))
+        return gcov(geom((X[0] + dX[0], X[1] + dX[1], X[2] + dX[2]), t + dt, **kw))
 
     dg = np.empty((4, 4, 4))   # dg[c][a][b] = d_c g_ab
     dg[0] = (g_at(h[0]) - g_at(-h[0])) / (2 * h[0])
@@ -179,10 +186,11 @@ def test_gpu_metric_matches_host():
 # ---------------------------------------------------------------------------
 # the advance on the binary metric (geometry levels t, t + dt/2, t + dt per step)
 
-def _bbh_run(hostcheck, precision="fp64", steps=3, graph=False, shape=(20, 16, 24), in_kernels=None):
+def _bbh_run(hostcheck, precision="fp64", steps=3, graph=False, shape=(20, 16, 24), in_kernels=None,
+             coords="mks"):
     from coolharm3d_amd import Harm3D, bbh_grid, bbh_disk
     from coolharm3d_amd.problems import bbh_gdet
-    g = bbh_grid(*shape)
+    g = bbh_grid(*shape, coords=coords)
     p0 = bbh_disk(g, bbh_gdet(g, PnMetric, hostcheck=True))
     H = Harm3D(g, hostcheck=hostcheck, precision=precision, track_nfail=1)
     if in_kernels is not None:
@@ -197,11 +205,12 @@ def _bbh_run(hostcheck, precision="fp64", steps=3, graph=False, shape=(20, 16, 2
     return H, d0
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_bbh_step_host_build(precision):
+@pytest.mark.parametrize("precision,coords", [("fp64", "mks"), ("mixed", "mks"), ("fp64", "diag3")])
+def test_bbh_step_host_build(precision, coords):
     """A few steps of the circumbinary disk on the moving metric: finite state,
-    constrained transport keeps div B at round-off, the clock advances."""
-    H, d0 = _bbh_run(True, precision)
+    constrained transport keeps div B at round-off, the clock advances (also
+    on the reference's binary-disk theta map, COORD_DIAGONAL3)."""
+    H, d0 = _bbh_run(True, precision, coords=coords)
     p = H.prims()
     assert np.isfinite(p).all()
     bmax = np.abs(p[3:-3, 3:-3, 3:-3, 5:8]).max()
@@ -212,15 +221,15 @@ def test_bbh_step_host_build(precision):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_gpu_bbh_step_matches_host(precision):
+@pytest.mark.parametrize("precision,coords", [("fp64", "mks"), ("mixed", "mks"), ("fp64", "diag3")])
+def test_gpu_bbh_step_matches_host(precision, coords):
     """GPU vs the host build of the same bodies on the moving metric.  In FP64
     they differ only through the device's sin/cos/exp/pow in the metric; the
     mixed face solve also uses the device's single-instruction FP32
     reciprocal, square root, log2 and exp2 (about 1 ulp each, the host build
     the correctly rounded libm forms), so its bar is FP32 rounding."""
-    G, _ = _bbh_run(False, precision)
-    H, _ = _bbh_run(True, precision)
+    G, _ = _bbh_run(False, precision, coords=coords)
+    H, _ = _bbh_run(True, precision, coords=coords)
     a, b = G.prims()[3:-3, 3:-3, 3:-3].reshape(-1, 8), H.prims()[3:-3, 3:-3, 3:-3].reshape(-1, 8)
     l1 = np.abs(a - b).sum(axis=0) / np.maximum(np.abs(b).sum(axis=0), 1e-300)
     print(f"[bbh] {precision} GPU vs host rel L1 after 3 steps: {l1}", flush=True)
