@@ -86,6 +86,8 @@ SIGNATURES = {
     "hgpu_get_nfailtot": (ctypes.c_int, [ctypes.c_void_p, c_ll_p]),
     "hgpu_reset_nfail": (ctypes.c_int, [ctypes.c_void_p]),
     "hgpu_get_status": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_double_p]),
+    "hgpu_get_crash": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
+                                      ctypes.POINTER(ctypes.c_longlong)]),
     "hgpu_download_fluxes": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     "hgpu_divb_max": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     "hgpu_sync": (ctypes.c_int, [ctypes.c_void_p]),

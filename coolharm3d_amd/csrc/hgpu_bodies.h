@@ -1007,6 +1007,30 @@ HD void fixup_init_zone(const Params& P, const Ptrs& X, int i, int j, int k, Fai
 // --------------------------------------------------------------------------
 // timestep(), src/step_ch.c:546-623 (TIMESTEP_ORIGINAL); also advances t as
 // step_ch() does at src/step_ch.c:218-223.  One thread.
+// check_for_crash(), src/step_ch.c:1244-1339, once per step on the device: the
+// step's new entropy-solve failures (ntot1) and interpolation-fixup failures
+// (ntot2) from the running nfailtot words (a reset shows as a decrease: no new
+// failures), n_bad = max of the two, against nfail_fraction = 0.5 of N0 = 2
+// time levels of the block's cells (src/macros.h:36).  The reference ends the
+// run there (fail(FAIL_BASIC)); the device latches the step and n_bad, which
+// the host reads with hgpu_get_crash().
+HD void crash_check(const Params& P, const Ptrs& X) {
+  unsigned long long* st = X.status;
+  const unsigned long long n1 = st[ST_NFAILTOT + NFAIL_UTOPRIM_EE];
+  const unsigned long long n2 = st[ST_NFAILTOT + NFAIL_INTERP_PRIM] + st[ST_NFAILTOT + NFAIL_INTERP_V];
+  const unsigned long long d1 = (n1 < st[ST_CRASH_N1]) ? 0ull : n1 - st[ST_CRASH_N1];
+  const unsigned long long d2 = (n2 < st[ST_CRASH_N2]) ? 0ull : n2 - st[ST_CRASH_N2];
+  st[ST_CRASH_N1] = n1;
+  st[ST_CRASH_N2] = n2;
+  st[ST_NSTEP] += 1ull;
+  const unsigned long long n_bad = (d1 > d2) ? d1 : d2;
+  const double ncells = 2. * (double)P.n1 * (double)P.n2 * (double)P.n3;
+  if ((double)n_bad / ncells > 0.5 && st[ST_CRASH] == 0ull) {
+    st[ST_CRASH] = st[ST_NSTEP];
+    st[ST_CRASH_NBAD] = n_bad;
+  }
+}
+
 HD void timestep_update(const Params& P, const Ptrs& X) {
   double* sc = X.scal;
   sc[SC_T_OLD] = sc[SC_T];
@@ -1023,6 +1047,7 @@ HD void timestep_update(const Params& P, const Ptrs& X) {
   if (dtt > 1.3*sc[SC_DT]) dtt = 1.3*sc[SC_DT];
   if (sc[SC_T] + dtt > P.t_final) dtt = P.t_final - sc[SC_T];
   sc[SC_DT] = dtt;
+  crash_check(P, X);
 }
 
 // --------------------------------------------------------------------------

@@ -75,6 +75,11 @@ enum {
   ST_NFAILTOT = 8,           // 8..22: nfailtot[15]
   ST_SLOW = 23,              // zones queued for the full con-to-prim tree this stage
   ST_BOUNDARY_MPI = 24,      // boundary_mpi_pflag
+  ST_CRASH_N1 = 25,          // check_for_crash(): ntot1, ntot2 of the previous step
+  ST_CRASH_N2 = 26,
+  ST_NSTEP = 27,             // steps completed since the context was created
+  ST_CRASH = 28,             // step (1-based) that first crossed the crash threshold, 0 if none
+  ST_CRASH_NBAD = 29,        // its n_bad
   ST_WORDS = 32
 };
 

@@ -275,6 +275,14 @@ class Harm3D:
         self._chk(self.lib.hgpu_get_status(self.h, ctypes.byref(fe), _dp(v)), "hgpu_get_status")
         return fe.value, v
 
+
+    def crash(self):
+        """(crash_step, n_bad) of the device's check_for_crash(): the first
+        step (counted from 1) whose new failures exceed half of 2 N1 N2 N3,
+        0 when none has."""
+        st, nb = ctypes.c_longlong(), ctypes.c_longlong()
+        self._chk(self.lib.hgpu_get_crash(self.h, ctypes.byref(st), ctypes.byref(nb)), "hgpu_get_crash")
+        return int(st.value), int(nb.value)
     def fluxes(self):
         a = np.empty((3, 8) + self.shape)
         self._chk(self.lib.hgpu_download_fluxes(self.h, _dp(a)), "hgpu_download_fluxes")

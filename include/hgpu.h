@@ -215,6 +215,16 @@ int hgpu_reset_nfail(hgpu_ctx *ctx);
 /* failure_exists and max_char_speed[1..3] of the last numerical_flux(). */
 int hgpu_get_status(hgpu_ctx *ctx, int *failure_exists, double *vmax4);
 
+/* check_for_crash() (src/step_ch.c:1244-1339) evaluated on the device at the
+ * end of every step: the step's new Utoprim_1d_ee failures (ntot1) and
+ * interpolation-fixup failures (ntot2) from nfailtot, n_bad = max of the two,
+ * against half of 2 x N1 N2 N3 (N0 = 2 time levels, src/macros.h:36).  The
+ * reference ends the run at the first such step; the library keeps stepping
+ * and latches it: crash_step = that step (counted from 1 since hgpu_create),
+ * 0 if none; n_bad its count.  A decomposed run's reference compares with the
+ * global cell count; the library uses the block's. */
+int hgpu_get_crash(hgpu_ctx *ctx, long long *crash_step, long long *n_bad);
+
 /* Face fluxes of the last advance(): F[3][8][NT] (the conserved fluxes; the
  * entropy face fluxes are evaluated only for zones that take the entropy fix). */
 int hgpu_download_fluxes(hgpu_ctx *ctx, double *host);

@@ -354,3 +354,9 @@ int ref_restart_init(void)
 }
 void ref_set_n_restart(int n) { n_restart = n; }
 int ref_get_nstep(void) { return nstep; }
+
+/* The driver's per-step crash test (src/main.c:100 -> check_for_crash(),
+   src/step_ch.c:1244-1339): ends the process through fail(FAIL_BASIC) once a
+   step's new failures exceed its threshold. */
+extern void check_for_crash(void);
+void ref_check_for_crash(void) { check_for_crash(); }

@@ -168,6 +168,10 @@ class RefRun:
     def step(self, n=1):
         self.lib.ref_step(n)
 
+    def check_for_crash(self):
+        """The reference's check_for_crash(); exits the process on a crash."""
+        self.lib.ref_check_for_crash()
+
     BRANCH_NAMES = ("Utoprim_2d_fast", "Utoprim_1d", "Utoprim_1d_ee", "Utoprim_1d_ee2",
                     "fixup_interp_v", "fixup_interp_prim")
 

@@ -15,6 +15,10 @@ the reference's outputs on the configurations the parity tests use:
   reference's binary-disk set-up, COORD_DIAGONAL3 (src/coord.c:270-276;
   h_slope 0.0729419909586, exponent 9, src/init.nz_pn_bbh_disk.c:332-360);
   after 1, 10, 100 steps.
+* c0c_balsara1024_crash.npz -- the driver's crash test, check_for_crash()
+  (src/step_ch.c:1244-1339) after every step: the shock tube with rho < 0 in
+  the first 600, 520, 510 zones; the step at which the reference ends the
+  run and its n_bad (0 when it survives 5 steps).
 * c0_balsara1024_t04.npz -- the shock tube run as the reference's driver runs
   it, step_ch() while t < GridLength[0] = 0.4 (src/main.c:62-81,
   src/init.balsara1.c:56): the number of steps, the state, pflag, nfail, t
@@ -215,11 +219,55 @@ def make_c1(coords="mks"):
     np.savez_compressed(os.path.join(HERE, name), **out)
 
 
+CRASH_CASES = ((0, 600), (0, 520), (0, 510))   # zones [lo, hi) of the shock tube given rho = -1e-3
+CRASH_STEPS = 5
+
+
+def run_crash_case(lo, hi):
+    """One process: the shock tube with rho = -1e-3 in zones [lo, hi), stepped
+    with the driver's check_for_crash() after every step (src/main.c:81-100);
+    prints the step reached, the reference ends the process on a crash."""
+    from refwrap import RefRun
+    R = RefRun("c0", 1024, 1, 1)
+    R.set_time(0.0, 1.0e-5, 0.4)
+    p = R.prims()
+    p[3 + lo:3 + hi, :, :, 0] = -1.0e-3
+    R.set_prims(p)
+    R.reset_nfail()
+    for s in range(1, CRASH_STEPS + 1):
+        R.step(1)
+        R.check_for_crash()
+        print("survived", s, flush=True)
+
+
+def make_c0c():
+    import re
+    out = {"cases": np.array(CRASH_CASES), "steps": np.int64(CRASH_STEPS), "cour": np.float64(0.4),
+           "dt0": np.float64(1.0e-5)}
+    crash_step, n_bad = [], []
+    for lo, hi in CRASH_CASES:
+        r = subprocess.run([sys.executable, __file__, "c0c_run", str(lo), str(hi)], capture_output=True, text=True)
+        txt = r.stdout + r.stderr
+        m = re.search(r"Reached critical fraction.*?nstep\s*=\s*(\d+).*?n_bad\s*=\s*(\d+)", txt, re.S)
+        if m:
+            crash_step.append(int(m.group(1)))
+            n_bad.append(int(m.group(2)))
+        else:
+            assert f"survived {CRASH_STEPS}" in r.stdout, txt[-2000:]
+            crash_step.append(0)
+            n_bad.append(0)
+    out["crash_step"] = np.array(crash_step)
+    out["n_bad"] = np.array(n_bad)
+    np.savez_compressed(os.path.join(HERE, "c0c_balsara1024_crash.npz"), **out)
+
+
 if __name__ == "__main__":
-    if len(sys.argv) > 1:
-        {"c0": make_c0, "c1": make_c1, "c0f": make_c0f, "c1f": make_c1f, "c0t": make_c0_full,
+    if len(sys.argv) > 1 and sys.argv[1] == "c0c_run":
+        run_crash_case(int(sys.argv[2]), int(sys.argv[3]))
+    elif len(sys.argv) > 1:
+        {"c0c": make_c0c, "c0": make_c0, "c1": make_c1, "c0f": make_c0f, "c1f": make_c1f, "c0t": make_c0_full,
          "c1d3": lambda: make_c1("diag3")}[sys.argv[1]]()
     else:
-        for cfg in ("c0", "c1", "c0f", "c1f", "c0t", "c1d3"):
+        for cfg in ("c0", "c1", "c0f", "c1f", "c0t", "c1d3", "c0c"):
             subprocess.run([sys.executable, __file__, cfg], check=True, stdout=subprocess.DEVNULL)
             print("wrote", cfg)
