@@ -299,6 +299,8 @@ HD void cold_guess(const U2d& s, double* prim) {
   prim[U1] = prim[U2] = prim[U3] = 0.;
 }
 
+HD void c2p_start_learned(C2pCell& c, int rc_plain, const float* y);
+
 // Geometry-dependent set-up of one zone.  prim: incoming primitives (read when
 // plain_guess == 0), written with B from U and the plain guess's rho, u, u~ --
 // what the zone holds if no attempt succeeds.  Returns an early return code of
@@ -312,24 +314,47 @@ HD int c2p_begin(const double* U, const Geom* geom, double gam, double gmog, con
   if (rc) return rc;
   if (plain_guess == 1) cold_guess(c.s, prim);
   const int rc_plain = u2d_guess(c.s, geom, gam, prim, c.x_plain);
-  double x0[2];
   if (mlp) {
     float f[MLP_IN], y[MLP_OUT];
     mlp_features(c.s, f);
     mlp_forward(mlp, f, y);
-    const double lg = fmax((double)y[1], 0.);
-    const double vsq = -expm1(-2.*lg);     // 1 - 1/gamma^2
-    // a learned W close to the bracket's lower bound is nudged up by 2 % steps
-    // rather than tenfold, so a guess within a few per cent stays that close
-    u2d_guess_w(c.s, c.s.D*exp((double)y[0]), x0, vsq, 1.02, 200);
-    c.has_fallback = (rc_plain == 0);
+    c2p_start_learned(c, rc_plain, y);
   } else {
     if (rc_plain) return rc_plain;
-    x0[0] = c.x_plain[0];
-    x0[1] = c.x_plain[1];
+    u2d_newton_init(c.it, c.x_plain);
   }
-  u2d_newton_init(c.it, x0);
   return C2P_RUNNING;
+}
+
+// The two halves of c2p_begin() with the learned guess for callers that
+// evaluate the network for many zones at once (the device's wavefront-wide
+// matrix-core forward pass): c2p_prepare_learned() sets the zone up and
+// writes its features (returns an early code of Utoprim_2d_fast() or
+// C2P_RUNNING, with *rc_plain the plain guess's code; f may be null when the
+// outputs are already known); c2p_start_learned()
+// starts the Newton iteration from the network's outputs y.
+HD int c2p_prepare_learned(const double* U, const Geom* geom, double gam, double gmog, int plain_guess,
+                           double* prim, C2pCell& c, int* rc_plain, float* f) {
+  c.total = 0;
+  c.attempt = 0;
+  c.has_fallback = 0;
+  int rc = u2d_prepare(U, geom, gmog, prim, c.s, c.aux);
+  if (rc) return rc;
+  if (plain_guess == 1) cold_guess(c.s, prim);
+  *rc_plain = u2d_guess(c.s, geom, gam, prim, c.x_plain);
+  if (f) mlp_features(c.s, f);
+  return C2P_RUNNING;
+}
+
+HD void c2p_start_learned(C2pCell& c, int rc_plain, const float* y) {
+  double x0[2];
+  const double lg = fmax((double)y[1], 0.);
+  const double vsq = -expm1(-2.*lg);     // 1 - 1/gamma^2
+  // a learned W close to the bracket's lower bound is nudged up by 2 % steps
+  // rather than tenfold, so a guess within a few per cent stays that close
+  u2d_guess_w(c.s, c.s.D*exp((double)y[0]), x0, vsq, 1.02, 200);
+  c.has_fallback = (rc_plain == 0);
+  u2d_newton_init(c.it, x0);
 }
 
 // Up to `budget` Newton iterations; on convergence or failure the zone is
@@ -433,6 +458,23 @@ HD bool c2p_first_zone(const C2pArgs& A, const float* mlp, long long t, int budg
   double gm = 1.;
   int rc = c2p_begin(u, &G, A.gam, A.gmog, mlp, A.plain_guess, pr, c);
   if (rc == C2P_RUNNING) rc = c2p_advance(c, A.gam, budget, pr, &gm);
+  for (int l = 0; l < NP; l++) A.prim[l*n + t] = pr[l];
+  if (rc == C2P_RUNNING) return true;
+  A.code[t] = rc;
+  A.iters[t] = c.total;
+  return false;
+}
+
+// The rest of c2p_first_zone() once the network's outputs are known (the
+// zone's set-up returned rc, C2P_RUNNING to iterate): iterations, output.
+HD bool c2p_first_zone_finish(const C2pArgs& A, long long t, int budget, int rc, int rc_plain, const float* y,
+                              double* pr, C2pCell& c) {
+  const long long n = A.n;
+  double gm = 1.;
+  if (rc == C2P_RUNNING) {
+    c2p_start_learned(c, rc_plain, y);
+    rc = c2p_advance(c, A.gam, budget, pr, &gm);
+  }
   for (int l = 0; l < NP; l++) A.prim[l*n + t] = pr[l];
   if (rc == C2P_RUNNING) return true;
   A.code[t] = rc;

@@ -85,15 +85,190 @@ __device__ __forceinline__ long long wave_slot(bool park, unsigned* count) {
   return park ? (long long)base + __popcll(ballot & ((1ull << lane) - 1ull)) : -1;
 }
 
+// ---------------------------------------------------------------------------
+// The guess network on the matrix cores: one wavefront evaluates its 64 zones
+// with FP32 MFMA (v_mfma_f32_16x16x4_f32: A 16x4, B 4x16, C 16x16; lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds C[4 (l / 16) + r]
+// [l % 16] in element r).  Activations are kept transposed, units x zones:
+//   H1^T (64 x 64) = W1^T (64 x 8) X^T (8 x 64)         32 MFMA
+//   H2^T (64 x 64) = W2^T (64 x 64) gelu(H1^T + b1)     256 MFMA
+// so element r of an H1^T tile (unit 16 mt + 4 g + r, zone 16 nt + c, g = l / 16,
+// c = l % 16) is already lane l's B operand of the layer-2 step that sums the
+// units {16 mt + 4 g' + r : g' = 0..3}: the k-steps run over (mt, r) and the
+// A operand (W2, stored transposed in LDS) takes the same unit order.  The
+// output layer sums each lane's 16 units per zone and then across the four
+// lane groups.  FP32 throughout; the summation order and the activation's
+// exp2/reciprocal (gelu_hw) differ from the per-zone form (mlp_forward()), so
+// the outputs agree to FP32 rounding.
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+// LDS layout of the network's parameters for the wavefront-wide pass
+constexpr int NL_W2T = 0;                       // W2 transposed: [out][in]
+constexpr int NL_W1 = NL_W2T + MLP_H * MLP_H;   // W1 [in][unit]
+constexpr int NL_B1 = NL_W1 + MLP_IN * MLP_H;
+constexpr int NL_B2 = NL_B1 + MLP_H;
+constexpr int NL_W3 = NL_B2 + MLP_H;            // W3 [unit][out]
+constexpr int NL_MU = NL_W3 + MLP_H * MLP_OUT;
+constexpr int NL_ISD = NL_MU + MLP_IN;
+constexpr int NL_TAIL = NL_ISD + MLP_IN;        // b3, ymu, ysd
+constexpr int NL_SIZE = NL_TAIL + 3 * MLP_OUT;
+constexpr int NL_FEAT = MLP_IN + 1;             // padded feature row of one zone
+
+// gelu() with the hardware's single-instruction exp2 and reciprocal (about 1
+// ulp each), for the guess network's inference: x sigma(k (x + c x^3)).
+__device__ __forceinline__ float gelu_hw(float x) {
+  const float z = GELU_K * (x + GELU_C * x * x * x);
+  return x * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+
+// Every thread of the workgroup: the parameters into LDS.
+__device__ void mlp_stage(const float* __restrict__ w, float* nl) {
+  for (int e = threadIdx.x; e < MLP_H * MLP_H; e += blockDim.x) {
+    const int o = e / MLP_H, i = e - o * MLP_H;
+    nl[NL_W2T + e] = w[MLP_OFF_W2 + i * MLP_H + o];
+  }
+  for (int e = threadIdx.x; e < MLP_IN * MLP_H; e += blockDim.x) nl[NL_W1 + e] = w[MLP_OFF_W1 + e];
+  for (int e = threadIdx.x; e < MLP_H; e += blockDim.x) {
+    nl[NL_B1 + e] = w[MLP_OFF_B1 + e];
+    nl[NL_B2 + e] = w[MLP_OFF_B2 + e];
+  }
+  for (int e = threadIdx.x; e < MLP_H * MLP_OUT; e += blockDim.x) nl[NL_W3 + e] = w[MLP_OFF_W3 + e];
+  if (threadIdx.x < MLP_IN) {
+    nl[NL_MU + threadIdx.x] = w[MLP_OFF_MU + threadIdx.x];
+    nl[NL_ISD + threadIdx.x] = w[MLP_OFF_ISD + threadIdx.x];
+  }
+  if (threadIdx.x < MLP_OUT) {
+    nl[NL_TAIL + threadIdx.x] = w[MLP_OFF_B3 + threadIdx.x];
+    nl[NL_TAIL + MLP_OUT + threadIdx.x] = w[MLP_OFF_YMU + threadIdx.x];
+    nl[NL_TAIL + 2 * MLP_OUT + threadIdx.x] = w[MLP_OFF_YSD + threadIdx.x];
+  }
+}
+
+// Network outputs y of this lane's zone from its features f; every lane of
+// the wavefront calls it (lanes without a zone pass finite features).
+// feat: the wavefront's 64 x NL_FEAT LDS rows.
+__device__ void mlp_forward_wave(const float* nl, const float* f, float* y, float* feat) {
+  const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+#pragma unroll
+  for (int i = 0; i < MLP_IN; i++) feat[l * NL_FEAT + i] = (f[i] - nl[NL_MU + i]) * nl[NL_ISD + i];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  mfma_f4 h1[4][4];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+      mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < MLP_IN / 4; s++) {
+        const float a = nl[NL_W1 + (4 * s + g) * MLP_H + 16 * mt + c];
+        const float b = feat[(16 * nt + c) * NL_FEAT + 4 * s + g];
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc[r] = gelu_hw(acc[r] + nl[NL_B1 + 16 * mt + 4 * g + r]);
+      h1[mt][nt] = acc;
+    }
+  float o0[4], o1[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) { o0[nt] = 0.f; o1[nt] = 0.f; }
+#pragma unroll
+  for (int ot = 0; ot < 4; ot++) {
+    mfma_f4 acc[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) acc[nt] = (mfma_f4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      // W2^T[out 16 ot + c][in 16 mt + 4 g + r], r = 0..3: four consecutive floats
+      const mfma_f4 a4 = *reinterpret_cast<const mfma_f4*>(&nl[NL_W2T + (16 * ot + c) * MLP_H + 16 * mt + 4 * g]);
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+          acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[r], h1[mt][nt][r], acc[nt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int u = 16 * ot + 4 * g + r;
+        const float h = gelu_hw(acc[nt][r] + nl[NL_B2 + u]);
+        o0[nt] = fmaf(h, nl[NL_W3 + u * MLP_OUT + 0], o0[nt]);
+        o1[nt] = fmaf(h, nl[NL_W3 + u * MLP_OUT + 1], o1[nt]);
+      }
+  }
+  float m0 = 0.f, m1 = 0.f;
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+    float a = o0[nt], b = o1[nt];
+    a += __shfl_xor(a, 16, 64); a += __shfl_xor(a, 32, 64);
+    b += __shfl_xor(b, 16, 64); b += __shfl_xor(b, 32, 64);
+    if (nt == g) { m0 = a; m1 = b; }   // zone 16 g + c is this lane's
+  }
+  y[0] = (m0 + nl[NL_TAIL + 0]) * nl[NL_TAIL + 2 * MLP_OUT + 0] + nl[NL_TAIL + MLP_OUT + 0];
+  y[1] = (m1 + nl[NL_TAIL + 1]) * nl[NL_TAIL + 2 * MLP_OUT + 1] + nl[NL_TAIL + MLP_OUT + 1];
+}
+
+// The network's guess of every zone, one wavefront per 64 zones
+// (mlp_forward_wave()), handed to the first pass through the code and
+// iteration arrays, which the solve overwrites: y[0], y[1] as FP32 bits.  A
+// kernel of its own keeps the matrix-core pass's registers (its activations)
+// out of the Newton iteration's occupancy.
+__global__ __launch_bounds__(CB) void k_c2p_guess(C2pArgs A, const float* __restrict__ mlp) {
+  __shared__ __align__(16) float nl[NL_SIZE];
+  __shared__ float featbuf[(CB / 64) * 64 * NL_FEAT];
+  mlp_stage(mlp, nl);
+  __syncthreads();
+  const long long stride = (long long)gridDim.x * CB;
+  const long long n = A.n;
+  for (long long base = (long long)blockIdx.x * CB; base < n; base += stride) {
+    const long long t = base + threadIdx.x;
+    float f[MLP_IN], y[MLP_OUT];
+    for (int i = 0; i < MLP_IN; i++) f[i] = 0.f;
+    if (t < n) {
+      double u[NP], pr[NP];
+      for (int l = 0; l < NP; l++) u[l] = A.U[l*n + t];
+      for (int l = 0; l < NP; l++) pr[l] = (A.plain_guess == 0) ? A.prim[l*n + t] : 0.;
+      Geom G;
+      ks_record_rth(A.a, A.rth[t], A.rth[n + t], &G);
+      C2pCell c;
+      int rc_plain = 0;
+      if (c2p_prepare_learned(u, &G, A.gam, A.gmog, A.plain_guess, pr, c, &rc_plain, f) != C2P_RUNNING)
+        for (int i = 0; i < MLP_IN; i++) f[i] = 0.f;
+    }
+    mlp_forward_wave(nl, f, y, featbuf + (threadIdx.x >> 6) * 64 * NL_FEAT);
+    if (t < n) {
+      A.code[t] = __float_as_int(y[0]);
+      A.iters[t] = __float_as_int(y[1]);
+    }
+  }
+}
+
+// First pass; with MLP the network's outputs come from k_c2p_guess.
 template <bool MLP>
-__global__ __launch_bounds__(CB) void k_c2p_first(C2pArgs A, const float* __restrict__ mlp, int budget,
-                                                 C2pQueue Q, unsigned* count) {
+__global__ __launch_bounds__(CB) void k_c2p_first(C2pArgs A, int budget, C2pQueue Q, unsigned* count) {
   const long long stride = (long long)gridDim.x * CB;
   for (long long base = (long long)blockIdx.x * CB; base < A.n; base += stride) {
     const long long t = base + threadIdx.x;
     C2pCell c;
     bool park = false;
-    if (t < A.n) park = c2p_first_zone(A, MLP ? mlp : nullptr, t, budget, c);
+    if (t < A.n) {
+      if constexpr (MLP) {
+        const long long n = A.n;
+        double u[NP], pr[NP];
+        const float y[MLP_OUT] = {__int_as_float(A.code[t]), __int_as_float(A.iters[t])};
+        for (int l = 0; l < NP; l++) u[l] = A.U[l*n + t];
+        for (int l = 0; l < NP; l++) pr[l] = (A.plain_guess == 0) ? A.prim[l*n + t] : 0.;
+        Geom G;
+        ks_record_rth(A.a, A.rth[t], A.rth[n + t], &G);
+        int rc_plain = 0;
+        const int rc = c2p_prepare_learned(u, &G, A.gam, A.gmog, A.plain_guess, pr, c, &rc_plain, nullptr);
+        park = c2p_first_zone_finish(A, t, budget, rc, rc_plain, y, pr, c);
+      } else {
+        park = c2p_first_zone(A, nullptr, t, budget, c);
+      }
+    }
     if (Q.cap == 0) continue;   // one-pass solve: the budget covers every attempt
     const long long q = wave_slot(park, count);
     if (park) c2p_park(Q, q, (int)t, c);
@@ -331,8 +506,12 @@ int c2p_compacted(hipStream_t s, const C2pArgs& A, const float* mlp, int budget_
                   const C2pWork& W, unsigned G) {
   const int passes = c2p_npasses(budget_first, budget_next);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, W.count, 64);
-  if (mlp) hipLaunchKernelGGL(k_c2p_first<true>, dim3(G), dim3(CB), 0, s, A, mlp, budget_first, W.q[0], W.count);
-  else hipLaunchKernelGGL(k_c2p_first<false>, dim3(G), dim3(CB), 0, s, A, mlp, budget_first, W.q[0], W.count);
+  if (mlp) {
+    hipLaunchKernelGGL(k_c2p_guess, dim3(G), dim3(CB), 0, s, A, mlp);
+    hipLaunchKernelGGL(k_c2p_first<true>, dim3(G), dim3(CB), 0, s, A, budget_first, W.q[0], W.count);
+  } else {
+    hipLaunchKernelGGL(k_c2p_first<false>, dim3(G), dim3(CB), 0, s, A, budget_first, W.q[0], W.count);
+  }
   for (int p = 1; p <= passes; p++) {
     const int budget = (p == passes) ? C2P_MAX_ITERS : budget_next;
     hipLaunchKernelGGL(k_c2p_next, dim3(G), dim3(CB), 0, s, A, W.q[(p - 1) & 1], W.count + (p - 1), budget,
@@ -454,8 +633,12 @@ int hgpu_c2p_solve(void* stream, long long n, double a, double gam, const double
   const unsigned G = persistent_blocks(n);
   if (!compact) {
     C2pQueue none{nullptr, nullptr, nullptr, 0};
-    if (mlp) hipLaunchKernelGGL(k_c2p_first<true>, dim3(G), dim3(CB), 0, s, A, mlp, C2P_MAX_ITERS, none, nullptr);
-    else hipLaunchKernelGGL(k_c2p_first<false>, dim3(G), dim3(CB), 0, s, A, mlp, C2P_MAX_ITERS, none, nullptr);
+    if (mlp) {
+      hipLaunchKernelGGL(k_c2p_guess, dim3(G), dim3(CB), 0, s, A, mlp);
+      hipLaunchKernelGGL(k_c2p_first<true>, dim3(G), dim3(CB), 0, s, A, C2P_MAX_ITERS, none, nullptr);
+    } else {
+      hipLaunchKernelGGL(k_c2p_first<false>, dim3(G), dim3(CB), 0, s, A, C2P_MAX_ITERS, none, nullptr);
+    }
     return hipc(hipGetLastError(), "k_c2p_first");
   }
   return c2p_compacted(s, A, mlp, budget_first, budget_next, c2p_work_layout(work, n), G);

@@ -23,6 +23,7 @@ struct LaunchShape {
   unsigned occ = 0;      // 1: the register-capped kernel variant (flux, conserved update)
   unsigned fused = 0;    // conserved update only: 1 = update and inversion in one kernel
   unsigned fpt = 1;      // face fluxes only: faces per thread along the direction (1, 2, 4)
+  unsigned stage = 0;    // x3 face fluxes (FP64, PPM, fpt > 1): reconstruction from LDS-staged pencils
 };
 enum { FAM_FLUX1 = 0, FAM_FLUX2, FAM_FLUX3, FAM_EMF, FAM_CONS, FAM_INV, N_FAM };
 
@@ -241,6 +242,113 @@ __global__ __launch_bounds__(64) void k_orbit(Ptrs X, PnBinary B, PnOrbit* out) 
   if (l < 3) pn_orbit_at(B, level_time(X, l), &out[l]);
 }
 
+__device__ __forceinline__ long long wave_min_ll(long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const long long o = __shfl_xor(v, off, 64);
+    v = (o < v) ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ long long wave_max_ll(long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const long long o = __shfl_xor(v, off, 64);
+    v = (o > v) ? o : v;
+  }
+  return v;
+}
+
+// LDS capacity (doubles) of one primitive's x3 pencil of a wavefront: 64 face
+// groups of F faces need 64 F + 5 consecutive zones, plus the ghost zones
+// skipped where the wavefront's groups continue on the next (i,j) row.
+template <int F> constexpr int stage_cap() { return 64 * F + 5 + 3 * (F + 8); }
+
+// PPM+MC face states of the F consecutive x3 faces of every lane (ppm_faces()
+// with d = 2) from LDS-staged pencils.  The zones the wavefront's stencils
+// cover form one contiguous range of the array (k is its fastest index), so
+// each primitive's range is loaded once, coalesced, by all 64 lanes (about
+// F + 1 loads per lane instead of F + 5), stored to LDS, and each lane reads
+// its F + 5 window from there; the next primitive's loads are in flight
+// while this one's limiters run (double-buffered).  A wavefront whose range
+// exceeds the buffer (rows shorter than the wavefront) reads its windows from
+// global memory instead.  Same values, same arithmetic as ppm_faces(): the
+// face states are bit-identical.  Every lane of the wavefront must call it;
+// inactive lanes only help stage.
+template <int F>
+__device__ void ppm_faces_x3_staged(const Params& P, const Ptrs& X, const double* __restrict__ src, bool active,
+                                    int i, int j, int k, double (*pl)[NP], double (*pr)[NP], double* buf) {
+  constexpr int M = F + 5;
+  constexpr int CAP = stage_cap<F>();
+  constexpr int NL = (CAP + 63) / 64;
+  const long long NT = P.nt;
+  const int lane = threadIdx.x & 63;
+  int ii = i, jj = j;
+  if (P.degen[0]) ii = NG;
+  if (P.degen[1]) jj = NG;
+  const long long c0 = active ? cidx(P, ii, jj, k - 3) : 0;
+  int zm[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) zm[m] = imin(m, P.n3t - 1 - (k - 3));
+  const long long lo = wave_min_ll(active ? c0 : LLONG_MAX);
+  const long long hi = wave_max_ll(active ? c0 + zm[M - 1] : LLONG_MIN);
+  const long long span = (hi >= lo) ? hi - lo + 1 : 0;
+  const bool staged = span > 0 && span <= CAP;   // wavefront-uniform
+  const int off = (int)(c0 - lo);
+  double r[NL];
+  if (staged) {
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+      const int e = q * 64 + lane;
+      r[q] = (e < span) ? src[lo + e] : 0.;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NP; v++) {
+    double* bv = buf + (v & 1) * CAP;
+    if (staged) {
+#pragma unroll
+      for (int q = 0; q < NL; q++) {
+        const int e = q * 64 + lane;
+        if (e < span) bv[e] = r[q];
+      }
+      if (v + 1 < NP) {
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+          const int e = q * 64 + lane;
+          r[q] = (e < span) ? src[(v + 1) * NT + lo + e] : 0.;
+        }
+      }
+    }
+    // the wavefront's own buffer: LDS operations of one wavefront complete in
+    // order, so a wavefront-scope fence (no workgroup barrier) orders the
+    // stores before the other lanes' reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (active) {
+      const int s = rs_slot(v);
+      const double ra = X.rscale[s * P.n1t + ii], ri = X.irscale[s * P.n1t + ii];
+      double y[M];
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        const double raw = staged ? bv[off + zm[m]] : src[v * NT + c0 + zm[m]];
+        y[m] = (raw * ra) * ri;
+      }
+      double dq[M - 1], fi[M - 2], le[M - 2], re[M - 2];
+#pragma unroll
+      for (int m = 1; m < M - 1; m++) dq[m] = mc_slope_vect(y[m - 1], y[m], y[m + 1]);
+#pragma unroll
+      for (int m = 1; m < M - 2; m++) fi[m] = 0.5 * (y[m + 1] + y[m]) - PPM_FACTOR * (dq[m + 1] - dq[m]);
+#pragma unroll
+      for (int m = 2; m < M - 2; m++) ppm_limit(fi[m - 1], fi[m], y[m], &le[m], &re[m]);
+#pragma unroll
+      for (int f = 0; f < F; f++) {
+        pl[f][v] = re[2 + f];
+        pr[f][v] = le[3 + f];
+      }
+    }
+  }
+}
+
 // F consecutive faces along D per thread (PPM+MC, D not degenerate): the
 // thread's tile index along D counts groups of F faces from bg.lo[D]; faces
 // past hi_d (the flux box end) are skipped.  Each zone's parabola is computed
@@ -252,28 +360,38 @@ __global__ __launch_bounds__(64) void k_orbit(Ptrs X, PnBinary B, PnOrbit* out) 
 // consecutive faces.
 // OTF: the face records are evaluated from the orbit of level lvl (binary
 // metric) instead of read.
-template <int D, int F, int W, bool MIXED, bool OTF = false>
+// STG (x3, FP64): reconstruction from LDS-staged pencils (ppm_faces_x3_staged).
+template <int D, int F, int W, bool MIXED, bool OTF = false, bool STG = false>
 __global__ __launch_bounds__(BLOCK, W) void k_flux_multi(Params P, Ptrs X, const double* __restrict__ src, Box bg,
                                                          Tiling T, int hi_d, PnGrid G, int lvl) {
   using Real = typename std::conditional<MIXED, float, double>::type;
+  static_assert(!STG || (D == 2 && !MIXED && !OTF), "staged pencils: FP64 x3 faces of a static metric");
   constexpr bool TRANSPOSE = (D == 2);
   __shared__ Real tbuf[TRANSPOSE ? BLOCK * F : 1];
+  __shared__ double sbuf[STG ? (BLOCK / 64) * 2 * stage_cap<F>() : 1];
   int i, j, k;
   const bool active = tile_index(bg, T, i, j, k);
   FailRec fr; fr.clear();
   Real fo[F][NP];
   long long cb = 0;
   int nv = 0;
+  Real pl[F][NP], pr[F][NP];
   if (active) {
-    constexpr int di = (D == 0), dj = (D == 1), dk = (D == 2);
     const int lo = bg.lo[D];
     if (D == 0) i = lo + (i - lo) * F;
     else if (D == 1) j = lo + (j - lo) * F;
     else k = lo + (k - lo) * F;
+  }
+  if constexpr (STG) ppm_faces_x3_staged<F>(P, X, src, active, i, j, k, pl, pr, sbuf + (threadIdx.x >> 6) * 2 * stage_cap<F>());
+  if (active) {
+    constexpr int di = (D == 0), dj = (D == 1), dk = (D == 2);
     const int nvalid = min(F, hi_d - ((D == 0) ? i : ((D == 1) ? j : k)) + 1);
-    Real pl[F][NP], pr[F][NP];
-    if constexpr (MIXED) ppm_faces_mixed<D, F>(P, X, src, i, j, k, pl, pr);
-    else ppm_faces<D, F>(P, X, src, i, j, k, pl, pr);
+    if constexpr (STG) {
+    } else if constexpr (MIXED) {
+      ppm_faces_mixed<D, F>(P, X, src, i, j, k, pl, pr);
+    } else {
+      ppm_faces<D, F>(P, X, src, i, j, k, pl, pr);
+    }
     PnOrbit orb;
     if constexpr (OTF) orb = orbit_load(X.orbit + lvl);
 #pragma unroll
@@ -339,13 +457,20 @@ static void launch_flux_multi(const Params& P, const Ptrs& X, const LaunchShape&
   const dim3 grid(T.nblocks), blk(T.bs);
   if (P.otf) {
     // three waves per SIMD fit the FP32 face solve without spilling
-    if (MIXED) hipLaunchKernelGGL((k_flux_multi<D, F, 3, true, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
+    if constexpr (MIXED) hipLaunchKernelGGL((k_flux_multi<D, F, 3, true, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
     else hipLaunchKernelGGL((k_flux_multi<D, F, 1, false, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
     return;
   }
+  if constexpr (D == 2 && !MIXED) {
+    if (L.stage) {   // reconstruction from LDS-staged pencils
+      if (L.occ) hipLaunchKernelGGL((k_flux_multi<D, F, 3, false, false, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
+      else hipLaunchKernelGGL((k_flux_multi<D, F, 1, false, false, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
+      return;
+    }
+  }
   // register-capped variants: 4 waves/SIMD for the FP32 solve, 3 for FP64
   if (L.occ) {
-    if (MIXED) hipLaunchKernelGGL((k_flux_multi<D, F, 4, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
+    if constexpr (MIXED) hipLaunchKernelGGL((k_flux_multi<D, F, 4, true>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
     else hipLaunchKernelGGL((k_flux_multi<D, F, 3, false>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
   } else {
     hipLaunchKernelGGL((k_flux_multi<D, F, 1, MIXED>), grid, blk, 0, st, P, X, src, bg, T, b.hi[D], G, lvl);
@@ -958,13 +1083,14 @@ struct GpuBE {
   // inversion keeps its own shape (it rewrites the state) unless set.
   static unsigned fpt_code(unsigned fpt) { return fpt == 4 ? 2u : (fpt == 2 ? 1u : 0u); }
   static unsigned shape_flags(const LaunchShape& L) {
-    return L.remap | (L.order << 1) | (L.occ << 2) | (L.fused << 3) | (fpt_code(L.fpt) << 4);
+    return L.remap | (L.order << 1) | (L.occ << 2) | (L.fused << 3) | (fpt_code(L.fpt) << 4) | (L.stage << 6);
   }
   static int check_shape(const LaunchShape& L) {
     if (L.bs != 64 && L.bs != 128 && L.bs != 256) return set_error(-2, "workgroup size must be 64, 128 or 256");
     if (L.kc != 0 && (L.kc > L.bs || L.bs % L.kc != 0)) return set_error(-2, "tile k-extent must divide the workgroup size");
-    if (L.remap > 1 || L.order > 1 || L.occ > 1 || L.fused > 1)
-      return set_error(-2, "remap, order, occ and fused must be 0 or 1");
+    if (L.remap > 1 || L.order > 1 || L.occ > 1 || L.fused > 1 || L.stage > 1)
+      return set_error(-2, "remap, order, occ, fused and stage must be 0 or 1");
+    if (L.stage && L.fpt == 1) return set_error(-2, "staged pencils need 2 or 4 faces per thread");
     if (L.fpt != 1 && L.fpt != 2 && L.fpt != 4) return set_error(-2, "faces per thread must be 1, 2 or 4");
     return 0;
   }
@@ -976,7 +1102,9 @@ struct GpuBE {
     L.fused = ((unsigned)flags >> 3) & 1u;
     const unsigned fcode = ((unsigned)flags >> 4) & 3u;
     L.fpt = 1u << fcode;
-    if (bs < 0 || kc < 0 || flags < 0 || flags > 63 || fcode > 2) return set_error(-2, "bad launch-shape field");
+    L.stage = ((unsigned)flags >> 6) & 1u;
+    if (bs < 0 || kc < 0 || flags < 0 || flags > 127 || fcode > 2) return set_error(-2, "bad launch-shape field");
+    if (L.stage && fam != FAM_FLUX3) return set_error(-2, "flag bit 6 (staged pencils) applies to the x3 face fluxes only");
     if (L.fused && fam != FAM_CONS) return set_error(-2, "flag bit 3 (fused update) applies to the conserved update only");
     if (L.fpt > 1 && fam > FAM_FLUX3) return set_error(-2, "flag bits 4-5 (faces per thread) apply to the face fluxes only");
     if (int rc = check_shape(L)) return rc;
@@ -989,7 +1117,7 @@ struct GpuBE {
     const LaunchShape& L = priv(c)->shape[fam];
     if (bs) *bs = (int)L.bs;
     if (kc) *kc = (int)L.kc;
-    if (remap) *remap = (int)(L.remap | (L.order << 1) | (L.occ << 2) | (L.fused << 3) | (fpt_code(L.fpt) << 4));
+    if (remap) *remap = (int)shape_flags(L);
     return 0;
   }
   static void launch_conserved(hgpu_ctx* c, const LaunchShape& L, const Tiling& T, const Box& b, const Ptrs& X,
@@ -1093,6 +1221,19 @@ struct GpuBE {
         const float ms = time_shape(L);
         if (rc) break;
         if (ms < best_ms) { best_ms = ms; best = L; }
+      }
+      if (multi && fam == FAM_FLUX3 && !c->P.mixed && !c->P.otf) {   // LDS-staged x3 pencils
+        for (unsigned fc = 1; fc < nfpt && !rc; fc++)
+        for (unsigned oc = 0; oc < 2; oc++)
+        for (unsigned bs : bss) for (unsigned kc : kcs) for (unsigned flags = 0; flags < 4; flags++) {
+          LaunchShape L;
+          L.bs = bs; L.kc = kc; L.remap = flags & 1u; L.order = flags >> 1; L.fpt = 1u << fc;
+          L.occ = oc; L.stage = 1;
+          if (check_shape(L)) { g_last_error.clear(); continue; }
+          const float ms = time_shape(L);
+          if (rc) break;
+          if (ms < best_ms) { best_ms = ms; best = L; }
+        }
       }
       if (!rc && fam != FAM_EMF) {   // the register-capped variant of the best shape
         LaunchShape L = best;

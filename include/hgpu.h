@@ -138,7 +138,9 @@ int hgpu_set_graph(hgpu_ctx *ctx, int on);
  * register-capped kernel variant (families 0-2 and 4), bit 3 (family 4 only):
  * the conserved update and the inversion in one kernel, bits 4-5 (families
  * 0-2, PPM reconstruction): 0, 1, 2 for 1, 2, 4 consecutive faces per thread
- * sharing each zone's parabola.
+ * sharing each zone's parabola, bit 6 (family 2, FP64, 2 or 4 faces per
+ * thread): the reconstruction reads LDS-staged x3 pencils (each wavefront's
+ * zones loaded once, coalesced, into LDS) instead of per-lane stencil loads.
  * hgpu_autotune() times every candidate of families 0-4 on the current state
  * (reps launches each; family 4 as the whole predictor update, split or fused)
  * and keeps the fastest; report (NULL or [5][4]) gets

@@ -322,6 +322,31 @@ def test_gpu_mlp_gradient_matches_cpu(gpu):
 
 
 @pytest.mark.gpu
+def test_gpu_learned_solve_matches_host(gpu, host):
+    """The device evaluates the guess network on the matrix cores (a wavefront
+    per 64 zones, FP32 MFMA, hardware exp2/reciprocal in the activation); the
+    host build evaluates it zone by zone (mlp_forward()).  With the same
+    weights the guesses agree to FP32 rounding, so the solves take the same
+    iterations and reach the same solutions."""
+    w, _ = gpu.train_guess(1 << 15, 4242, A_SPIN, GAM, steps=300, batch=2048, lr=3e-3)
+    wh = host.upload(w.numpy())
+    n = 1 << 15
+    gd = gpu.samples(n, 2024, A_SPIN, GAM)
+    g = _np(gd)
+    p, c, it = (x.numpy() for x in gpu.solve(gd["U"], gd["rth"], A_SPIN, GAM, plain_guess="cold", mlp=w))
+    Uh, rh = host.upload(g["U"]), host.upload(g["rth"])
+    ph, ch, ih = (x.numpy() for x in host.solve(Uh, rh, A_SPIN, GAM, plain_guess="cold", mlp=wh, compact=False))
+    print(f"[c2p] learned: codes equal {np.mean(c == ch):.6f} iters equal {np.mean(it == ih):.6f} "
+          f"mean iters {it.mean():.4f} / {ih.mean():.4f}", flush=True)
+    assert np.mean(c == ch) >= 0.9995
+    assert np.mean(it == ih) >= 0.99
+    assert abs(it.mean() - ih.mean()) <= 0.01 * ih.mean()
+    both = (c == 0) & (ch == 0)
+    for v in range(5):
+        assert np.all(np.abs(p[v, both] - ph[v, both]) <= 1e-8 * (np.abs(ph[v, both]) + ph[0, both]))
+
+
+@pytest.mark.gpu
 def test_gpu_learned_guess_gain(gpu):
     """Trained on zones it solved itself, the network's guess lowers the mean
     Newton iterations of the cold-guess solve without adding failures, and

@@ -95,14 +95,14 @@ def test_mixed_precision_error_small_torus():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_launch_shapes_and_autotune_keep_results(precision):
-    """Workgroup size, tile shape, tile order, the register-capped variants and
-    the fused update change nothing in the results; the autotuner leaves
-    state, flags and counters as they were."""
+    """Workgroup size, tile shape, tile order, the register-capped variants,
+    LDS-staged x3 pencils and the fused update change nothing in the results;
+    the autotuner leaves state, flags and counters as they were."""
     from coolharm3d_amd import Harm3D, torus_grid, torus
     g = torus_grid(40, 32, 24)
     p0 = torus(g)
     out = []
-    for mode in ("default", "shapes", "fused", "fused_capped", "multi", "autotune"):
+    for mode in ("default", "shapes", "fused", "fused_capped", "multi", "staged", "staged_tiles", "autotune"):
         print(f"[shapes] {precision} {mode}", flush=True)
         H = Harm3D(g, track_nfail=1, precision=precision)
         H.set_prims(p0)
@@ -119,6 +119,13 @@ def test_launch_shapes_and_autotune_keep_results(precision):
             for fam, (bs, kc, flags) in zip(H.FAMILIES[:3], [(256, 0, 16 | 3), (128, 32, 32 | 1), (64, 0, 16 | 4)]):
                 H.set_launch_shape(fam, bs, kc, flags)
                 assert H.launch_shape(fam) == (bs, kc, flags)
+        if mode.startswith("staged"):
+            # x3 reconstruction from LDS-staged pencils (flag bit 6; FP64 only, ignored in
+            # mixed precision): 2 and 4 faces per thread, flat tiles (wavefronts crossing
+            # rows) and 2D tiles (ranges beyond the buffer: the global-read fallback)
+            flags3 = (256, 0, 64 | 16 | 1) if mode == "staged" else (64, 16, 64 | 32 | 4)
+            H.set_launch_shape("flux3", *flags3)
+            assert H.launch_shape("flux3") == flags3
         if mode.startswith("fused"):
             cons = (256, 0, 8) if mode == "fused" else (128, 32, 15)
             for fam, (bs, kc, flags) in zip(H.FAMILIES, [(256, 0, 4), (256, 0, 5), (128, 0, 6),
