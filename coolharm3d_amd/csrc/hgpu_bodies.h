@@ -424,8 +424,11 @@ HD void source_dyn_finish(const Params& P, const SrcDyn& s, const double* dg, lo
 // stored level X.conn; 2 binary metric, the terms themselves precomputed in
 // X.conn ([4][nconn], UU..U3, by k_source_otf, which counts their failures);
 // 3 binary metric, derivatives passed in dgl[28].
-template <int MET = -1>
-HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, int i, int j, int k,
+// UNI: every lane of the calling wavefront is in the zone's (i, j) column
+// (row-aligned tiles of a 2D static metric), so the connection record is read
+// through the scalar cache.
+template <int MET = -1, bool UNI = false, class GP = const Geom*>
+HD int source(const Params& P, const Ptrs& X, const double* ph, GP g, int i, int j, int k,
               double* dU, const double* dgl = nullptr) {
   for (int l = 0; l < NP; l++) dU[l] = 0.;
   if constexpr (MET == 2) {
@@ -433,16 +436,18 @@ HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, i
     for (int m = 0; m < 4; m++) dU[UU + m] = X.conn[m*P.nconn + cid];
     return 0;
   }
-  const bool dyn = (MET < 0) ? (P.dyn != 0) : (MET >= 1);
-  if (dyn) {   // the binary metric (3D, not flat)
-    SrcDyn s;
-    source_dyn_state(P, X, ph, g, i, s);
-    if constexpr (MET == 3) {
-      source_dyn_finish(P, s, dgl, 1, g->g, dU);
-    } else {
-      source_dyn_finish(P, s, X.conn + conn_id(P, i, j, k), P.nconn, g->g, dU);
+  if constexpr (MET != 0) {
+    const bool dyn = (MET < 0) ? (P.dyn != 0) : (MET >= 1);
+    if (dyn) {   // the binary metric (3D, not flat)
+      SrcDyn s;
+      source_dyn_state(P, X, ph, g, i, s);
+      if constexpr (MET == 3) {
+        source_dyn_finish(P, s, dgl, 1, g->g, dU);
+      } else {
+        source_dyn_finish(P, s, X.conn + conn_id(P, i, j, k), P.nconn, g->g, dU);
+      }
+      return s.gfail;
     }
-    return s.gfail;
   }
   int gfail = 0;
   State q;
@@ -460,6 +465,16 @@ HD int source(const Params& P, const Ptrs& X, const double* ph, const Geom* g, i
         // over (a, b) in the same order, the 3D test outside the loop so the
         // record's loads issue together
         const double* cn = X.conn + cid * 64;
+#ifdef __HIP_DEVICE_COMPILE__
+        if constexpr (UNI) {
+          typedef const __attribute__((address_space(4))) double* CDp;
+          const CDp cs = (CDp)(X.conn + (long long)__builtin_amdgcn_readfirstlane((int)cid) * 64);
+          for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) {
+            dU[U1] += mhd[a][b]*cs[((b*4 + 1)*4 + a)];
+            dU[U2] += mhd[a][b]*cs[((b*4 + 2)*4 + a)];
+          }
+        } else
+#endif
         for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) {
           dU[U1] += mhd[a][b]*cn[((b*4 + 1)*4 + a)];
           dU[U2] += mhd[a][b]*cn[((b*4 + 2)*4 + a)];
@@ -646,9 +661,9 @@ HD double floor_uu(const Params& P, const Ptrs& X, int i, long long c) {
 // U^{n+1} of the zone (src/step_ch.c:330-375)
 // gn: geometry of U^n (time t); g: geometry of the stage (source terms).  With a
 // static metric both are the zone's one record.
-template <bool MIXED, int MET = -1>
+template <bool MIXED, int MET = -1, bool UNI = false, class GP = const Geom*>
 HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int i, int j, int k,
-                       const Geom* gn, const Geom* g, FailRec& fr, double* U) {
+                       GP gn, GP g, FailRec& fr, double* U) {
   const long long NT = P.nt;
   const long long c = cidx(P, i, j, k);
   if (stage == 0) {
@@ -674,7 +689,7 @@ HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int
     const double* src = (stage == 0) ? X.p : X.ph;
 #pragma unroll
     for (int v = 0; v < NP; v++) phz[v] = rt_val(P, X, v, i, src[v*NT + c]);
-    fr.nf[NFAIL_GAMMA_CALC] += source<MET>(P, X, phz, g, i, j, k, dU);
+    fr.nf[NFAIL_GAMMA_CALC] += source<MET, UNI>(P, X, phz, g, i, j, k, dU);
   }
   if (MIXED) {
 #pragma unroll
@@ -716,8 +731,9 @@ HD void zone_guess(const Params& P, const Ptrs& X, int stage, int i, long long c
 // inversion succeeds with an acceptable gamma and the entropy fix is not
 // needed.  Returns false, with pf, fr and the zone untouched in memory, when
 // the zone needs the full tree.
+template <class GP>
 HD bool recover_fast(const Params& P, const Ptrs& X, int i, int j, int k, double* pf, const double* U,
-                     const Geom* g, double rhoflr, double uuflr, double* p_gamma, FailRec& fr) {
+                     GP g, double rhoflr, double uuflr, double* p_gamma, FailRec& fr) {
   double pt[NP];
 #pragma unroll
   for (int l = 0; l < NP; l++) pt[l] = pf[l];
@@ -771,12 +787,12 @@ HD void recover_zone(const Params& P, const Ptrs& X, int stage, double dt, int i
 // Whole zone in one pass (the host build's sequence; the device runs the two
 // passes).  Returns false and leaves U filled when the full tree is needed.
 // gn, g, gnew: the zone's records at U^n's level, the stage's and the result's.
-template <bool MIXED, int MET = -1>
+template <bool MIXED, int MET = -1, bool UNI = false, class GP = const Geom*>
 HD bool update_zone_fast_g(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
-                           FailRec& fr, double* U, const Geom* gn, const Geom* g, const Geom* gnew) {
+                           FailRec& fr, double* U, GP gn, GP g, GP gnew) {
   const long long c = cidx(P, i, j, k);
   const double Dt = (stage == 0) ? 0.5*dt : dt;
-  zone_conserved<MIXED, MET>(P, X, stage, Dt, i, j, k, gn, g, fr, U);
+  zone_conserved<MIXED, MET, UNI>(P, X, stage, Dt, i, j, k, gn, g, fr, U);
   double pf[NP];
   zone_guess(P, X, stage, i, c, pf);
   double pgam = 1.;
@@ -786,12 +802,21 @@ HD bool update_zone_fast_g(const Params& P, const Ptrs& X, int stage, double dt,
   return true;
 }
 
-template <bool MIXED, int MET = -1>
+template <bool MIXED, int MET = -1, bool UNI = false>
 HD bool update_zone_fast(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
                          FailRec& fr, double* U) {
   const int gid = geom_id(P, i, j, k, CENT);
-  return update_zone_fast_g<MIXED, MET>(P, X, stage, dt, i, j, k, fr, U, X.geom_n + gid, X.geom + gid,
-                                        X.geom_new + gid);
+#ifdef __HIP_DEVICE_COMPILE__
+  if constexpr (UNI) {
+    // the wavefront's zones share one record: read through the scalar cache
+    typedef const __attribute__((address_space(4))) Geom* CGp;
+    const int gu = __builtin_amdgcn_readfirstlane(gid);
+    return update_zone_fast_g<MIXED, MET, UNI>(P, X, stage, dt, i, j, k, fr, U, (CGp)(X.geom_n + gu),
+                                               (CGp)(X.geom + gu), (CGp)(X.geom_new + gu));
+  }
+#endif
+  return update_zone_fast_g<MIXED, MET, UNI>(P, X, stage, dt, i, j, k, fr, U, X.geom_n + gid, X.geom + gid,
+                                             X.geom_new + gid);
 }
 
 template <bool MIXED>

@@ -576,7 +576,9 @@ __global__ __launch_bounds__(BLOCK) void k_invert(Params P, Ptrs X, int stage, B
 // predictor) and the result's (lvl + 1) are evaluated from the orbit.
 // MET: 0 static metric, 1 binary metric (stored levels or, with OTF,
 // evaluated records); each kernel carries only its metric's source branch.
-template <bool MIXED, int W, bool OTF = false, int MET = 0>
+// UNI: row-aligned tiles (each wavefront in one (i, j) column of a 2D static
+// metric): the connection records are read through the scalar cache.
+template <bool MIXED, int W, bool OTF = false, int MET = 0, bool UNI = false>
 __global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage, Box b, Tiling T, PnGrid G,
                                                      int lvl) {
   int i, j, k;
@@ -608,7 +610,7 @@ __global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage
       done = recover_fast(P, X, i, j, k, pf, U, &gnew, floor_rho(P, X, i, c), floor_uu(P, X, i, c), &pgam, fr);
       if (done) store_zone(P, X, stage, c, pf, pgam, fr);
     } else {
-      done = update_zone_fast<MIXED, MET>(P, X, stage, dt, i, j, k, fr, U);
+      done = update_zone_fast<MIXED, MET, UNI>(P, X, stage, dt, i, j, k, fr, U);
     }
     if (!done) {
       queue = true;
@@ -1147,12 +1149,24 @@ struct GpuBE {
       else hipLaunchKernelGGL((k_update<false, 1, false, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
       return;
     }
+    // row-aligned flat tiles: every wavefront's 64 zones share one (i, j)
+    const bool uni = c->P.metric_dim == 2 && !c->P.flat && T.kc == 0 && T.nk % 64 == 0 && T.bs % 64 == 0;
     if (c->P.mixed) {
-      if (L.occ) hipLaunchKernelGGL((k_update<true, 3>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
-      else hipLaunchKernelGGL((k_update<true, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      if (uni) {
+        if (L.occ) hipLaunchKernelGGL((k_update<true, 3, false, 0, true>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+        else hipLaunchKernelGGL((k_update<true, 1, false, 0, true>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      } else {
+        if (L.occ) hipLaunchKernelGGL((k_update<true, 3>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+        else hipLaunchKernelGGL((k_update<true, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      }
     } else {
-      if (L.occ) hipLaunchKernelGGL((k_update<false, 3>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
-      else hipLaunchKernelGGL((k_update<false, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      if (uni) {
+        if (L.occ) hipLaunchKernelGGL((k_update<false, 3, false, 0, true>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+        else hipLaunchKernelGGL((k_update<false, 1, false, 0, true>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      } else {
+        if (L.occ) hipLaunchKernelGGL((k_update<false, 3>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+        else hipLaunchKernelGGL((k_update<false, 1>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+      }
     }
   }
   // The autotuner times the conserved-update family as the whole predictor

@@ -60,14 +60,21 @@ HD void atomic_or_u64(unsigned long long* a, unsigned long long v) {
 HD bool is_finite(double x) { return isfinite(x); }
 
 // ---------------------------------------------------------------------------
+// The functions of the per-zone path that read a geometry record take its
+// pointer as a template parameter GP: a plain `const Geom*`, or on the device a
+// pointer into the constant address space when every lane of the wavefront
+// reads the same record (its fields then load through the scalar cache).
+
 // Metric contractions, src/metric.c:338-383
-HD void lower(const double* ucon, const Geom* g, double* ucov) {
+template <class GP>
+HD void lower(const double* ucon, GP g, double* ucov) {
   ucov[0] = g->gcov[0][0]*ucon[0] + g->gcov[0][1]*ucon[1] + g->gcov[0][2]*ucon[2] + g->gcov[0][3]*ucon[3];
   ucov[1] = g->gcov[1][0]*ucon[0] + g->gcov[1][1]*ucon[1] + g->gcov[1][2]*ucon[2] + g->gcov[1][3]*ucon[3];
   ucov[2] = g->gcov[2][0]*ucon[0] + g->gcov[2][1]*ucon[1] + g->gcov[2][2]*ucon[2] + g->gcov[2][3]*ucon[3];
   ucov[3] = g->gcov[3][0]*ucon[0] + g->gcov[3][1]*ucon[1] + g->gcov[3][2]*ucon[2] + g->gcov[3][3]*ucon[3];
 }
-HD void raise_v(const double* ucov, const Geom* g, double* ucon) {
+template <class GP>
+HD void raise_v(const double* ucov, GP g, double* ucon) {
   ucon[0] = g->gcon[0][0]*ucov[0] + g->gcon[0][1]*ucov[1] + g->gcon[0][2]*ucov[2] + g->gcon[0][3]*ucov[3];
   ucon[1] = g->gcon[1][0]*ucov[0] + g->gcon[1][1]*ucov[1] + g->gcon[1][2]*ucov[2] + g->gcon[1][3]*ucov[3];
   ucon[2] = g->gcon[2][0]*ucov[0] + g->gcon[2][1]*ucov[1] + g->gcon[2][2]*ucov[2] + g->gcon[2][3]*ucov[3];
@@ -78,7 +85,8 @@ HD double dot4(const double* a, const double* b) {
 }
 
 // gamma_calc(), src/phys.c:244-268.  Returns 1 on failure (gamma = 1).
-HD int gamma_calc(const double* pr, const Geom* g, double* gamma) {
+template <class GP>
+HD int gamma_calc(const double* pr, GP g, double* gamma) {
   double qsq = g->gcov[1][1]*pr[U1]*pr[U1]
              + g->gcov[2][2]*pr[U2]*pr[U2]
              + g->gcov[3][3]*pr[U3]*pr[U3]
@@ -95,7 +103,8 @@ HD int gamma_calc(const double* pr, const Geom* g, double* gamma) {
 
 // get_state(), src/phys.c:199-241 (ucon_calc, lower, bcon_calc, lower).
 // Returns the gamma_calc failure flag (the caller records FAIL_GAMMA_CALC).
-HD int get_state(const double* pr, const Geom* g, double gam, State* q) {
+template <class GP>
+HD int get_state(const double* pr, GP g, double gam, State* q) {
   double gamma;
   int fl = gamma_calc(pr, g, &gamma);
   q->ucon[0] = gamma*g->ncon[0];
@@ -128,7 +137,8 @@ HD void mhd_calc(const State* q, int dir, double* mhd) {
 }
 
 // primtoflux(), src/phys.c:28-63
-HD void primtoflux(const double* pr, const State* q, int dir, const Geom* g, double* flux) {
+template <class GP>
+HD void primtoflux(const double* pr, const State* q, int dir, GP g, double* flux) {
   double mhd[4];
   mhd_calc(q, dir, mhd);
   flux[RHO] = pr[RHO]*q->ucon[dir];
@@ -333,7 +343,8 @@ struct U2dAux {
 // Utoprim_2d_fast() + Utoprim_new_body() up to the initial guess
 // (src/utoprim_2d_fast.c:155-300): the invariants of U the Newton solve works
 // with.  Sets prim's B components.  Returns -100 when U[RHO] <= 0.
-HD int u2d_prepare(const double* U, const Geom* geom, double gmog, double* prim, U2d& s, U2dAux& aux) {
+template <class GP>
+HD int u2d_prepare(const double* U, GP geom, double gmog, double* prim, U2d& s, U2dAux& aux) {
   if (U[0] <= 0.) return -100;
   double inv_gdet = geom->g_inv;
   double alpha = geom->alpha;
@@ -392,7 +403,8 @@ HD void u2d_guess_w(const U2d& s, double W_last, double* x2d, double vsq_guess =
 
 // The reference's initial guess from the incoming primitives' u and u~
 // (src/utoprim_2d_fast.c:302-330).  Returns 2 when their u~^2 is unusable.
-HD int u2d_guess(const U2d& s, const Geom* geom, double gam, const double* prim, double* x2d) {
+template <class GP>
+HD int u2d_guess(const U2d& s, GP geom, double gam, const double* prim, double* x2d) {
   double utsq = geom->gcov[1][1]*prim[U1]*prim[U1]
               + geom->gcov[2][2]*prim[U2]*prim[U2]
               + geom->gcov[3][3]*prim[U3]*prim[U3]
@@ -411,7 +423,8 @@ HD int u2d_guess(const U2d& s, const Geom* geom, double gam, const double* prim,
   return 0;
 }
 
-HD int u2d_setup(const double* U, const Geom* geom, double gam, double gmog, double* prim, U2d& s,
+template <class GP>
+HD int u2d_setup(const double* U, GP geom, double gam, double gmog, double* prim, U2d& s,
                  U2dAux& aux, double* x2d) {
   int rc = u2d_prepare(U, geom, gmog, prim, s, aux);
   if (rc) return rc;
@@ -450,7 +463,8 @@ HD int u2d_finish(const U2d& s, const U2dAux& aux, double gam, const double* x2d
 // gmog = (gam-1)/gam (gam_m1_o_gam, src/main.c:299).
 // n_extra: confirming iterations after convergence (the reference's
 // EXTRA_NEWT_ITER unless the caller asks for fewer).
-HD int utoprim_2d_fast(const double* U, const Geom* geom, double gam, double gmog, double* prim,
+template <class GP>
+HD int utoprim_2d_fast(const double* U, GP geom, double gam, double gmog, double* prim,
                        double* gamma_out, double* bsq_out, int* iters, int n_extra = EXTRA_NEWT_ITER) {
   if (iters) *iters = 0;
   U2d s;

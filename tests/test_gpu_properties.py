@@ -99,7 +99,9 @@ def test_launch_shapes_and_autotune_keep_results(precision):
     LDS-staged x3 pencils and the fused update change nothing in the results;
     the autotuner leaves state, flags and counters as they were."""
     from coolharm3d_amd import Harm3D, torus_grid, torus
-    g = torus_grid(40, 32, 24)
+    # 64 zones along x3: the flat-tile shapes run the update's row-aligned
+    # variant (connection records through the scalar cache), the 2D tiles not
+    g = torus_grid(40, 32, 64)
     p0 = torus(g)
     out = []
     for mode in ("default", "shapes", "fused", "fused_capped", "multi", "staged", "staged_tiles", "autotune"):
