@@ -330,13 +330,15 @@ def bbh_report(args):
     el = time.perf_counter() - t0
     divb = H.divb_max()
     finite = bool(np.isfinite(H.prims()).all())
+    dev_gb = H.device_bytes() / 1e9
     H.close()
     ncell = shape[0] * shape[1] * shape[2]
     return {"workload": f"bbh_disk_{shape[0]}x{shape[1]}x{shape[2]}_mixed", "value": ncell * args.bbh_steps / el,
             "unit": "zone-cycles/s", "ms_per_step": el / args.bbh_steps * 1e3, "steps": args.bbh_steps,
             "steps_total": 5 + args.warmup + args.bbh_steps,
             "kernel_ms_per_step_eager": kern, "launch_shapes": shapes, "graph": args.graph == "on", "finite": finite,
-            "divb_max": divb, "metric": "1PN near-zone, quasi-circular inspiral (Peters), separation 20 M"}
+            "divb_max": divb, "device_gb": dev_gb,
+            "metric": "1PN near-zone, quasi-circular inspiral (Peters), separation 20 M"}
 
 
 def timed_run(args, g, p0, precision):

@@ -36,9 +36,13 @@ struct hgpu_ctx {
   hgpu::Params P;
   hgpu::Ptrs X;
   // time-dependent metric: geometry levels at t, t + dt/2, t + dt, their FP32
-  // copies (t, t + dt/2) and connections (t, t + dt/2); level 0 is X.geom's buffer
+  // copies (t, t + dt/2) and connections (t, t + dt/2); level 0 is X.geom's buffer.
+  // The binary metric evaluated in the kernels reads no geometry level: its
+  // levels are allocated when first needed (lev_alloc).
   hgpu::Geom* lev[3] = {nullptr, nullptr, nullptr};
   hgpu::GeomF* levf[2] = {nullptr, nullptr};
+  bool lev_alloc = false;
+  long long dev_bytes = 0;         // device memory the context's allocations requested
   double* connlev[2] = {nullptr, nullptr};
   hgpu::PnGrid pn;
   hgpu::PnOrbit* orbit = nullptr;  // device: the orbit at levels t, t + dt/2, t + dt
@@ -235,6 +239,31 @@ inline void soa_to_aos(const double* soa, double* aos, long long nt, int nv) {
 
 template <class BE>
 struct Api {
+  static int dalloc(hgpu_ctx* c, void** p, size_t bytes) {
+    const int rc = BE::alloc(c, p, bytes);
+    if (!rc) c->dev_bytes += (long long)bytes;
+    return rc;
+  }
+  // Geometry records in memory: the static metric's cache (level 0, FP32 copy
+  // in mixed precision), or the binary metric's three levels and two FP32 face
+  // copies.  A binary-metric context evaluating the metric in its kernels
+  // allocates them here when it first needs them (the stored mode, a geometry
+  // download); the level pointers of the current state are set again.
+  static int alloc_levels(hgpu_ctx* c) {
+    if (c->lev_alloc) return 0;
+    const Params& P = c->P;
+    const int nl = P.dyn ? 3 : 1, nlf = P.dyn ? 2 : 1;
+    int rc = 0;
+    for (int l = 0; l < nl; l++) rc |= dalloc(c, (void**)&c->lev[l], sizeof(double) * GEOM_DOUBLES * c->n_geom);
+    for (int l = 0; l < nlf; l++) rc |= dalloc(c, (void**)&c->levf[l], sizeof(GeomF) * (P.mixed ? c->n_geom : 1));
+    if (rc) return set_error(-3, "device allocation of the geometry records failed: " + g_last_error);
+    c->lev_alloc = true;
+    Ptrs& X = c->X;
+    const int l = P.dyn ? X.cent_lvl : 0;
+    X.geom = X.geom_n = X.geom_new = c->lev[l];
+    X.geomf = c->levf[l < nlf ? l : nlf - 1];
+    return 0;
+  }
   static int create(const hgpu_config* cfg, hgpu_ctx** out) {
     if (!cfg || !out) return set_error(-1, "null argument");
     hgpu_ctx* c = new hgpu_ctx();
@@ -261,48 +290,43 @@ struct Api {
     Ptrs& X = c->X;
     const long long NT = P.nt;
     rc = 0;
-    rc |= BE::alloc(c, (void**)&X.p, sizeof(double) * NP * NT);
-    rc |= BE::alloc(c, (void**)&X.ph, sizeof(double) * NP * NT);
+    rc |= dalloc(c, (void**)&X.p, sizeof(double) * NP * NT);
+    rc |= dalloc(c, (void**)&X.ph, sizeof(double) * NP * NT);
     const long long nF = P.mixed ? 1 : 3 * NFLUX * NT, nE = P.mixed ? 1 : 3 * NT;
-    rc |= BE::alloc(c, (void**)&X.F, sizeof(double) * nF);
-    rc |= BE::alloc(c, (void**)&X.emf, sizeof(double) * nE);
-    rc |= BE::alloc(c, (void**)&X.Ff, sizeof(float) * (P.mixed ? 3 * NFLUX * NT : 2));
-    rc |= BE::alloc(c, (void**)&X.emff, sizeof(float) * (P.mixed ? 3 * NT : 2));
-    rc |= BE::alloc(c, (void**)&c->levf[0], sizeof(GeomF) * (P.mixed ? c->n_geom : 1));
-    rc |= BE::alloc(c, (void**)&c->lev[0], sizeof(double) * GEOM_DOUBLES * c->n_geom);
+    rc |= dalloc(c, (void**)&X.F, sizeof(double) * nF);
+    rc |= dalloc(c, (void**)&X.emf, sizeof(double) * nE);
+    rc |= dalloc(c, (void**)&X.Ff, sizeof(float) * (P.mixed ? 3 * NFLUX * NT : 2));
+    rc |= dalloc(c, (void**)&X.emff, sizeof(float) * (P.mixed ? 3 * NT : 2));
     // connection records [zone][64] of a static metric; derivative levels
     // [PN_NDG][zone] of the binary metric
     const long long nconn_rec = P.dyn ? PN_NDG : 64;
-    rc |= BE::alloc(c, (void**)&c->connlev[0], sizeof(double) * nconn_rec * c->n_conn);
-    X.geom = c->lev[0];
-    X.geomf = c->levf[0];
+    rc |= dalloc(c, (void**)&c->connlev[0], sizeof(double) * nconn_rec * c->n_conn);
     X.conn = c->connlev[0];
     if (P.dyn) {
-      for (int l = 1; l < 3; l++) rc |= BE::alloc(c, (void**)&c->lev[l], sizeof(double) * GEOM_DOUBLES * c->n_geom);
-      rc |= BE::alloc(c, (void**)&c->levf[1], sizeof(GeomF) * (P.mixed ? c->n_geom : 1));
-      rc |= BE::alloc(c, (void**)&c->connlev[1], sizeof(double) * nconn_rec * c->n_conn);
-      rc |= BE::alloc(c, (void**)&c->orbit, sizeof(PnOrbit) * 3);
+      rc |= dalloc(c, (void**)&c->connlev[1], sizeof(double) * nconn_rec * c->n_conn);
+      rc |= dalloc(c, (void**)&c->orbit, sizeof(PnOrbit) * 3);
       X.orbit = c->orbit;
-      rc |= BE::alloc(c, (void**)&c->pn_tab, sizeof(double) * pn_tab_size(P));
+      rc |= dalloc(c, (void**)&c->pn_tab, sizeof(double) * pn_tab_size(P));
     }
-    X.geom_n = X.geom_new = X.geom;
-    rc |= BE::alloc(c, (void**)&X.omega, sizeof(double) * P.n1t);
-    rc |= BE::alloc(c, (void**)&X.tr, sizeof(double) * P.n1t);
-    rc |= BE::alloc(c, (void**)&X.rscale, sizeof(double) * 6 * P.n1t);
-    rc |= BE::alloc(c, (void**)&X.irscale, sizeof(double) * 6 * P.n1t);
-    rc |= BE::alloc(c, (void**)&X.rhoflr, sizeof(double) * T.rhoflr.size());
-    rc |= BE::alloc(c, (void**)&X.uuflr, sizeof(double) * T.uuflr.size());
-    rc |= BE::alloc(c, (void**)&X.gamma, sizeof(double) * NT);
+    rc |= dalloc(c, (void**)&X.omega, sizeof(double) * P.n1t);
+    rc |= dalloc(c, (void**)&X.tr, sizeof(double) * P.n1t);
+    rc |= dalloc(c, (void**)&X.rscale, sizeof(double) * 6 * P.n1t);
+    rc |= dalloc(c, (void**)&X.irscale, sizeof(double) * 6 * P.n1t);
+    rc |= dalloc(c, (void**)&X.rhoflr, sizeof(double) * T.rhoflr.size());
+    rc |= dalloc(c, (void**)&X.uuflr, sizeof(double) * T.uuflr.size());
+    rc |= dalloc(c, (void**)&X.gamma, sizeof(double) * NT);
     const long long nint = (long long)P.n1 * P.n2 * P.n3;
-    rc |= BE::alloc(c, (void**)&X.Uslow, sizeof(double) * NP * NT);
-    rc |= BE::alloc(c, (void**)&X.Un, sizeof(double) * NP * NT);
-    rc |= BE::alloc(c, (void**)&X.slow_idx, sizeof(unsigned) * nint);
-    rc |= BE::alloc(c, (void**)&X.pflag, sizeof(int) * NT);
-    rc |= BE::alloc(c, (void**)&X.nfail, sizeof(int) * (P.track_nfail ? N_NFAIL * NT : 1));
-    rc |= BE::alloc(c, (void**)&X.status, sizeof(unsigned long long) * ST_WORDS);
-    rc |= BE::alloc(c, (void**)&X.scal, sizeof(double) * SC_WORDS);
+    rc |= dalloc(c, (void**)&X.Uslow, sizeof(double) * NP * NT);
+    rc |= dalloc(c, (void**)&X.Un, sizeof(double) * NP * NT);
+    rc |= dalloc(c, (void**)&X.slow_idx, sizeof(unsigned) * nint);
+    rc |= dalloc(c, (void**)&X.pflag, sizeof(int) * NT);
+    rc |= dalloc(c, (void**)&X.nfail, sizeof(int) * (P.track_nfail ? N_NFAIL * NT : 1));
+    rc |= dalloc(c, (void**)&X.status, sizeof(unsigned long long) * ST_WORDS);
+    rc |= dalloc(c, (void**)&X.scal, sizeof(double) * SC_WORDS);
     if (rc) { destroy(c); return set_error(-3, "device allocation failed: " + g_last_error); }
-    rc |= BE::h2d(c, (void*)X.geom, T.geom.data(), sizeof(double) * T.geom.size());
+    if (!c->P.otf) rc |= alloc_levels(c);
+    if (rc) { destroy(c); return rc; }
+    if (!T.geom.empty()) rc |= BE::h2d(c, (void*)X.geom, T.geom.data(), sizeof(double) * T.geom.size());
     if (P.dyn) {
       // coordinate functions along each direction, evaluated once here and
       // read by every record evaluation of the binary metric
@@ -487,6 +511,8 @@ struct Api {
                            "reconstruction, no degenerate direction, on the GPU build");
     int rc = BE::invalidate_graph(c);
     const bool was = c->P.otf != 0;
+    if (!on) rc |= alloc_levels(c);
+    if (rc) return rc;
     c->P.otf = on ? 1 : 0;
     // leaving the evaluated mode: the state's centre level must exist in memory
     if (was && !on && c->P.dyn) rc |= BE::fill_geometry(c, const_cast<Geom*>(c->X.geom), nullptr, c->X.cent_lvl, 1u << CENT, 0u);
@@ -496,6 +522,8 @@ struct Api {
   // none): for hgpu_download_geometry.
   static int materialize_state_geometry(hgpu_ctx* c) {
     if (!c->P.dyn || !c->P.otf) return 0;
+    const int rc = alloc_levels(c);
+    if (rc) return rc;
     return BE::fill_geometry(c, const_cast<Geom*>(c->X.geom), nullptr, c->X.cent_lvl, 1u << CENT, 0u);
   }
   // Metric derivatives of the last stage's level (the evaluated mode keeps the

@@ -164,6 +164,12 @@ class Harm3D:
         self._chk(self.lib.hgpu_get_metric_eval(self.h, _ip(v)), "hgpu_get_metric_eval")
         return bool(v[0])
 
+    def device_bytes(self):
+        """Device memory the context holds (hgpu_device_bytes)."""
+        v = ctypes.c_longlong()
+        self._chk(self.lib.hgpu_device_bytes(self.h, ctypes.byref(v)), "hgpu_device_bytes")
+        return int(v.value)
+
     FAMILIES = ("flux1", "flux2", "flux3", "emf", "conserved", "invert")
 
     def autotune(self, reps=2):

@@ -158,10 +158,18 @@ int hgpu_get_launch_shape(hgpu_ctx *ctx, int family, int *bs, int *kc, int *flag
  * of their own from the metric derivatives; 0 fills and reads stored levels
  * (records and metric derivatives).  Both give the same results bit for bit.
  * Default 1 where possible: the GPU build with PPM reconstruction and no
- * degenerate direction (an error is returned otherwise).  No counterpart in
- * the reference (whose binary metric bodies are absent). */
+ * degenerate direction (an error is returned otherwise).  The stored levels
+ * (three positions-by-zone record levels and, in mixed precision, two FP32
+ * face copies) are allocated when first needed: on the switch to 0, or by
+ * hgpu_download_geometry.  No counterpart in the reference (whose binary
+ * metric bodies are absent). */
 int hgpu_set_metric_eval(hgpu_ctx *ctx, int in_kernels);
 int hgpu_get_metric_eval(hgpu_ctx *ctx, int *in_kernels);
+
+/* Device memory the context holds: the bytes its allocations requested
+ * (state, face fluxes, queues, geometry and metric buffers).  No counterpart
+ * in the reference. */
+int hgpu_device_bytes(hgpu_ctx *ctx, long long *bytes);
 
 /* One advance() stage (0: predictor into ph, 1: corrector into p) followed by
  * bounds and fixups, as step_ch() sequences them (src/step_ch.c:63-121,

@@ -247,6 +247,9 @@ def test_metric_eval_modes_host_build():
     with pytest.raises(HgpuError, match="evaluated in the kernels"):
         H.set_metric_eval(True)
     H.set_metric_eval(False)
+    # the stored mode holds its three record levels from the start
+    ng, _ = H.geometry_counts()
+    assert H.device_bytes() > 3 * 42 * 8 * ng
 
 
 @pytest.mark.gpu
@@ -258,6 +261,13 @@ def test_gpu_bbh_metric_in_kernels_matches_stored(precision):
     A, _ = _bbh_run(False, precision, steps=3, in_kernels=True)
     B, _ = _bbh_run(False, precision, steps=3, in_kernels=False)
     assert A.metric_eval() and not B.metric_eval()
+    # the evaluated mode has not allocated the stored levels: three FP64
+    # record levels (42 doubles a record) and two FP32 face copies (43 floats:
+    # the same fields and 1 - alpha; one record each outside mixed precision)
+    ng, _ = A.geometry_counts()
+    levels = 3 * 42 * 8 * ng + 2 * 43 * 4 * (ng if precision == "mixed" else 1)
+    print(f"[bbh] device bytes: evaluated {A.device_bytes()}, stored {B.device_bytes()}", flush=True)
+    assert B.device_bytes() - A.device_bytes() == levels
     assert np.array_equal(A.prims(), B.prims())
     assert np.array_equal(A.pflag(), B.pflag())
     assert np.array_equal(A.nfail(), B.nfail())
@@ -269,6 +279,11 @@ def test_gpu_bbh_metric_in_kernels_matches_stored(precision):
     cent = slice(3 * A.nt, 4 * A.nt)   # position 3 (zone centres) of [5][NT]
     assert np.array_equal(ga[cent], gb[cent])
     assert np.array_equal(da, db)
+    assert A.device_bytes() == B.device_bytes()   # the download allocated them
+    # and the evaluated mode still steps the same after the allocation
+    A.step_ch(1)
+    B.step_ch(1)
+    assert np.array_equal(A.prims(), B.prims())
 
 
 @pytest.mark.gpu
