@@ -666,17 +666,26 @@ HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int
                        GP gn, GP g, FailRec& fr, double* U) {
   const long long NT = P.nt;
   const long long c = cidx(P, i, j, k);
-  if (stage == 0) {
-    // U^n = primtoflux(p_i) with geom_i (src/step_ch.c:319, :339-341), kept for
-    // the corrector as KEEP_CONSERVED_VARS keeps U_gf (src/step_ch.c:321-333)
+  // U^n = primtoflux(p_i) with geom_i (src/step_ch.c:319, :339-341): kept for
+  // the corrector as KEEP_CONSERVED_VARS keeps U_gf (src/step_ch.c:321-333), or,
+  // in the mixed-precision step on a static metric, formed again by the
+  // corrector from p^n (which it reads for the Newton guess anyway): the same
+  // operations on the same values, so the same bits, for 128 B less traffic
+  // per zone and step.  Its gamma_calc failure is counted once, as the
+  // reference counts it, in the predictor.
+  const bool keep_un = !MIXED || ((MET < 0) ? P.dyn != 0 : MET != 0);
+  if (stage == 0 || !keep_un) {
     double pi[NP];
 #pragma unroll
     for (int v = 0; v < NP; v++) pi[v] = rt_val(P, X, v, i, X.p[v*NT + c]);
     State q;
-    if (get_state(pi, gn, P.gam, &q)) fr.nf[NFAIL_GAMMA_CALC]++;
+    const int gfail = get_state(pi, gn, P.gam, &q);
+    if (gfail && stage == 0) fr.nf[NFAIL_GAMMA_CALC]++;
     primtoflux(pi, &q, 0, gn, U);
+    if (stage == 0 && keep_un) {
 #pragma unroll
-    for (int v = 0; v < NP; v++) X.Un[v*NT + c] = U[v];
+      for (int v = 0; v < NP; v++) X.Un[v*NT + c] = U[v];
+    }
   } else {
 #pragma unroll
     for (int v = 0; v < NP; v++) U[v] = X.Un[v*NT + c];

@@ -317,7 +317,8 @@ struct Api {
     rc |= dalloc(c, (void**)&X.gamma, sizeof(double) * NT);
     const long long nint = (long long)P.n1 * P.n2 * P.n3;
     rc |= dalloc(c, (void**)&X.Uslow, sizeof(double) * NP * NT);
-    rc |= dalloc(c, (void**)&X.Un, sizeof(double) * NP * NT);
+    // U^n for the corrector (the mixed step on a static metric forms it again)
+    rc |= dalloc(c, (void**)&X.Un, sizeof(double) * ((P.mixed && !P.dyn) ? 1 : NP * NT));
     rc |= dalloc(c, (void**)&X.slow_idx, sizeof(unsigned) * nint);
     rc |= dalloc(c, (void**)&X.pflag, sizeof(int) * NT);
     rc |= dalloc(c, (void**)&X.nfail, sizeof(int) * (P.track_nfail ? N_NFAIL * NT : 1));
