@@ -515,8 +515,10 @@ struct Api {
     if (!on) rc |= alloc_levels(c);
     if (rc) return rc;
     c->P.otf = on ? 1 : 0;
-    // leaving the evaluated mode: the state's centre level must exist in memory
-    if (was && !on && c->P.dyn) rc |= BE::fill_geometry(c, const_cast<Geom*>(c->X.geom), nullptr, c->X.cent_lvl, 1u << CENT, 0u);
+    // leaving the evaluated mode: the levels of the running step must exist in
+    // memory (the switch may come between its predictor and corrector), the
+    // state's centre level among them
+    if (was && !on && c->P.dyn) rc |= fill_levels(c);
     return rc;
   }
   // Centre records of the state's level in memory (the evaluated mode keeps

@@ -161,7 +161,8 @@ int hgpu_get_launch_shape(hgpu_ctx *ctx, int family, int *bs, int *kc, int *flag
  * degenerate direction (an error is returned otherwise).  The stored levels
  * (three positions-by-zone record levels and, in mixed precision, two FP32
  * face copies) are allocated when first needed: on the switch to 0, or by
- * hgpu_download_geometry.  No counterpart in the reference (whose binary
+ * hgpu_download_geometry.  The switch to 0 fills the running step's levels,
+ * so it may come between the predictor and the corrector.  No counterpart in the reference (whose binary
  * metric bodies are absent). */
 int hgpu_set_metric_eval(hgpu_ctx *ctx, int in_kernels);
 int hgpu_get_metric_eval(hgpu_ctx *ctx, int *in_kernels);

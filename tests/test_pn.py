@@ -287,6 +287,29 @@ def test_gpu_bbh_metric_in_kernels_matches_stored(precision):
 
 
 @pytest.mark.gpu
+def test_gpu_bbh_switch_to_stored_between_stages():
+    """Leaving the evaluated mode between the predictor and the corrector: the
+    corrector reads the step's stored levels, filled at the switch, and the
+    step equals one taken wholly in the evaluated mode, bit for bit."""
+    A, _ = _bbh_run(False, "fp64", steps=2, in_kernels=True)
+    B, _ = _bbh_run(False, "fp64", steps=2, in_kernels=True)
+    for H, switch in ((A, False), (B, True)):
+        H.advance(0)
+        if switch:
+            H.set_metric_eval(False)
+        H.advance(1)
+        H.end_step()
+    assert not B.metric_eval()
+    assert np.array_equal(A.prims(), B.prims())
+    assert np.array_equal(A.pflag(), B.pflag())
+    assert np.array_equal(A.nfail(), B.nfail())
+    assert A.time() == B.time()
+    A.step_ch(1)
+    B.step_ch(1)
+    assert np.array_equal(A.prims(), B.prims())
+
+
+@pytest.mark.gpu
 def test_gpu_bbh_autotune_keeps_results():
     """Launch shapes on the binary metric (evaluated records): the autotuner
     leaves the state unchanged and the tuned steps equal the untuned ones."""
