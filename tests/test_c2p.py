@@ -215,6 +215,27 @@ def test_host_training_and_fallback(host):
     assert np.all(np.abs(p1[0, ok] - p0[0, ok]) <= 1e-8 * p0[0, ok])
 
 
+def _ragged_check(B, U, rth, sizes, budgets):
+    """Zones are independent: the first n zones solved as a batch of their
+    own (sizes that are not multiples of the wavefront or the workgroup, one
+    zone, none) get the results they have in the full batch, one-pass and
+    compacted with small budgets (ragged queue tails)."""
+    p0, c0, i0 = _solve_np(B, U, rth, compact=False)
+    for n in sizes:
+        for compact in (False, True):
+            kw = dict(compact=compact)
+            if compact:
+                kw.update(budget_first=budgets[0], budget_next=budgets[1])
+            p, c, it = _solve_np(B, np.ascontiguousarray(U[:, :n]), np.ascontiguousarray(rth[:, :n]), **kw)
+            assert p.shape == (8, n) and c.shape == (n,)
+            assert np.array_equal(c, c0[:n]) and np.array_equal(it, i0[:n]), (n, compact)
+            assert np.array_equal(p, p0[:, :n]), (n, compact)
+
+
+def test_host_ragged_and_empty_batches(host, host_samples):
+    _ragged_check(host, host_samples["U"], host_samples["rth"], (0, 1, 63, 65, 257, 2999), (2, 1))
+
+
 def test_entry_point_errors(host):
     d = host.samples(10, 1, A_SPIN, GAM)
     code, iters = host.empty((10,), np.int32), host.empty((10,), np.int32)
@@ -303,6 +324,14 @@ def test_gpu_compaction_is_exact(gpu, budgets):
     p1, c1, i1 = (x.numpy() for x in gpu.solve(g["U"], g["rth"], A_SPIN, GAM, compact=True,
                                                budget_first=budgets[0], budget_next=budgets[1]))
     assert np.array_equal(c0, c1) and np.array_equal(i0, i1) and np.array_equal(p0, p1)
+
+
+@pytest.mark.gpu
+def test_gpu_ragged_and_empty_batches(gpu):
+    """The persistent grid and the compaction queues over batch sizes that
+    are not multiples of the wavefront, the workgroup or the grid's stride."""
+    g = _np(gpu.samples(300000, 31, A_SPIN, GAM))
+    _ragged_check(gpu, g["U"], g["rth"], (0, 1, 63, 65, 257, 4097, 65537, 299999), (2, 1))
 
 
 @pytest.mark.gpu
