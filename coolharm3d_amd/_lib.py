@@ -78,6 +78,7 @@ SIGNATURES = {
     "hgpu_set_metric_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "hgpu_get_metric_eval": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
     "hgpu_device_bytes": (ctypes.c_int, [ctypes.c_void_p, c_ll_p]),
+    "hgpu_get_m_tot": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     "hgpu_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_double_p]),
     "hgpu_set_launch_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "hgpu_get_launch_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p, c_int_p, c_int_p]),

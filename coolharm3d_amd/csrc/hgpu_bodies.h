@@ -829,14 +829,16 @@ HD bool update_zone_fast(const Params& P, const Ptrs& X, int stage, double dt, i
 }
 
 template <bool MIXED>
-HD void update_zone(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
-                    FailRec& fr) {
+// Returns U^{n+1}[RHO], the zone's term of M_tot (src/step_ch.c:389).
+HD double update_zone(const Params& P, const Ptrs& X, int stage, double dt, int i, int j, int k,
+                      FailRec& fr) {
   double U[NP];
   if (!update_zone_fast<MIXED>(P, X, stage, dt, i, j, k, fr, U)) {
     Ptrs Xn = X;
     Xn.geom = X.geom_new;   // the full tree works at the result's time level
     recover_zone<MIXED>(P, Xn, stage, dt, i, j, k, U, fr);
   }
+  return U[RHO];
 }
 
 // --------------------------------------------------------------------------

@@ -43,6 +43,7 @@ struct hgpu_ctx {
   hgpu::GeomF* levf[2] = {nullptr, nullptr};
   bool lev_alloc = false;
   long long dev_bytes = 0;         // device memory the context's allocations requested
+  long long mpart_cap = 0;         // wavefronts an update launch may have (M_tot parts)
   double* connlev[2] = {nullptr, nullptr};
   hgpu::PnGrid pn;
   hgpu::PnOrbit* orbit = nullptr;  // device: the orbit at levels t, t + dt/2, t + dt
@@ -324,6 +325,11 @@ struct Api {
     rc |= dalloc(c, (void**)&X.nfail, sizeof(int) * (P.track_nfail ? N_NFAIL * NT : 1));
     rc |= dalloc(c, (void**)&X.status, sizeof(unsigned long long) * ST_WORDS);
     rc |= dalloc(c, (void**)&X.scal, sizeof(double) * SC_WORDS);
+    // at most n1 (n2 n3 / 64 + n2 + n3 + 2) + 8 workgroups of at most 4
+    // wavefronts over the interior, for any tile of 64 to 256 threads
+    c->mpart_cap = 4 * ((long long)P.n1 * ((long long)P.n2 * P.n3 / 64 + P.n2 + P.n3 + 2) + 8);
+    rc |= dalloc(c, (void**)&X.mpart, sizeof(double) * c->mpart_cap);
+    rc |= dalloc(c, (void**)&X.mpart2, sizeof(double) * 1024);
     if (rc) { destroy(c); return set_error(-3, "device allocation failed: " + g_last_error); }
     if (!c->P.otf) rc |= alloc_levels(c);
     if (rc) { destroy(c); return rc; }
@@ -376,7 +382,8 @@ struct Api {
     // buffer allocated for them), so the levels are what is released
     void* ptrs[] = {X.p, X.ph, X.F, X.emf, (void*)X.omega, (void*)X.tr,
                     (void*)X.rscale, (void*)X.irscale, (void*)X.rhoflr, (void*)X.uuflr, X.gamma,
-                    X.pflag, X.nfail, X.status, X.scal, X.Ff, X.emff, X.Uslow, X.Un, X.slow_idx,
+                    X.pflag, X.nfail, X.status, X.scal, X.mpart, X.mpart2, X.Ff, X.emff, X.Uslow, X.Un,
+                    X.slow_idx,
                     c->lev[0], c->lev[1], c->lev[2], c->levf[0], c->levf[1], c->connlev[0], c->connlev[1],
                     (void*)c->orbit, c->pn_tab};
     for (void* p : ptrs) if (p) BE::release(c, p);
@@ -430,12 +437,20 @@ struct Api {
     if (dt_old) *dt_old = sc[SC_DT_OLD];
     return 0;
   }
+  // M_tot of the last stage (src/step_ch.c:309, :389)
+  static int get_m_tot(hgpu_ctx* c, double* m_tot) {
+    double sc[SC_WORDS];
+    const int rc = BE::d2h(c, sc, c->X.scal, sizeof(sc));
+    if (!rc) *m_tot = sc[SC_MTOT];
+    return rc;
+  }
 
   // bounds(prim, skip_mpi): skip_mpi = 0 always applies the physical BCs;
   // skip_mpi = 1 only if boundary_phys_pflag (src/bounds.c:56)
-  static int bounds(hgpu_ctx* c, double* S, int skip_mpi) {
+  // after_update: the first bounds after an update (it completes M_tot)
+  static int bounds(hgpu_ctx* c, double* S, int skip_mpi, bool after_update = false) {
     int rc = 0;
-    for (int dir = 0; dir < 3; dir++) rc |= BE::bounds_dir(c, S, dir, skip_mpi);
+    for (int dir = 0; dir < 3; dir++) rc |= BE::bounds_dir(c, S, dir, skip_mpi, after_update && dir == 0);
     return rc;
   }
 
@@ -477,7 +492,7 @@ struct Api {
     rc |= BE::update(c, stage);
     if (c->P.dyn) { X.geom = X.geom_new; X.cent_lvl = stage + 1; }
     rc |= BE::recover_queued(c, stage);
-    rc |= bounds(c, dst, 0);
+    rc |= bounds(c, dst, 0, true);
     return rc;
   }
 

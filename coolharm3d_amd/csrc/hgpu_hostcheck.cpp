@@ -108,17 +108,19 @@ struct HostBE {
   static int update(hgpu_ctx* c, int stage) {
     Box b = interior_box(c->P);
     const double dt = c->X.scal[SC_DT];
+    double mtot = 0.;   // in the reference's loop order, k fastest (src/step_ch.c:309, :389)
     for (long long t = 0; t < b.count(); t++) {
       int i, j, k;
       b.at(t, i, j, k);
       FailRec fr; fr.clear();
-      if (c->P.mixed) update_zone<true>(c->P, c->X, stage, dt, i, j, k, fr);
-      else update_zone<false>(c->P, c->X, stage, dt, i, j, k, fr);
+      if (c->P.mixed) mtot += update_zone<true>(c->P, c->X, stage, dt, i, j, k, fr);
+      else mtot += update_zone<false>(c->P, c->X, stage, dt, i, j, k, fr);
       host_commit(c->P, c->X, fr, cidx(c->P, i, j, k), true);
     }
+    c->X.scal[SC_MTOT] = mtot;
     return 0;
   }
-  static int bounds_dir(hgpu_ctx* c, double* S, int dir, int skip_mpi) {
+  static int bounds_dir(hgpu_ctx* c, double* S, int dir, int skip_mpi, bool = false) {
     if (skip_mpi && !c->X.status[ST_BOUNDARY_PHYS]) return 0;
     const int do_pflag = c->X.status[ST_FAILURE_EXISTS] ? 1 : 0;
     Box b = ghost_box(c->P, dir);

@@ -516,6 +516,58 @@ __global__ __launch_bounds__(BLOCK) void k_emf_mixed(Params P, Ptrs X, Box b, Ti
   emf_edge_mixed(P, X, i, j, k);
 }
 
+// M_tot of advance() (src/step_ch.c:309, :389): the sum of U^{n+1}[RHO] over
+// the stage's zones, in three fixed-order levels so that the result does not
+// depend on scheduling (it differs from the reference's serial loop only in
+// the order of the additions):
+//  1. the update: each wavefront's sum (a butterfly over the lanes), one part
+//     per wavefront, no barrier;
+//  2. k_recover (which follows every update): thread g of the grid adds the
+//     parts g, g + G, g + 2G ... (G threads), each workgroup sums its threads;
+//  3. the first boundary kernel of the stage: its first workgroup adds the
+//     second-level parts.
+// The kernel boundaries order each level's stores before the next's reads.
+__device__ __forceinline__ double wave_sum_fixed(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double block_sum_fixed(double v, double* s_w) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  v = wave_sum_fixed(v);
+  if (lane == 0) s_w[wv] = v;
+  __syncthreads();
+  double b = 0.;
+  for (int q = 0; q < nw; q++) b += s_w[q];
+  __syncthreads();
+  return b;
+}
+__device__ __forceinline__ void mtot_wave_part(const Ptrs& X, double v) {
+  v = wave_sum_fixed(v);
+  const unsigned nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) X.mpart[blockIdx.x * nw + (threadIdx.x >> 6)] = v;
+  if (blockIdx.x == 0 && threadIdx.x == 0) X.status[ST_MPARTS] = (unsigned long long)gridDim.x * nw;
+}
+// every thread of every workgroup of the grid
+__device__ __forceinline__ void mtot_level2(const Ptrs& X) {
+  __shared__ double s_w[BLOCK / 64];
+  const unsigned n = (unsigned)X.status[ST_MPARTS];
+  const unsigned G = gridDim.x * blockDim.x;
+  double s = 0.;
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += G) s += X.mpart[q];
+  const double b = block_sum_fixed(s, s_w);
+  if (threadIdx.x == 0) X.mpart2[blockIdx.x] = b;
+  if (blockIdx.x == 0 && threadIdx.x == 0) X.status[ST_MPARTS2] = gridDim.x;
+}
+// every thread of one workgroup
+__device__ __forceinline__ void mtot_level3(const Ptrs& X) {
+  __shared__ double s_w[BLOCK / 64];
+  const unsigned n = (unsigned)X.status[ST_MPARTS2];
+  double s = 0.;
+  for (unsigned q = threadIdx.x; q < n; q += blockDim.x) s += X.mpart2[q];
+  const double tot = block_sum_fixed(s, s_w);
+  if (threadIdx.x == 0) X.scal[SC_MTOT] = tot;
+}
+
 // Conserved update U^{n+1} of every interior zone into the U buffer
 // ([8][NT], indexed like the state).
 template <bool MIXED, int W>
@@ -524,6 +576,7 @@ __global__ __launch_bounds__(BLOCK, W) void k_conserved(Params P, Ptrs X, int st
   const bool active = tile_index(b, T, i, j, k);
   FailRec fr; fr.clear();
   long long c = 0;
+  double urho = 0.;
   if (active) {
     const double dt = X.scal[SC_DT];
     const double Dt = (stage == 0) ? 0.5*dt : dt;
@@ -533,8 +586,10 @@ __global__ __launch_bounds__(BLOCK, W) void k_conserved(Params P, Ptrs X, int st
     zone_conserved<MIXED>(P, X, stage, Dt, i, j, k, X.geom_n + gid, X.geom + gid, fr, U);
 #pragma unroll
     for (int l = 0; l < NP; l++) X.Uslow[l*P.nt + c] = U[l];
+    urho = U[RHO];
   }
   commit_fail(P, X, fr, active, c, MASK_GAMMA);
+  mtot_wave_part(X, urho);
 }
 
 // The common con-to-prim branch; zones that need the full recover_primitives()
@@ -586,6 +641,7 @@ __global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage
   FailRec fr; fr.clear();
   long long c = 0;
   bool queue = false;
+  double urho = 0.;
   if (active) {
     const double dt = X.scal[SC_DT];
     c = cidx(P, i, j, k);
@@ -612,6 +668,7 @@ __global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage
     } else {
       done = update_zone_fast<MIXED, MET, UNI>(P, X, stage, dt, i, j, k, fr, U);
     }
+    urho = U[RHO];
     if (!done) {
       queue = true;
 #pragma unroll
@@ -628,6 +685,7 @@ __global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage
     if (queue) X.slow_idx[base + (unsigned)__popcll(ballot & ((1ull << lane) - 1ull))] = (unsigned)c;
   }
   commit_fail(P, X, fr, active, c, MASK_ALL);
+  mtot_wave_part(X, urho);
 }
 
 // Full con-to-prim tree for the queued zones: a fixed grid strides over the
@@ -636,6 +694,7 @@ __global__ __launch_bounds__(BLOCK, W) void k_update(Params P, Ptrs X, int stage
 // orbit of level lvl.
 template <bool MIXED>
 __global__ __launch_bounds__(BLOCK) void k_recover(Params P, Ptrs X, int stage, PnGrid G, int lvl) {
+  mtot_level2(X);   // the update's M_tot parts
   const unsigned n = (unsigned)X.status[ST_SLOW];
   const unsigned stride = gridDim.x * BLOCK;
   const unsigned rounds = (n + stride - 1) / stride;
@@ -661,7 +720,11 @@ __global__ __launch_bounds__(BLOCK) void k_recover(Params P, Ptrs X, int stage, 
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_bounds(Params P, Ptrs X, double* S, int dir, int skip_mpi, Box b) {
+// mtot: the first boundary kernel after an update, whose first workgroup
+// completes that update's M_tot.
+__global__ __launch_bounds__(BLOCK) void k_bounds(Params P, Ptrs X, double* S, int dir, int skip_mpi, Box b,
+                                                  int mtot) {
+  if (mtot && blockIdx.x == 0) mtot_level3(X);
   if (skip_mpi && !X.status[ST_BOUNDARY_PHYS]) return;
   const int do_pflag = X.status[ST_FAILURE_EXISTS] ? 1 : 0;
   long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
@@ -1343,6 +1406,7 @@ struct GpuBE {
     Scope s(c, "update");
     const Tiling T = make_tiling(b, priv(c)->shape[FAM_CONS]);
     const Tiling T2 = make_tiling(b, priv(c)->shape[FAM_INV]);
+    if ((long long)T.nblocks * (T.bs / 64) > c->mpart_cap) return set_error(-3, "update grid exceeds the M_tot parts buffer");
     hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_SLOW, 1);
     hipStream_t st = priv(c)->stream;
     if (priv(c)->shape[FAM_CONS].fused || c->P.otf) {   // the evaluated metric has the fused kernel only
@@ -1365,10 +1429,11 @@ struct GpuBE {
       hipLaunchKernelGGL(k_recover<false>, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, stage, c->pn, stage);
     return hip_check(hipGetLastError(), "k_recover");
   }
-  static int bounds_dir(hgpu_ctx* c, double* S, int dir, int skip_mpi) {
+  static int bounds_dir(hgpu_ctx* c, double* S, int dir, int skip_mpi, bool mtot = false) {
     Box b = ghost_box(c->P, dir);
     Scope s(c, "bounds");
-    hipLaunchKernelGGL(k_bounds, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, dir, skip_mpi, b);
+    hipLaunchKernelGGL(k_bounds, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, dir, skip_mpi, b,
+                       mtot ? 1 : 0);
     return hip_check(hipGetLastError(), "k_bounds");
   }
   static int fixup_interp(hgpu_ctx* c, double* S, int which) {

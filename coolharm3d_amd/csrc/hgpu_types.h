@@ -80,11 +80,14 @@ enum {
   ST_NSTEP = 27,             // steps completed since the context was created
   ST_CRASH = 28,             // step (1-based) that first crossed the crash threshold, 0 if none
   ST_CRASH_NBAD = 29,        // its n_bad
+  ST_MPARTS = 30,            // M_tot parts of the last update (one per wavefront)
+  ST_MPARTS2 = 31,           // their second-level parts (one per k_recover workgroup)
   ST_WORDS = 32
 };
 
 // Device scalars of the integrator (t, dt ... of src/globals.h:380-390).
-enum { SC_T = 0, SC_DT = 1, SC_DT_OLD = 2, SC_T_OLD = 3, SC_WORDS = 8 };
+// SC_MTOT: M_tot, the sum of U[RHO] over the last stage's zones (src/step_ch.c:309, :389).
+enum { SC_T = 0, SC_DT = 1, SC_DT_OLD = 2, SC_T_OLD = 3, SC_MTOT = 4, SC_WORDS = 8 };
 
 struct Params {
   int n1, n2, n3;          // interior cells
@@ -173,6 +176,8 @@ struct Ptrs {
   int* nfail;              // [15][NT]
   unsigned long long* status;  // [ST_WORDS]
   double* scal;            // [SC_WORDS]
+  double* mpart;           // M_tot parts of the update, one per wavefront
+  double* mpart2;          // their sums, one per workgroup of k_recover (<= 1024)
 };
 
 // FP32 copy of a geometry record (mixed precision); 1 - alpha is formed in FP64.

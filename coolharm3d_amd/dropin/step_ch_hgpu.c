@@ -6,7 +6,7 @@
  * and reads the state and the failure bookkeeping through process globals.
  * This file provides that step_ch() with the same observable effect on the
  * globals it writes (src/step_ch.c:50-234): p, p_gamma, pflag, nfail,
- * nfailtot, failure_exists, t, t_old, dx[0], dt_old.
+ * nfailtot, failure_exists, M_tot, t, t_old, dx[0], dt_old.
  *
  * Build, inside the reference's own build (no other source changes):
  *   - compile src/step_ch.c with -Dstep_ch=step_ch_cpu (keeps advance(),
@@ -27,7 +27,8 @@
  * circular orbit), evaluated on the device at every level; nothing of the
  * host's geometry is uploaded then.  After every step it copies back the
  * clock, failure_exists and nfailtot (check_for_crash() reads them,
- * src/step_ch.c:1244) and, depending on HGPU_HOST_SYNC (compile-time default,
+ * src/step_ch.c:1244), M_tot (printed by the driver every step, src/main.c:71)
+ * and, depending on HGPU_HOST_SYNC (compile-time default,
  * overridden by the environment variable of the same name), the state and
  * maps p, p_gamma, pflag, nfail:
  *   1  after every step;
@@ -321,6 +322,7 @@ void step_ch(void) {
   failure_exists = fe;
   hg_check(hgpu_get_nfailtot(hg, nft), "hgpu_get_nfailtot");
   for (l = 0; l < N_NFAIL; l++) nfailtot[l] = hg_nfailtot_base[l] + (ulint)nft[l];
+  hg_check(hgpu_get_m_tot(hg, &M_tot), "hgpu_get_m_tot");
   if (hg_sync_mode < 0)
     hg_sync_mode = getenv("HGPU_HOST_SYNC") ? atoi(getenv("HGPU_HOST_SYNC")) : HGPU_HOST_SYNC;
   if (hg_sync_mode == 1 || (hg_sync_mode == 2 && hg_host_due())) step_ch_hgpu_sync_host();

@@ -164,6 +164,12 @@ class Harm3D:
         self._chk(self.lib.hgpu_get_metric_eval(self.h, _ip(v)), "hgpu_get_metric_eval")
         return bool(v[0])
 
+    def m_tot(self):
+        """M_tot of the last stage: the sum of U[RHO] over the zones (hgpu_get_m_tot)."""
+        v = ctypes.c_double()
+        self._chk(self.lib.hgpu_get_m_tot(self.h, ctypes.byref(v)), "hgpu_get_m_tot")
+        return float(v.value)
+
     def device_bytes(self):
         """Device memory the context holds (hgpu_device_bytes)."""
         v = ctypes.c_longlong()

@@ -167,6 +167,14 @@ int hgpu_get_launch_shape(hgpu_ctx *ctx, int family, int *bs, int *kc, int *flag
 int hgpu_set_metric_eval(hgpu_ctx *ctx, int in_kernels);
 int hgpu_get_metric_eval(hgpu_ctx *ctx, int *in_kernels);
 
+/* M_tot after the last advance() stage: the sum of the updated conserved
+ * density U[RHO] (sqrt(-g) rho u^t) over the block's zones, as advance()
+ * accumulates it (src/step_ch.c:309, :389) and the driver prints it
+ * (src/main.c:71).  The device adds the zones' terms in a fixed order of its
+ * own (deterministic; the reference's serial loop order differs, so the last
+ * bits may too). */
+int hgpu_get_m_tot(hgpu_ctx *ctx, double *m_tot);
+
 /* Device memory the context holds: the bytes its allocations requested
  * (state, face fluxes, queues, geometry and metric buffers).  No counterpart
  * in the reference. */

@@ -46,6 +46,7 @@ for s in (1, 10, 100):
     out[f"s{{s}}_nfail"] = R.nfail()
     out[f"s{{s}}_t"] = sc["t"]
     out[f"s{{s}}_dt"] = sc["dt"]
+    out[f"s{{s}}_mtot"] = R.diag()["M_tot"]
 np.savez({path!r}, **out)
 """
 
@@ -63,6 +64,17 @@ def _run(cfg, tmp_path, stages=False):
     print(f"[drop-in] running the reference driver with {cfg} (stages {stages})", flush=True)
     subprocess.run([sys.executable, "-c", code], check=True, stdout=subprocess.DEVNULL, timeout=600, env=env)
     return dict(np.load(path))
+
+
+def _check_m_tot(R, cfg_ref, tmp_path, rel):
+    """M_tot, which the driver prints every step (src/main.c:71), against the
+    reference build's own after the same steps: rel = 0 where the sum runs in
+    the reference's loop order (host build), else the order's round-off."""
+    ref = _run(cfg_ref, tmp_path)
+    for s in (1, 10, 100):
+        a, b = float(R[f"s{s}_mtot"]), float(ref[f"s{s}_mtot"])
+        assert b != 0.
+        assert abs(a - b) <= rel * abs(b), (s, a, b)
 
 
 def _check_c0(R):
@@ -88,7 +100,9 @@ def _check_c1(R, tol100, fixture="c1_torus24x24x16.npz"):
 
 @pytest.mark.skipif(not _built("dropin_c0_host"), reason="drop-in build not present (make -C oracle dropin)")
 def test_dropin_host_balsara_bit_identical(tmp_path):
-    _check_c0(_run("dropin_c0_host", tmp_path))
+    R = _run("dropin_c0_host", tmp_path)
+    _check_c0(R)
+    _check_m_tot(R, "c0", tmp_path, 0.)
 
 
 @pytest.mark.skipif(not _built("dropin_c0_host"), reason="drop-in build not present (make -C oracle dropin)")
@@ -102,19 +116,25 @@ def test_dropin_host_balsara_staged_bit_identical(tmp_path):
 @pytest.mark.slow
 @pytest.mark.skipif(not _built("dropin_c1_host"), reason="drop-in build not present (make -C oracle dropin)")
 def test_dropin_host_torus(tmp_path):
-    _check_c1(_run("dropin_c1_host", tmp_path), 1e-12)
+    R = _run("dropin_c1_host", tmp_path)
+    _check_c1(R, 1e-12)
+    _check_m_tot(R, "c1", tmp_path, 1e-12)
 
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not _built("dropin_c0_gpu"), reason="drop-in build not present (make -C oracle dropin)")
 def test_dropin_gpu_balsara_bit_identical(tmp_path):
-    _check_c0(_run("dropin_c0_gpu", tmp_path))
+    R = _run("dropin_c0_gpu", tmp_path)
+    _check_c0(R)
+    _check_m_tot(R, "c0", tmp_path, 1e-14)
 
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not _built("dropin_c1_gpu"), reason="drop-in build not present (make -C oracle dropin)")
 def test_dropin_gpu_torus(tmp_path):
-    _check_c1(_run("dropin_c1_gpu", tmp_path), 1e-9)
+    R = _run("dropin_c1_gpu", tmp_path)
+    _check_c1(R, 1e-9)
+    _check_m_tot(R, "c1", tmp_path, 1e-12)
 
 
 @pytest.mark.slow

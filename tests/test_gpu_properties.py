@@ -60,6 +60,7 @@ def test_graph_replay_matches_eager(precision):
         assert np.array_equal(E.pflag(), G.pflag())
         assert np.array_equal(E.nfail(), G.nfail())
         assert E.time() == G.time()
+        assert E.m_tot() == G.m_tot() and E.m_tot() > 0.   # fixed-order sum: the same bits
     G.set_graph(False)
     E.step_ch(1)
     G.step_ch(1)
