@@ -527,9 +527,33 @@ __global__ __launch_bounds__(BLOCK) void k_emf_mixed(Params P, Ptrs X, Box b, Ti
 //  3. the first boundary kernel of the stage: its first workgroup adds the
 //     second-level parts.
 // The kernel boundaries order each level's stores before the next's reads.
+// The sum of a wavefront's 64 values, the same bits in every lane and on every
+// run: within each row of 16 lanes, pairs, quads, halves and the row by DPP
+// permutations (a + b == b + a exactly, so every lane of a row ends with the
+// same row sum); then (row0 + row1) + (row2 + row3).  Every lane active.
+__device__ __forceinline__ double dpp_f64(double v, int ctrl_sel) {
+  const unsigned long long u = __double_as_longlong(v);
+  int lo = (int)(unsigned)u, hi = (int)(unsigned)(u >> 32);
+  switch (ctrl_sel) {   // the DPP control must be a constant
+    case 0: lo = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, false); hi = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, false); break;
+    case 1: lo = __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, false); hi = __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, false); break;
+    case 2: lo = __builtin_amdgcn_mov_dpp(lo, 0x141, 0xF, 0xF, false); hi = __builtin_amdgcn_mov_dpp(hi, 0x141, 0xF, 0xF, false); break;
+    default: lo = __builtin_amdgcn_mov_dpp(lo, 0x140, 0xF, 0xF, false); hi = __builtin_amdgcn_mov_dpp(hi, 0x140, 0xF, 0xF, false); break;
+  }
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double lane_f64(double v, int lane) {
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 __device__ __forceinline__ double wave_sum_fixed(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_f64(v, 0);   // quad_perm [1,0,3,2]: pairs
+  v += dpp_f64(v, 1);   // quad_perm [2,3,0,1]: quads
+  v += dpp_f64(v, 2);   // row_half_mirror: eights
+  v += dpp_f64(v, 3);   // row_mirror: rows of 16
+  return (lane_f64(v, 0) + lane_f64(v, 16)) + (lane_f64(v, 32) + lane_f64(v, 48));
 }
 __device__ __forceinline__ double block_sum_fixed(double v, double* s_w) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
