@@ -1005,7 +1005,8 @@ struct GpuBE {
     // Untuned defaults: the autotuner's picks on the 192x192x128 torus.  The
     // face fluxes run 2 faces per thread, FP64 in the kernels capped at 3
     // waves per SIMD, FP32 uncapped (3 waves) but along x2 (capped at 4); the
-    // update is the fused kernel capped at 3 waves per SIMD.
+    // update is the fused kernel, register-capped (3 waves per SIMD; the
+    // mixed-precision row-aligned variant 4).
     const bool mixed = c->P.mixed != 0;
     for (int f = FAM_FLUX1; f <= FAM_FLUX3; f++) {
       g->shape[f].fpt = 2;
@@ -1240,7 +1241,9 @@ struct GpuBE {
     const bool uni = c->P.metric_dim == 2 && !c->P.flat && T.kc == 0 && T.nk % 64 == 0 && T.bs % 64 == 0;
     if (c->P.mixed) {
       if (uni) {
-        if (L.occ) hipLaunchKernelGGL((k_update<true, 3, false, 0, true>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
+        // capped at 4 waves per SIMD (some registers spilled): 5 % faster
+        // than at 3 on the 192x192x128 torus
+        if (L.occ) hipLaunchKernelGGL((k_update<true, 4, false, 0, true>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
         else hipLaunchKernelGGL((k_update<true, 1, false, 0, true>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
       } else {
         if (L.occ) hipLaunchKernelGGL((k_update<true, 3>), grid, blk, 0, st, c->P, c->X, stage, b, T, G, lvl);
@@ -1293,19 +1296,21 @@ struct GpuBE {
     for (int d = 0; d < 3 && !rc; d++) rc |= flux(c, c->X.p, d);
     const unsigned bss[3] = {64, 128, 256}, kcs[4] = {0, 16, 32, 64};
     for (int fam = FAM_FLUX1; fam <= FAM_CONS && !rc; fam++) {
-      LaunchShape best = g->shape[fam];
+      const LaunchShape incumbent = g->shape[fam];
+      LaunchShape best = incumbent;
       float best_ms = 1e30f;
-      auto time_shape = [&](const LaunchShape& L) -> float {
+      auto time_shape_n = [&](const LaunchShape& L, int n) -> float {
         g->shape[fam] = L;
         rc |= launch_family(c, fam);   // warm-up
         rc |= hip_check(hipEventRecord(e0, st), "hipEventRecord");
-        for (int r = 0; r < reps; r++) rc |= launch_family(c, fam);
+        for (int r = 0; r < n; r++) rc |= launch_family(c, fam);
         rc |= hip_check(hipEventRecord(e1, st), "hipEventRecord");
         rc |= hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
         float ms = 0.f;
         rc |= hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-        return rc ? 1e30f : ms / (float)reps;
+        return rc ? 1e30f : ms / (float)n;
       };
+      auto time_shape = [&](const LaunchShape& L) -> float { return time_shape_n(L, reps); };
       const bool multi = fam <= FAM_FLUX3 && c->P.recon == RECON_PPM_MC && !c->P.degen[fam - FAM_FLUX1];
       const unsigned nfpt = multi ? 3u : 1u;
       // face groups: every group size with and without the register cap
@@ -1356,6 +1361,23 @@ struct GpuBE {
           L.occ = 1;
           const float ms = time_shape(L);
           if (!rc && ms < best_ms) { best_ms = ms; best = L; }
+        }
+      }
+      // The winner of the sweep against the shape the family had, both timed
+      // again, alternately, with more repetitions: a winner not 1 % faster
+      // keeps the shape the family had (the sweep's single timings differ by
+      // about that much from run to run).
+      const bool moved = shape_flags(best) != shape_flags(incumbent) || best.bs != incumbent.bs ||
+                         best.kc != incumbent.kc;
+      if (!rc && moved) {
+        float t_inc = 1e30f, t_best = 1e30f;
+        for (int round = 0; round < 2 && !rc; round++) {
+          t_inc = std::min(t_inc, time_shape_n(incumbent, 3 * reps));
+          t_best = std::min(t_best, time_shape_n(best, 3 * reps));
+        }
+        if (!rc) {
+          if (t_best < 0.99f * t_inc) best_ms = t_best;
+          else { best = incumbent; best_ms = t_inc; }
         }
       }
       g->shape[fam] = best;
