@@ -31,6 +31,7 @@ struct ProfEvent { std::string name; hipEvent_t a, b; };
 
 struct GpuPriv {
   hipStream_t stream = nullptr;
+  unsigned flag_grid = 1024;           // workgroups of the kernels that mostly find nothing to do
   bool prof = false;
   bool graph = false;                  // replay captured steps
   hipGraphExec_t step_exec = nullptr;  // one step_ch(), captured on first use
@@ -751,8 +752,11 @@ __global__ __launch_bounds__(BLOCK) void k_bounds(Params P, Ptrs X, double* S, i
   if (mtot && blockIdx.x == 0) mtot_level3(X);
   if (skip_mpi && !X.status[ST_BOUNDARY_PHYS]) return;
   const int do_pflag = X.status[ST_FAILURE_EXISTS] ? 1 : 0;
-  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
-  const bool active = t < b.count();
+  const long long n = b.count(), stride = (long long)gridDim.x * BLOCK;
+  const long long rounds = (n + stride - 1) / stride;   // the same in every workgroup (barriers)
+  for (long long r = 0; r < rounds; r++) {
+  const long long t = r * stride + (long long)blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = t < n;
   FailRec fr; fr.clear();
   long long c = 0;
   if (active) {
@@ -776,22 +780,27 @@ __global__ __launch_bounds__(BLOCK) void k_bounds(Params P, Ptrs X, double* S, i
     if (dir == 0) c = cidx(P, side ? (n1e + 1 + g) : (NG - 1 - g), a, bb);
   }
   commit_fail(P, X, fr, active && dir == 0, c, MASK_GAMMA);
+  }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_fixup_interp(Params P, Ptrs X, double* S, int which, Box b) {
   if (!X.status[ST_FAILURE_EXISTS]) return;
   if (!X.status[(which == PFLAG_INTERP_PRIM) ? ST_FAILTYPE_PRIM : ST_FAILTYPE_V]) return;
-  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
-  const bool active = t < b.count();
-  FailRec fr; fr.clear();
-  long long c = 0;
-  if (active) {
-    int i, j, k;
-    b.at(t, i, j, k);
-    fixup_interp_zone(P, X, S, which, i, j, k, fr);
-    c = cidx(P, i, j, k);
+  const long long n = b.count(), stride = (long long)gridDim.x * BLOCK;
+  const long long rounds = (n + stride - 1) / stride;   // the same in every workgroup (barriers)
+  for (long long r = 0; r < rounds; r++) {
+    const long long t = r * stride + (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const bool active = t < n;
+    FailRec fr; fr.clear();
+    long long c = 0;
+    if (active) {
+      int i, j, k;
+      b.at(t, i, j, k);
+      fixup_interp_zone(P, X, S, which, i, j, k, fr);
+      c = cidx(P, i, j, k);
+    }
+    commit_fail(P, X, fr, active, c, MASK_ALL);
   }
-  commit_fail(P, X, fr, active, c, MASK_ALL);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_fixup_init(Params P, Ptrs X, Box b) {
@@ -810,8 +819,8 @@ __global__ __launch_bounds__(BLOCK) void k_fixup_init(Params P, Ptrs X, Box b) {
 
 __global__ __launch_bounds__(BLOCK) void k_clear_pflag(Ptrs X, long long nt) {
   if (!X.status[ST_FAILURE_EXISTS]) return;
-  long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
-  if (t < nt) X.pflag[t] = 0;
+  const long long stride = (long long)gridDim.x * BLOCK;
+  for (long long t = (long long)blockIdx.x * BLOCK + threadIdx.x; t < nt; t += stride) X.pflag[t] = 0;
 }
 
 __global__ void k_reset_failure_words(Ptrs X) {
@@ -1021,6 +1030,16 @@ struct GpuBE {
     g->shape[FAM_CONS].fused = 1;
     g->shape[FAM_CONS].remap = 0;
     g->shape[FAM_CONS].occ = 1;
+    // Kernels gated by a device flag (fixups, flag clears, the boundaries of
+    // bounds(prim,1)) stride over their zones with 4 workgroups per CU: a
+    // full grid of workgroups that find the flag clear costs several
+    // microseconds just to dispatch.
+    {
+      int dev = 0, cus = 0;
+      if (hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+        g->flag_grid = 4u * (unsigned)cus;
+    }
     return hip_check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
   }
   static void fini(hgpu_ctx* c) {
@@ -1478,14 +1497,18 @@ struct GpuBE {
   static int bounds_dir(hgpu_ctx* c, double* S, int dir, int skip_mpi, bool mtot = false) {
     Box b = ghost_box(c->P, dir);
     Scope s(c, "bounds");
-    hipLaunchKernelGGL(k_bounds, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, dir, skip_mpi, b,
+    // bounds(prim,0) has work every time: one zone per thread; bounds(prim,1)
+    // mostly finds its flag clear: a strided grid
+    const unsigned nb = skip_mpi ? std::min(nblocks(b.count()), priv(c)->flag_grid) : nblocks(b.count());
+    hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, dir, skip_mpi, b,
                        mtot ? 1 : 0);
     return hip_check(hipGetLastError(), "k_bounds");
   }
   static int fixup_interp(hgpu_ctx* c, double* S, int which) {
     Box b = interior_box(c->P);
     Scope s(c, "fixup");
-    hipLaunchKernelGGL(k_fixup_interp, dim3(nblocks(b.count())), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, which, b);
+    const unsigned nb = std::min(nblocks(b.count()), priv(c)->flag_grid);
+    hipLaunchKernelGGL(k_fixup_interp, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, which, b);
     return hip_check(hipGetLastError(), "k_fixup_interp");
   }
   static int clear_boundary_flag(hgpu_ctx* c) {
@@ -1534,7 +1557,8 @@ struct GpuBE {
     return rc ? rc : sync(c);
   }
   static int clear_failures(hgpu_ctx* c) {
-    hipLaunchKernelGGL(k_clear_pflag, dim3(nblocks(c->P.nt)), dim3(BLOCK), 0, priv(c)->stream, c->X, c->P.nt);
+    hipLaunchKernelGGL(k_clear_pflag, dim3(std::min(nblocks(c->P.nt), priv(c)->flag_grid)), dim3(BLOCK), 0,
+                       priv(c)->stream, c->X, c->P.nt);
     hipLaunchKernelGGL(k_reset_failure_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X);
     return hip_check(hipGetLastError(), "k_clear_pflag");
   }
