@@ -754,32 +754,32 @@ __global__ __launch_bounds__(BLOCK) void k_bounds(Params P, Ptrs X, double* S, i
   const int do_pflag = X.status[ST_FAILURE_EXISTS] ? 1 : 0;
   const long long n = b.count(), stride = (long long)gridDim.x * BLOCK;
   const long long rounds = (n + stride - 1) / stride;   // the same in every workgroup (barriers)
-  for (long long r = 0; r < rounds; r++) {
-  const long long t = r * stride + (long long)blockIdx.x * BLOCK + threadIdx.x;
-  const bool active = t < n;
-  FailRec fr; fr.clear();
-  long long c = 0;
-  if (active) {
-    int sg, a, bb;
-    if (dir == 2) {
-      // x3 ghosts: the 2 NG layers of one (i, j) row are the fastest index, so
-      // a wavefront's accesses stay within the rows' ends rather than one
-      // cache line per lane
-      const int nsg = b.hi[0] - b.lo[0] + 1, nb = b.hi[2] - b.lo[2] + 1;
-      sg = b.lo[0] + (int)(t % nsg);
-      const long long r = t / nsg;
-      bb = b.lo[2] + (int)(r % nb);
-      a = b.lo[1] + (int)(r / nb);
-    } else {
-      b.at(t, sg, a, bb);
+  for (long long rd = 0; rd < rounds; rd++) {
+    const long long t = rd * stride + (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const bool active = t < n;
+    FailRec fr; fr.clear();
+    long long c = 0;
+    if (active) {
+      int sg, a, bb;
+      if (dir == 2) {
+        // x3 ghosts: the 2 NG layers of one (i, j) row are the fastest index, so
+        // a wavefront's accesses stay within the rows' ends rather than one
+        // cache line per lane
+        const int nsg = b.hi[0] - b.lo[0] + 1, nb = b.hi[2] - b.lo[2] + 1;
+        sg = b.lo[0] + (int)(t % nsg);
+        const long long r = t / nsg;
+        bb = b.lo[2] + (int)(r % nb);
+        a = b.lo[1] + (int)(r / nb);
+      } else {
+        b.at(t, sg, a, bb);
+      }
+      const int side = sg / NG, g = sg % NG;
+      bound_zone(P, X, S, dir, side, g, a, bb, do_pflag, fr);
+      // nfail of inflow_check() belongs to the ghost zone
+      const int n1e = P.n1 + NG - 1;
+      if (dir == 0) c = cidx(P, side ? (n1e + 1 + g) : (NG - 1 - g), a, bb);
     }
-    const int side = sg / NG, g = sg % NG;
-    bound_zone(P, X, S, dir, side, g, a, bb, do_pflag, fr);
-    // nfail of inflow_check() belongs to the ghost zone
-    const int n1e = P.n1 + NG - 1;
-    if (dir == 0) c = cidx(P, side ? (n1e + 1 + g) : (NG - 1 - g), a, bb);
-  }
-  commit_fail(P, X, fr, active && dir == 0, c, MASK_GAMMA);
+    commit_fail(P, X, fr, active && dir == 0, c, MASK_GAMMA);
   }
 }
 
@@ -788,8 +788,8 @@ __global__ __launch_bounds__(BLOCK) void k_fixup_interp(Params P, Ptrs X, double
   if (!X.status[(which == PFLAG_INTERP_PRIM) ? ST_FAILTYPE_PRIM : ST_FAILTYPE_V]) return;
   const long long n = b.count(), stride = (long long)gridDim.x * BLOCK;
   const long long rounds = (n + stride - 1) / stride;   // the same in every workgroup (barriers)
-  for (long long r = 0; r < rounds; r++) {
-    const long long t = r * stride + (long long)blockIdx.x * BLOCK + threadIdx.x;
+  for (long long rd = 0; rd < rounds; rd++) {
+    const long long t = rd * stride + (long long)blockIdx.x * BLOCK + threadIdx.x;
     const bool active = t < n;
     FailRec fr; fr.clear();
     long long c = 0;
