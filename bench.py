@@ -68,16 +68,17 @@ STATE_BYTES_PER_ZONE_CYCLE = 320.0
 UPDATE_STATE_BYTES = STATE_BYTES_PER_ZONE_CYCLE / 2
 
 # Per-kernel byte models (FP64 state), reported beside it:
-#   flux   : per face   -- 8 primitives of the zone read once (64 B) + 8 face fluxes written (64 B)
+#   flux   : per face   -- 8 primitives of the zone read once (64 B) + 7 face fluxes written (56 B; the
+#                          normal field's flux is never read and not stored)
 #   emf    : per edge   -- 6 B-field face fluxes read (48 B) + 3 EMFs written (24 B)
 #   update : per zone   -- p read (64 B); 15 non-magnetic face fluxes + 3 EMFs read (144 B); result
 #                          (64 B) and gamma (8 B) written; the predictor writes U^n (64 B), the
 #                          corrector reads U^n and ph (128 B): 344 / 408 B
-ALGO_BYTES = {"flux": 128.0, "emf": 72.0, "update": 0.5 * (344.0 + 408.0)}
-# mixed precision: face fluxes and EMFs are FP32 (flux writes 32 B/face, EMF 24+12 B/edge,
+ALGO_BYTES = {"flux": 120.0, "emf": 72.0, "update": 0.5 * (344.0 + 408.0)}
+# mixed precision: face fluxes and EMFs are FP32 (flux writes 28 B/face, EMF 24+12 B/edge,
 # the update reads 18 FP32 values), and the corrector forms U^n again from the p^n it
 # reads anyway (no U^n written or read: 64 B less per launch on average)
-ALGO_BYTES_MIXED = {"flux": 96.0, "emf": 36.0, "update": 0.5 * (344.0 + 408.0) - 72.0 - 64.0}
+ALGO_BYTES_MIXED = {"flux": 92.0, "emf": 36.0, "update": 0.5 * (344.0 + 408.0) - 72.0 - 64.0}
 
 
 def measured_traffic(workload):

@@ -223,7 +223,10 @@ HD void flux_riemann_g(const Params& P, const Ptrs& X, const double* pl, const d
     for (int l = 0; l < NP; l++) out[l] = Fv[l];
   } else {
     const long long c = cidx(P, i, j, k);
-    for (int l = 0; l < NP; l++) X.F[(d*NFLUX + l)*NT + c] = Fv[l];
+    // the normal field's flux is not stored: flux_ct() zeroes it and nothing
+    // reads it (src/flux_ct.c:853-863)
+    for (int l = 0; l < NP; l++)
+      if (l != B1 + d) X.F[(d*NFLUX + l)*NT + c] = Fv[l];
   }
 }
 

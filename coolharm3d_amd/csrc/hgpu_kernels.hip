@@ -434,6 +434,7 @@ __global__ __launch_bounds__(BLOCK, W) void k_flux_multi(Params P, Ptrs X, const
     }
 #pragma unroll
     for (int l = 0; l < NP; l++) {
+      if (l == B3) continue;   // the normal field's flux: never read
       __syncthreads();
 #pragma unroll
       for (int f = 0; f < F; f++) wbuf[lane*F + f] = fo[f][l];

@@ -364,7 +364,8 @@ HD void flux_riemann_mixed_g(const Params& P, const Ptrs& X, const float* pl, co
   } else {
     const long long c = cidx(P, i, j, k);
 #pragma unroll
-    for (int l = 0; l < NP; l++) X.Ff[(d*NFLUX + l)*NT + c] = Fv[l];
+    for (int l = 0; l < NP; l++)
+      if (l != B1 + d) X.Ff[(d*NFLUX + l)*NT + c] = Fv[l];   // the normal field's flux: never read
   }
 }
 
