@@ -677,8 +677,8 @@ HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int
   // per zone and step.  Its gamma_calc failure is counted once, as the
   // reference counts it, in the predictor.
   const bool keep_un = !MIXED || ((MET < 0) ? P.dyn != 0 : MET != 0);
+  double pi[NP];
   if (stage == 0 || !keep_un) {
-    double pi[NP];
 #pragma unroll
     for (int v = 0; v < NP; v++) pi[v] = rt_val(P, X, v, i, X.p[v*NT + c]);
     State q;
@@ -698,9 +698,17 @@ HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int
     source<2>(P, X, nullptr, g, i, j, k, dU);
   } else {
     double phz[NP];
-    const double* src = (stage == 0) ? X.p : X.ph;
+    if (MIXED && stage == 0) {
+      // the predictor's source terms are evaluated at p^n: the values just
+      // read (the mixed update is 4 % faster for it; the FP64 update, whose
+      // registers are tighter, 6 % slower, and reads p^n again)
 #pragma unroll
-    for (int v = 0; v < NP; v++) phz[v] = rt_val(P, X, v, i, src[v*NT + c]);
+      for (int v = 0; v < NP; v++) phz[v] = pi[v];
+    } else {
+      const double* src = (stage == 0) ? X.p : X.ph;
+#pragma unroll
+      for (int v = 0; v < NP; v++) phz[v] = rt_val(P, X, v, i, src[v*NT + c]);
+    }
     fr.nf[NFAIL_GAMMA_CALC] += source<MET, UNI>(P, X, phz, g, i, j, k, dU);
   }
   if (MIXED) {
