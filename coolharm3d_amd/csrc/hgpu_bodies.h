@@ -666,7 +666,7 @@ HD double floor_uu(const Params& P, const Ptrs& X, int i, long long c) {
 // static metric both are the zone's one record.
 template <bool MIXED, int MET = -1, bool UNI = false, class GP = const Geom*>
 HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int i, int j, int k,
-                       GP gn, GP g, FailRec& fr, double* U) {
+                       GP gn, GP g, FailRec& fr, double* U, double* pn_out = nullptr, bool want_pn = false) {
   const long long NT = P.nt;
   const long long c = cidx(P, i, j, k);
   // U^n = primtoflux(p_i) with geom_i (src/step_ch.c:319, :339-341): kept for
@@ -688,6 +688,10 @@ HD void zone_conserved(const Params& P, const Ptrs& X, int stage, double Dt, int
     if (stage == 0 && keep_un) {
 #pragma unroll
       for (int v = 0; v < NP; v++) X.Un[v*NT + c] = U[v];
+    }
+    if (want_pn) {
+#pragma unroll
+      for (int v = 0; v < NP; v++) pn_out[v] = pi[v];
     }
   } else {
 #pragma unroll
@@ -812,9 +816,12 @@ HD bool update_zone_fast_g(const Params& P, const Ptrs& X, int stage, double dt,
                            FailRec& fr, double* U, GP gn, GP g, GP gnew) {
   const long long c = cidx(P, i, j, k);
   const double Dt = (stage == 0) ? 0.5*dt : dt;
-  zone_conserved<MIXED, MET, UNI>(P, X, stage, Dt, i, j, k, gn, g, fr, U);
   double pf[NP];
-  zone_guess(P, X, stage, i, c, pf);
+  // the mixed corrector forms U^n from p^n, which is also its Newton guess:
+  // the values already read
+  const bool reuse = MIXED && stage == 1 && !((MET < 0) ? P.dyn != 0 : MET != 0);
+  zone_conserved<MIXED, MET, UNI>(P, X, stage, Dt, i, j, k, gn, g, fr, U, pf, reuse);
+  if (!reuse) zone_guess(P, X, stage, i, c, pf);
   double pgam = 1.;
   if (!recover_fast(P, X, i, j, k, pf, U, gnew, floor_rho(P, X, i, c), floor_uu(P, X, i, c), &pgam, fr))
     return false;
