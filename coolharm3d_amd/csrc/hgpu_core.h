@@ -469,6 +469,8 @@ struct Api {
     return rc;
   }
 
+  // The stage itself; BE::begin_stage() has cleared its failure words, flag
+  // map and fastest speeds.
   static int substep(hgpu_ctx* c, int stage) {
     Ptrs& X = c->X;
     double* src = (stage == 0) ? X.p : X.ph;
@@ -486,7 +488,6 @@ struct Api {
       X.cent_lvl = stage;
       c->otf_stage = stage;
     }
-    rc |= BE::reset_vmax(c);
     for (int d = 0; d < 3; d++) rc |= BE::flux(c, src, d);
     rc |= BE::emf(c);
     rc |= BE::update(c, stage);
@@ -501,8 +502,7 @@ struct Api {
   // flags ask for it, the flags cleared.
   static int stage_fixup(hgpu_ctx* c, int stage) {
     double* dst = (stage == 0) ? c->X.ph : c->X.p;
-    int rc = BE::fixup_interp(c, dst, PFLAG_INTERP_V);
-    rc |= BE::fixup_interp(c, dst, PFLAG_INTERP_PRIM);
+    int rc = BE::fixup_interp(c, dst);   // fixup_interp_v(), then fixup_interp_prim()
     rc |= bounds(c, dst, 1);
     rc |= BE::clear_boundary_flag(c);
     return rc;
@@ -571,11 +571,11 @@ struct Api {
   // One advance() stage and bounds(prim,0)'s physical part, through the
   // public entry point: the step's geometry levels first (stage 0, binary
   // metric), the flag map and failure words cleared as step_ch() does before
-  // each stage (src/step_ch.c:65-69, :117-121).
+  // each stage (src/step_ch.c:65-69, :117-121), with the stage's fastest speeds.
   static int advance_api(hgpu_ctx* c, int stage) {
     int rc = 0;
     if (c->P.dyn && stage == 0) rc |= fill_levels(c);
-    rc |= BE::clear_failures(c);
+    rc |= BE::begin_stage(c);
     rc |= substep(c, stage);
     return rc;
   }
@@ -626,10 +626,10 @@ struct Api {
   static int one_step(hgpu_ctx* c) {
     int rc = 0;
     if (c->P.dyn) rc |= fill_levels(c);
-    rc |= BE::clear_failures(c);      // src/step_ch.c:65-69
+    rc |= BE::begin_stage(c);         // src/step_ch.c:65-69
     rc |= substep(c, 0);
     rc |= stage_fixup(c, 0);
-    rc |= BE::clear_failures(c);      // src/step_ch.c:117-121
+    rc |= BE::begin_stage(c);         // src/step_ch.c:117-121
     rc |= substep(c, 1);
     rc |= stage_fixup(c, 1);
     rc |= BE::timestep(c);            // src/step_ch.c:218-223

@@ -63,10 +63,6 @@ struct HostBE {
     if (n) *n = 0;
     return 0;
   }
-  static int reset_vmax(hgpu_ctx* c) {
-    for (int l = 0; l < 4; l++) c->X.status[ST_VMAX + l] = 0;
-    return 0;
-  }
   static int reset_nfailtot(hgpu_ctx* c) {
     for (int l = 0; l < N_NFAIL; l++) c->X.status[ST_NFAILTOT + l] = 0;
     return 0;
@@ -135,16 +131,20 @@ struct HostBE {
     }
     return 0;
   }
-  static int fixup_interp(hgpu_ctx* c, double* S, int which) {
+  // fixup_interp_v() over every zone, then fixup_interp_prim(), in the
+  // reference's order
+  static int fixup_interp(hgpu_ctx* c, double* S) {
     if (!c->X.status[ST_FAILURE_EXISTS]) return 0;
-    if (!c->X.status[(which == PFLAG_INTERP_PRIM) ? ST_FAILTYPE_PRIM : ST_FAILTYPE_V]) return 0;
-    Box b = interior_box(c->P);
-    for (long long t = 0; t < b.count(); t++) {
-      int i, j, k;
-      b.at(t, i, j, k);
-      FailRec fr; fr.clear();
-      fixup_interp_zone(c->P, c->X, S, which, i, j, k, fr);
-      host_commit(c->P, c->X, fr, cidx(c->P, i, j, k), true);
+    for (int which : {PFLAG_INTERP_V, PFLAG_INTERP_PRIM}) {
+      if (!c->X.status[(which == PFLAG_INTERP_PRIM) ? ST_FAILTYPE_PRIM : ST_FAILTYPE_V]) continue;
+      Box b = interior_box(c->P);
+      for (long long t = 0; t < b.count(); t++) {
+        int i, j, k;
+        b.at(t, i, j, k);
+        FailRec fr; fr.clear();
+        fixup_interp_zone(c->P, c->X, S, which, i, j, k, fr);
+        host_commit(c->P, c->X, fr, cidx(c->P, i, j, k), true);
+      }
     }
     return 0;
   }
@@ -181,13 +181,15 @@ struct HostBE {
     }
     return 0;
   }
-  static int clear_failures(hgpu_ctx* c) {
+  // before each stage: flag map and failure words cleared, fastest speeds reset
+  static int begin_stage(hgpu_ctx* c) {
     if (c->X.status[ST_FAILURE_EXISTS]) {
       for (long long t = 0; t < c->P.nt; t++) c->X.pflag[t] = 0;
     }
     c->X.status[ST_FAILURE_EXISTS] = 0;
     c->X.status[ST_FAILTYPE_V] = 0;
     c->X.status[ST_FAILTYPE_PRIM] = 0;
+    for (int l = 0; l < 4; l++) c->X.status[ST_VMAX + l] = 0;
     return 0;
   }
   static int timestep(hgpu_ctx* c) { timestep_update(c->P, c->X); return 0; }

@@ -784,9 +784,12 @@ __global__ __launch_bounds__(BLOCK) void k_bounds(Params P, Ptrs X, double* S, i
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_fixup_interp(Params P, Ptrs X, double* S, int which, Box b) {
+// Both interpolating fixups in one pass: a zone flagged for one of them reads
+// only unflagged neighbours and writes only itself, so the velocity and the
+// primitive fixups are independent and equal the reference's two sweeps.
+__global__ __launch_bounds__(BLOCK) void k_fixup_interp(Params P, Ptrs X, double* S, Box b) {
   if (!X.status[ST_FAILURE_EXISTS]) return;
-  if (!X.status[(which == PFLAG_INTERP_PRIM) ? ST_FAILTYPE_PRIM : ST_FAILTYPE_V]) return;
+  if (!X.status[ST_FAILTYPE_PRIM] && !X.status[ST_FAILTYPE_V]) return;
   const long long n = b.count(), stride = (long long)gridDim.x * BLOCK;
   const long long rounds = (n + stride - 1) / stride;   // the same in every workgroup (barriers)
   for (long long rd = 0; rd < rounds; rd++) {
@@ -797,8 +800,9 @@ __global__ __launch_bounds__(BLOCK) void k_fixup_interp(Params P, Ptrs X, double
     if (active) {
       int i, j, k;
       b.at(t, i, j, k);
-      fixup_interp_zone(P, X, S, which, i, j, k, fr);
       c = cidx(P, i, j, k);
+      const int which = X.pflag[c];
+      if (which == PFLAG_INTERP_V || which == PFLAG_INTERP_PRIM) fixup_interp_zone(P, X, S, which, i, j, k, fr);
     }
     commit_fail(P, X, fr, active, c, MASK_ALL);
   }
@@ -824,12 +828,14 @@ __global__ __launch_bounds__(BLOCK) void k_clear_pflag(Ptrs X, long long nt) {
   for (long long t = (long long)blockIdx.x * BLOCK + threadIdx.x; t < nt; t += stride) X.pflag[t] = 0;
 }
 
-__global__ void k_reset_failure_words(Ptrs X) {
+// the words a stage starts from: failure words and fastest speeds
+__global__ void k_reset_stage_words(Ptrs X) {
   if (threadIdx.x == 0) {
     X.status[ST_FAILURE_EXISTS] = 0;
     X.status[ST_FAILTYPE_V] = 0;
     X.status[ST_FAILTYPE_PRIM] = 0;
   }
+  if (threadIdx.x < 4) X.status[ST_VMAX + threadIdx.x] = 0;
 }
 
 __global__ void k_reset_words(Ptrs X, int first, int n) {
@@ -1433,10 +1439,6 @@ struct GpuBE {
   // launches ------------------------------------------------------------------
   static unsigned nblocks(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
-  static int reset_vmax(hgpu_ctx* c) {
-    hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_VMAX, 4);
-    return hip_check(hipGetLastError(), "k_reset_words");
-  }
   static int reset_nfailtot(hgpu_ctx* c) {
     hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_NFAILTOT, N_NFAIL);
     return hip_check(hipGetLastError(), "k_reset_words");
@@ -1505,11 +1507,11 @@ struct GpuBE {
                        mtot ? 1 : 0);
     return hip_check(hipGetLastError(), "k_bounds");
   }
-  static int fixup_interp(hgpu_ctx* c, double* S, int which) {
+  static int fixup_interp(hgpu_ctx* c, double* S) {
     Box b = interior_box(c->P);
     Scope s(c, "fixup");
     const unsigned nb = std::min(nblocks(b.count()), priv(c)->flag_grid);
-    hipLaunchKernelGGL(k_fixup_interp, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, which, b);
+    hipLaunchKernelGGL(k_fixup_interp, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, b);
     return hip_check(hipGetLastError(), "k_fixup_interp");
   }
   static int clear_boundary_flag(hgpu_ctx* c) {
@@ -1557,11 +1559,12 @@ struct GpuBE {
     }
     return rc ? rc : sync(c);
   }
-  static int clear_failures(hgpu_ctx* c) {
+  // before each stage: flag map and failure words cleared, fastest speeds reset
+  static int begin_stage(hgpu_ctx* c) {
     hipLaunchKernelGGL(k_clear_pflag, dim3(std::min(nblocks(c->P.nt), priv(c)->flag_grid)), dim3(BLOCK), 0,
                        priv(c)->stream, c->X, c->P.nt);
-    hipLaunchKernelGGL(k_reset_failure_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X);
-    return hip_check(hipGetLastError(), "k_clear_pflag");
+    hipLaunchKernelGGL(k_reset_stage_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X);
+    return hip_check(hipGetLastError(), "k_reset_stage_words");
   }
   static int timestep(hgpu_ctx* c) {
     hipLaunchKernelGGL(k_timestep, dim3(1), dim3(64), 0, priv(c)->stream, c->P, c->X);
