@@ -470,7 +470,7 @@ struct Api {
   }
 
   // The stage itself; BE::begin_stage() has cleared its failure words, flag
-  // map and fastest speeds.
+  // map, boundary flags and fastest speeds and emptied the slow-zone queue.
   static int substep(hgpu_ctx* c, int stage) {
     Ptrs& X = c->X;
     double* src = (stage == 0) ? X.p : X.ph;
@@ -490,7 +490,7 @@ struct Api {
     }
     for (int d = 0; d < 3; d++) rc |= BE::flux(c, src, d);
     rc |= BE::emf(c);
-    rc |= BE::update(c, stage);
+    rc |= BE::update(c, stage, false);   // begin_stage() emptied the slow-zone queue
     if (c->P.dyn) { X.geom = X.geom_new; X.cent_lvl = stage + 1; }
     rc |= BE::recover_queued(c, stage);
     rc |= bounds(c, dst, 0, true);
@@ -503,8 +503,7 @@ struct Api {
   static int stage_fixup(hgpu_ctx* c, int stage) {
     double* dst = (stage == 0) ? c->X.ph : c->X.p;
     int rc = BE::fixup_interp(c, dst);   // fixup_interp_v(), then fixup_interp_prim()
-    rc |= bounds(c, dst, 1);
-    rc |= BE::clear_boundary_flag(c);
+    rc |= bounds(c, dst, 1);   // the boundary flags are cleared when the next stage starts
     return rc;
   }
 

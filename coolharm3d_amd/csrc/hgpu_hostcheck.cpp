@@ -101,7 +101,7 @@ struct HostBE {
     return 0;
   }
   static int recover_queued(hgpu_ctx*, int) { return 0; }   // update() runs the whole tree
-  static int update(hgpu_ctx* c, int stage) {
+  static int update(hgpu_ctx* c, int stage, bool = true) {
     Box b = interior_box(c->P);
     const double dt = c->X.scal[SC_DT];
     double mtot = 0.;   // in the reference's loop order, k fastest (src/step_ch.c:309, :389)
@@ -148,11 +148,6 @@ struct HostBE {
     }
     return 0;
   }
-  static int clear_boundary_flag(hgpu_ctx* c) {
-    c->X.status[ST_BOUNDARY_PHYS] = 0;
-    c->X.status[ST_BOUNDARY_MPI] = 0;
-    return 0;
-  }
   static int set_boundary_flags(hgpu_ctx* c, int phys, int mpi) {
     c->X.status[ST_BOUNDARY_PHYS] = phys ? 1 : 0;
     c->X.status[ST_BOUNDARY_MPI] = mpi ? 1 : 0;
@@ -181,7 +176,8 @@ struct HostBE {
     }
     return 0;
   }
-  // before each stage: flag map and failure words cleared, fastest speeds reset
+  // before each stage: flag map, failure words and boundary flags cleared,
+  // fastest speeds reset
   static int begin_stage(hgpu_ctx* c) {
     if (c->X.status[ST_FAILURE_EXISTS]) {
       for (long long t = 0; t < c->P.nt; t++) c->X.pflag[t] = 0;
@@ -190,6 +186,8 @@ struct HostBE {
     c->X.status[ST_FAILTYPE_V] = 0;
     c->X.status[ST_FAILTYPE_PRIM] = 0;
     for (int l = 0; l < 4; l++) c->X.status[ST_VMAX + l] = 0;
+    c->X.status[ST_BOUNDARY_PHYS] = 0;
+    c->X.status[ST_BOUNDARY_MPI] = 0;
     return 0;
   }
   static int timestep(hgpu_ctx* c) { timestep_update(c->P, c->X); return 0; }

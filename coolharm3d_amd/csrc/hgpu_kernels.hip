@@ -828,12 +828,16 @@ __global__ __launch_bounds__(BLOCK) void k_clear_pflag(Ptrs X, long long nt) {
   for (long long t = (long long)blockIdx.x * BLOCK + threadIdx.x; t < nt; t += stride) X.pflag[t] = 0;
 }
 
-// the words a stage starts from: failure words and fastest speeds
+// the words a stage starts from: failure words, boundary flags, fastest
+// speeds, an empty slow-zone queue
 __global__ void k_reset_stage_words(Ptrs X) {
   if (threadIdx.x == 0) {
     X.status[ST_FAILURE_EXISTS] = 0;
     X.status[ST_FAILTYPE_V] = 0;
     X.status[ST_FAILTYPE_PRIM] = 0;
+    X.status[ST_BOUNDARY_PHYS] = 0;
+    X.status[ST_BOUNDARY_MPI] = 0;
+    X.status[ST_SLOW] = 0;
   }
   if (threadIdx.x < 4) X.status[ST_VMAX + threadIdx.x] = 0;
 }
@@ -1461,7 +1465,9 @@ struct GpuBE {
     else hipLaunchKernelGGL(k_emf, dim3(T.nblocks), dim3(T.bs), 0, priv(c)->stream, c->P, c->X, b, T);
     return hip_check(hipGetLastError(), "k_emf");
   }
-  static int update(hgpu_ctx* c, int stage) {
+  // reset_slow: empty the slow-zone queue first (the stage's update finds it
+  // emptied by begin_stage(); the autotuner's repeated updates do not)
+  static int update(hgpu_ctx* c, int stage, bool reset_slow = true) {
     if (c->P.otf) {   // the stage's source terms, into the buffer the update reads (the stage's level)
       Scope sm(c, "metric");
       const long long n = (long long)c->P.n1 * c->P.n2 * c->P.n3;
@@ -1475,7 +1481,7 @@ struct GpuBE {
     const Tiling T = make_tiling(b, priv(c)->shape[FAM_CONS]);
     const Tiling T2 = make_tiling(b, priv(c)->shape[FAM_INV]);
     if ((long long)T.nblocks * (T.bs / 64) > c->mpart_cap) return set_error(-3, "update grid exceeds the M_tot parts buffer");
-    hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_SLOW, 1);
+    if (reset_slow) hipLaunchKernelGGL(k_reset_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, (int)ST_SLOW, 1);
     hipStream_t st = priv(c)->stream;
     if (priv(c)->shape[FAM_CONS].fused || c->P.otf) {   // the evaluated metric has the fused kernel only
       launch_fused(c, priv(c)->shape[FAM_CONS], T, b, stage);
@@ -1513,10 +1519,6 @@ struct GpuBE {
     const unsigned nb = std::min(nblocks(b.count()), priv(c)->flag_grid);
     hipLaunchKernelGGL(k_fixup_interp, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, b);
     return hip_check(hipGetLastError(), "k_fixup_interp");
-  }
-  static int clear_boundary_flag(hgpu_ctx* c) {
-    hipLaunchKernelGGL(k_boundary_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, 0, 0, 0);
-    return hip_check(hipGetLastError(), "k_boundary_words");
   }
   static int set_boundary_flags(hgpu_ctx* c, int phys, int mpi) {
     hipLaunchKernelGGL(k_boundary_words, dim3(1), dim3(64), 0, priv(c)->stream, c->X, 1, phys, mpi);
@@ -1559,7 +1561,8 @@ struct GpuBE {
     }
     return rc ? rc : sync(c);
   }
-  // before each stage: flag map and failure words cleared, fastest speeds reset
+  // before each stage: flag map, failure words and boundary flags cleared,
+  // fastest speeds reset, the slow-zone queue emptied
   static int begin_stage(hgpu_ctx* c) {
     hipLaunchKernelGGL(k_clear_pflag, dim3(std::min(nblocks(c->P.nt), priv(c)->flag_grid)), dim3(BLOCK), 0,
                        priv(c)->stream, c->X, c->P.nt);

@@ -198,8 +198,9 @@ int hgpu_substep(hgpu_ctx *ctx, int stage);
  *   hgpu_fixup_stage(): with the boundary flags set to phys / mpi (the
  *                       globally reduced values): fixup_interp_v(),
  *                       fixup_interp_prim() (src/fixup.c:499-761), then the
- *                       physical part of bounds(prim,1) if phys; the flags are
- *                       then cleared.  The caller exchanges ghosts again if mpi.
+ *                       physical part of bounds(prim,1) if phys (the flags
+ *                       are cleared when the next stage starts).  The caller
+ *                       exchanges ghosts again if mpi.
  * hgpu_get_boundary_flags() reads failure_exists, boundary_phys_pflag and
  * boundary_mpi_pflag of the stage (waits for the stream). */
 int hgpu_advance(hgpu_ctx *ctx, int stage);
