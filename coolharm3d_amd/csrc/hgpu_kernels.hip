@@ -747,9 +747,12 @@ __global__ __launch_bounds__(BLOCK) void k_recover(Params P, Ptrs X, int stage, 
 }
 
 // mtot: the first boundary kernel after an update, whose first workgroup
-// completes that update's M_tot.
-__global__ __launch_bounds__(BLOCK) void k_bounds(Params P, Ptrs X, double* S, int dir, int skip_mpi, Box b,
-                                                  int mtot) {
+// completes that update's M_tot.  One kernel per direction: only x1 runs
+// inflow_check() (its geometry record and registers), so the x2 and x3 fills
+// are lean copies.
+template <int DIR>
+__global__ __launch_bounds__(BLOCK) void k_bounds(Params P, Ptrs X, double* S, int skip_mpi, Box b, int mtot) {
+  constexpr int dir = DIR;
   if (mtot && blockIdx.x == 0) mtot_level3(X);
   if (skip_mpi && !X.status[ST_BOUNDARY_PHYS]) return;
   const int do_pflag = X.status[ST_FAILURE_EXISTS] ? 1 : 0;
@@ -1509,8 +1512,12 @@ struct GpuBE {
     // bounds(prim,0) has work every time: one zone per thread; bounds(prim,1)
     // mostly finds its flag clear: a strided grid
     const unsigned nb = skip_mpi ? std::min(nblocks(b.count()), priv(c)->flag_grid) : nblocks(b.count());
-    hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, dir, skip_mpi, b,
-                       mtot ? 1 : 0);
+    if (dir == 0)
+      hipLaunchKernelGGL(k_bounds<0>, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, skip_mpi, b, mtot ? 1 : 0);
+    else if (dir == 1)
+      hipLaunchKernelGGL(k_bounds<1>, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, skip_mpi, b, mtot ? 1 : 0);
+    else
+      hipLaunchKernelGGL(k_bounds<2>, dim3(nb), dim3(BLOCK), 0, priv(c)->stream, c->P, c->X, S, skip_mpi, b, mtot ? 1 : 0);
     return hip_check(hipGetLastError(), "k_bounds");
   }
   static int fixup_interp(hgpu_ctx* c, double* S) {
