@@ -152,3 +152,24 @@ def test_launch_shapes_and_autotune_keep_results(precision):
         assert np.array_equal(out[0][1], o[1])
         assert np.array_equal(out[0][2], o[2])
         assert out[0][3] == o[3]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_normal_field_flux_not_stored(precision):
+    """The face-flux kernels store seven of the eight fluxes: the normal
+    field's flux F[d][B_d], which flux_ct() zeroes and nothing reads, stays at
+    the zero the arrays start from, while the transverse field fluxes the EMF
+    kernel reads are written."""
+    from coolharm3d_amd import Harm3D, torus_grid, torus
+    g = torus_grid(24, 24, 16)
+    H = Harm3D(g, precision=precision)
+    H.set_prims(torus(g))
+    H.fixup_bounds()
+    H.set_time(0.0, 1.0e-5)
+    H.step_ch(2)
+    F = H.fluxes()
+    for d in range(3):
+        assert not F[d, 5 + d].any(), f"F[{d}][B{d + 1}]"
+        others = [5 + e for e in range(3) if e != d]
+        assert np.abs(F[d, others]).max() > 0
+        assert np.abs(F[d, :5]).max() > 0
